@@ -1,0 +1,271 @@
+/*
+ * xr_api.h - C ABI of the MI355X (gfx950) path-tracing / ReSTIR GI layer.
+ *
+ * This library is a drop-in for the ray-tracing half of the reference's
+ * frame: PathTracer::Render (src/pathTracer.cpp:82-116) calls
+ *   case 1: ApplicationRT::Render        (src/Vulkan/RayTracing/ApplicationRT.cpp:117-183)
+ *   case 2: ApplicationReSTIRGI::Render  (src/Vulkan/RayTracing/ApplicationReSTIRGI.cpp:150-275)
+ * and both are replaced by xr_render().  The resources those classes create
+ * (acceleration structures, ApplicationRT.cpp:55-66; ReSTIR buffers and
+ * G-buffer images, ApplicationReSTIRGI.cpp:304-374) are owned by an xr_ctx.
+ * The operator contract is the reference's 19-binding descriptor table
+ * (src/Vulkan/RayTracing/ReSTIRGIPipeline.cpp:40-74); every binding maps to a
+ * buffer id below.
+ *
+ * Conventions: plain C, opaque context, int status (XR_OK = 0), no call
+ * throws or aborts, one context is not thread safe, each context owns one
+ * HIP stream, xr_render is asynchronous and xr_sync waits for it.  Host
+ * pointers unless a function says "device".
+ */
+#ifndef XR_API_H
+#define XR_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws std::runtime_error, XVKCommon.h:10-16) ---- */
+#define XR_OK 0
+#define XR_ERR_INVALID_ARGUMENT 1 /* null pointer, bad size, bad enum */
+#define XR_ERR_DEVICE 2           /* HIP runtime error (no device, launch failure) */
+#define XR_ERR_OUT_OF_MEMORY 3
+#define XR_ERR_STATE 4            /* call out of order, e.g. render before xr_build_accel */
+#define XR_ERR_UNSUPPORTED 5      /* feature outside this layer (textures, non-identity instances) */
+
+typedef struct xr_ctx xr_ctx;
+
+/* AVertex (src/Asset/Vertex.h:11-16), 36 bytes, read as 9 dwords by Vertex.hlsl:12-33. */
+typedef struct xr_vertex {
+    float pos[3];
+    float normal[3];
+    float tex_coord[2];
+    int32_t material_index;
+} xr_vertex;
+
+/* AMaterial (src/Asset/Material.h:46-57), 32 bytes. */
+#define XR_MATERIAL_LAMBERTIAN 0u
+#define XR_MATERIAL_METALLIC 1u
+#define XR_MATERIAL_DIELECTRIC 2u
+#define XR_MATERIAL_ISOTROPIC 3u
+#define XR_MATERIAL_DIFFUSE_LIGHT 4u
+typedef struct xr_material {
+    float base_color[4];
+    int32_t base_color_texture_index; /* must be -1: texture sampling is not in this layer */
+    float fuzziness;
+    float refraction_index;
+    uint32_t material_model;
+} xr_material;
+
+/* UniformBufferObject (src/Asset/UniformBuffer.h:16-32), 412 bytes, matrices
+ * column-major as glm stores them; built by PathTracer::GetUniformBufferObject
+ * (src/pathTracer.cpp:169-201). */
+typedef struct xr_ubo {
+    float view[16];
+    float view_prev[16];
+    float projection[16];
+    float projection_prev[16];
+    float view_inverse[16];
+    float projection_inverse[16];
+    uint32_t spp;
+    uint32_t total_number_of_samples;
+    uint32_t number_of_bounces;
+    uint32_t random_seed;
+    uint32_t has_sky_box;
+    uint32_t reuse;
+    uint32_t current_frame;
+} xr_ubo;
+
+/* ReSTIRGISample / ReSTIRGIReservoir (src/Vulkan/RayTracing/ReSTIRGICommon.h:8-32):
+ * 96 and 112 bytes.  The reference-layout form of xr_read_buffer/xr_write_buffer. */
+typedef struct xr_restir_sample {
+    float x_view[3];
+    uint32_t material_id;
+    float n_view[3];
+    uint32_t rand_seed;
+    float x_sample[3];
+    float p_q;
+    float n_sample[3];
+    uint32_t pad1;
+    float l_out[3];
+    uint32_t pad2;
+    float f[3];
+    uint32_t pad3;
+} xr_restir_sample;
+
+typedef struct xr_restir_reservoir {
+    float w_sum;
+    float w;
+    uint32_t m;
+    float pad;
+    xr_restir_sample s;
+} xr_restir_reservoir;
+
+/* The scene as AScene concatenates it (src/Asset/Scene.cpp:16-87): one
+ * vertex, index and material buffer, plus (indexOffset, vertexOffset) per
+ * instance (Scene.cpp:37).  Instance i owns the triangles of index range
+ * [offsets[2i], next instance's offsets[2i] or index_count), which must be
+ * increasing.  Instances have identity transforms, as the reference's TLAS
+ * (XVKTLAccelerationStructure.cpp:68-93).  All arrays are copied to the device
+ * by xr_create. */
+typedef struct xr_scene_desc {
+    const xr_vertex* vertices;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count;
+    const xr_material* materials;
+    uint32_t material_count;
+    const uint32_t* offsets; /* 2 x uint32 per instance */
+    uint32_t instance_count;
+} xr_scene_desc;
+
+/* ---- render modes (UserSettings::renderingMode, src/pathTracer.cpp:93-108) ---- */
+#define XR_MODE_PLAIN 1  /* ApplicationRT::Render: rgen.hlsl */
+#define XR_MODE_RESTIR 2 /* ApplicationReSTIRGI::Render: initial -> temporal -> spatial */
+
+/* visibility test of the spatial pass */
+#define XR_VIS_REFERENCE 0 /* CheckVisibility as the reference binds it: always visible (Spatial.rgen:44-62) */
+#define XR_VIS_TRACE 1     /* trace the any-hit shadow ray */
+
+/* flags: which passes of mode 2 run (all when none is set) */
+#define XR_PASS_INITIAL 0x1u
+#define XR_PASS_TEMPORAL 0x2u
+#define XR_PASS_SPATIAL 0x4u
+
+typedef struct xr_restir_params {
+    uint32_t temporal_m_cap;   /* M_MAX of ReSTIRGITemporalReuse.rgen.hlsl:3 (reference 30) */
+    uint32_t spatial_m_cap;    /* M_MAX of ReSTIRGISpatialReuse.rgen.hlsl:6 (reference 300) */
+    uint32_t spatial_samples;  /* NUM_SPATIAL_SAMPLES, Spatial.rgen:4 (reference 6) */
+    float spatial_radius;      /* SPATIAL_RADIUS in pixels, Spatial.rgen:5 (reference 5.0) */
+    uint32_t visibility;       /* XR_VIS_* (reference XR_VIS_REFERENCE) */
+    uint32_t plain_max_depth;  /* mode 1 bounce cap, 0 = unbounded as rgen.hlsl:59 */
+    uint32_t passes;           /* XR_PASS_* mask for mode 2, 0 = all three */
+    uint32_t reserved[9];      /* must be zero */
+} xr_restir_params;
+
+/* Fills p with the reference's compile-time constants (30 / 300 / 6 / 5.0,
+ * always-visible, unbounded plain depth, all passes). */
+void xr_default_params(xr_restir_params* p);
+
+/* ---- lifecycle ---- */
+
+/* Copies the scene to device 0 and allocates the frame buffers for a
+ * width x height target (InitReSTIRGITracer, ApplicationReSTIRGI.cpp:304-374).
+ * On failure *out is NULL and xr_global_last_error() says why. */
+int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ctx** out);
+
+/* Builds the acceleration structure (CreateAccelerationStructures,
+ * ApplicationRT.cpp:55-66).  builder: XR_BUILDER_LBVH. */
+#define XR_BUILDER_LBVH 0
+int xr_build_accel(xr_ctx* ctx, int builder);
+
+/* Reallocates the frame buffers (swapchain recreation,
+ * ApplicationReSTIRGI.cpp:115-148); reservoir history is cleared. */
+int xr_resize(xr_ctx* ctx, uint32_t width, uint32_t height);
+
+/* Records one frame on the context's stream.  mode: XR_MODE_*; params may be
+ * NULL (reference defaults). */
+int xr_render(xr_ctx* ctx, const xr_ubo* ubo, int mode, const xr_restir_params* params);
+
+/* Waits for every call queued on the context. */
+int xr_sync(xr_ctx* ctx);
+
+/* Destroys the context and frees its device memory.  NULL is allowed. */
+void xr_destroy(xr_ctx* ctx);
+
+/* ---- read back / inject (parity dumps, fixtures) ---- */
+
+/* buffer ids = reference bindings */
+#define XR_BUF_ACCUMULATION 1 /* binding 1, RGBA32F, 16 B/px */
+#define XR_BUF_OUTPUT 2       /* binding 2, B8G8R8A8_UNORM, 4 B/px */
+#define XR_BUF_SAMPLES 9      /* binding 9, ReSTIRGISample, N entries */
+#define XR_BUF_TEMPORAL 10    /* binding 10, ReSTIRGIReservoir, 2N entries (two ping-pong slots) */
+#define XR_BUF_SPATIAL 11     /* binding 11, ReSTIRGIReservoir, 2N entries */
+#define XR_BUF_GBUF_POS 12    /* binding 12 (= 16), RGBA32F */
+#define XR_BUF_GBUF_NRM 13    /* binding 13 (= 17), RGBA32F */
+#define XR_BUF_GBUF_MATUV 14  /* binding 14, RGBA32F */
+#define XR_BUF_MOTION 15      /* binding 15, RG16F, 4 B/px */
+
+#define XR_LAYOUT_REFERENCE 0 /* AoS exactly as the reference binds it (96 / 112-byte structs) */
+#define XR_LAYOUT_NATIVE 1    /* as stored on the device: six float4 planes per entry */
+
+/* bytes must equal the buffer's size in the chosen layout.  Synchronous. */
+int xr_read_buffer(xr_ctx* ctx, int which, void* dst, size_t bytes, int layout);
+int xr_write_buffer(xr_ctx* ctx, int which, const void* src, size_t bytes, int layout);
+/* Shorthand for xr_read_buffer(ctx, XR_BUF_OUTPUT, ...). */
+int xr_read_image(xr_ctx* ctx, uint32_t* dst_bgra8, size_t bytes);
+/* Size in bytes of a buffer in a layout (0 for an unknown id). */
+size_t xr_buffer_bytes(const xr_ctx* ctx, int which, int layout);
+
+/* ---- ray batches (TraceRay with the reference's flags) ---- */
+
+typedef struct xr_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} xr_ray;
+
+typedef struct xr_hit {
+    float t; /* -1 on miss (Ray.rmiss.hlsl:6-7) */
+    float u; /* hit-attribute barycentrics: weight of v1 */
+    float v; /* weight of v2 */
+    uint32_t pad0;
+    uint32_t instance;  /* InstanceIndex(), 0xFFFFFFFF on miss */
+    uint32_t primitive; /* PrimitiveIndex() */
+    uint32_t triangle;  /* canonical id = first index of the triangle / 3 */
+    uint32_t pad1;
+} xr_hit;
+
+/* any_hit = 0: closest hit (RAY_FLAG_NONE, InitialSample.rgen:90-98).
+ * any_hit = 1: first hit found (ACCEPT_FIRST_HIT_AND_END_SEARCH, Spatial.rgen:58); t/u/v/ids of
+ * some hit in [tmin, tmax].  Host arrays, synchronous. */
+int xr_trace_rays(xr_ctx* ctx, const xr_ray* rays, uint32_t n, xr_hit* hits, int any_hit);
+/* Same on device arrays, queued on the context's stream (no copies). */
+int xr_trace_rays_device(xr_ctx* ctx, const xr_ray* d_rays, uint32_t n, xr_hit* d_hits, int any_hit);
+
+/* ---- measurement ---- */
+
+/* Milliseconds of the last xr_render's kernels (HIP events on the context's
+ * stream), in pass order: mode 2 -> {initial, temporal, spatial}, mode 1 ->
+ * {plain}.  Returns the number written.  Waits for the frame. */
+int xr_pass_times(xr_ctx* ctx, float* ms, uint32_t n);
+
+/* Device counters since xr_create / the last reset: [0] extend rays
+ * (camera + bounce), [1] shadow rays.  Waits for queued work. */
+int xr_counters(xr_ctx* ctx, uint64_t* out, uint32_t n);
+int xr_reset_counters(xr_ctx* ctx);
+
+typedef struct xr_accel_info {
+    uint32_t triangle_count;
+    uint32_t node_count;
+    uint32_t max_depth;
+    uint32_t builder;
+    float build_ms;
+    uint64_t node_bytes;
+    uint64_t triangle_bytes;
+} xr_accel_info;
+int xr_get_accel_info(const xr_ctx* ctx, xr_accel_info* info);
+
+/* Copies the acceleration structure to host memory for validation: node_count
+ * 64-byte nodes {float4 a, b, c; int4 d} with a = (c0.lo.x, c0.hi.x, c0.lo.y,
+ * c0.hi.y), b = the same for child 1, c = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z),
+ * d = (link0, link1, 0, 0) where a link >= 0 names a node and a link < 0 a
+ * leaf ~(first * 16 + count - 1); and triangle_count 48-byte BVH-ordered
+ * triangles {v0.xyz, canonical id; v1.xyz, instance; v2.xyz, primitive}. */
+int xr_read_accel(xr_ctx* ctx, void* nodes, size_t node_bytes, void* triangles, size_t triangle_bytes);
+
+/* HIP stream the context records on (a hipStream_t). */
+void* xr_stream(xr_ctx* ctx);
+
+/* ---- errors ---- */
+const char* xr_last_error(const xr_ctx* ctx);
+/* Error of the last failed call that had no context (xr_create). Thread local. */
+const char* xr_global_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XR_API_H */

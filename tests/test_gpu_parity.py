@@ -1,0 +1,208 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+The kernels and the oracle evaluate the same FP32 expressions in the same
+order (no contraction, correctly rounded division and sqrt, the same cos/sin
+polynomial), so traversal results, samples, G-buffers and reservoirs are
+compared bit for bit.  The tone-mapped BGRA8 image goes through powf, whose
+last bit may differ between the device library and glibc, so it is compared
+to 1 LSB per channel.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import (CORNELL_EYE, CORNELL_TARGET, bytes_equal_rows, cornell_scene, describe_mismatch,
+                           look_view, oracle_scene, random_rays, rays_f64, say, small_terrain)
+from xvk_ray_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _max_lsb_diff(a: np.ndarray, b: np.ndarray) -> int:
+    a8 = a.view(np.uint8).astype(np.int32)
+    b8 = b.view(np.uint8).astype(np.int32)
+    return int(np.abs(a8 - b8).max()) if len(a8) else 0
+
+
+# ---------------------------------------------------------------------------
+# TraceRay
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("which", ["cornell", "terrain"])
+def test_trace_closest_bit_exact(which):
+    scene = cornell_scene() if which == "cornell" else small_terrain()
+    say(f"[trace] {which}: {scene.triangle_count} tris")
+    ctx = N.Context(scene, 8, 8)
+    orc = oracle_scene(scene)
+    lo, hi = ((0.02, 0.98) if which == "cornell" else (0.0, 0.5))
+    rays = random_rays(20000, lo, hi, seed=7)
+    gpu = ctx.trace(rays)
+    ref = orc.trace32(rays)
+    eq = bytes_equal_rows(gpu, ref)
+    assert eq.all(), describe_mismatch("closest hit", gpu, ref, eq)
+    assert (gpu["t"] > 0).mean() > 0.3  # the batch actually hits geometry
+
+
+def test_trace_matches_fp64_judge():
+    """North-star criterion: >= 99.99% identical closest-hit ids vs an FP64
+    search; every mismatch is a distance tie within 1e-6 * t."""
+    scene = small_terrain(grid=256, spheres=64, lights=16)
+    ctx = N.Context(scene, 8, 8)
+    orc = oracle_scene(scene)
+    rng = np.random.default_rng(3)
+    n = 50000
+    rays = np.zeros(n, N.RAY_DTYPE)
+    rays["origin"] = np.stack([rng.uniform(0, 1, n), rng.uniform(0.3, 0.6, n), rng.uniform(0, 1, n)], 1)
+    d = np.stack([rng.normal(size=n), -np.abs(rng.normal(size=n)) - 0.2, rng.normal(size=n)], 1)
+    rays["direction"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"] = 1e-4
+    rays["tmax"] = 1e4
+    say(f"[trace-fp64] {scene.triangle_count} tris, {n} rays")
+    gpu = ctx.trace(rays)
+    ids, ts = orc.trace64(rays_f64(rays))
+    same = gpu["triangle"] == ids
+    assert same.mean() >= 0.9999, f"id match {same.mean():.6f}"
+    bad = ~same
+    if bad.any():
+        t_gpu = gpu["t"][bad].astype(np.float64)
+        t_ref = ts[bad]
+        assert np.all(np.abs(t_gpu - t_ref) <= 1e-6 * np.maximum(np.abs(t_ref), 1e-3) + 1e-7), \
+            list(zip(t_gpu[:5], t_ref[:5]))
+
+
+def test_trace_any_hit_matches():
+    scene = small_terrain()
+    ctx = N.Context(scene, 8, 8)
+    orc = oracle_scene(scene)
+    rays = random_rays(20000, 0.0, 0.6, seed=11, tmin=0.0, tmax=0.2)
+    gpu = ctx.trace(rays, any_hit=True)
+    ref = orc.trace32(rays, any_hit=True)
+    assert np.array_equal(gpu["t"] >= 0, ref["t"] >= 0)
+
+
+# ---------------------------------------------------------------------------
+# ReSTIR GI frame, pass by pass
+# ---------------------------------------------------------------------------
+def _frames(scene, W, H, fov, eyes, target, bounces, params, check, sky=0):
+    """Render frames pass by pass on the GPU; after each pass compare the
+    pass's outputs with the oracle run on the GPU's own inputs."""
+    from oracle.binding import temporal as orc_temporal
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    views = [look_view(e, target) for e in eyes]
+    for k, view in enumerate(views):
+        frame = k + 1
+        ubo = N.make_ubo(view, views[k - 1] if k else view, fov, W, H, spp=1, total=1, bounces=bounces, sky=sky,
+                         reuse=1, frame=frame)
+        ub = N.ubo_to_bytes(ubo)
+        say(f"[restir] frame {frame} {W}x{H} bounces={bounces} vis={params.visibility}")
+        # initial sample pass
+        p = N.XrRestirParams.from_buffer_copy(params)
+        p.passes = N.XR_PASS_INITIAL
+        ctx.render(ubo, N.XR_MODE_RESTIR, p)
+        ctx.sync()
+        ref = orc.initial_sample(ub, W, H)
+        for name, which, key in [("samples", N.XR_BUF_SAMPLES, "samples"), ("gpos", N.XR_BUF_GBUF_POS, "gpos"),
+                                 ("gnrm", N.XR_BUF_GBUF_NRM, "gnrm"), ("gmatuv", N.XR_BUF_GBUF_MATUV, "gmatuv"),
+                                 ("motion", N.XR_BUF_MOTION, "motion")]:
+            g = ctx.read(which)
+            eq = bytes_equal_rows(g, ref[key])
+            check(eq, name, g, ref[key])
+        # temporal pass on the GPU's state
+        samples = ctx.read(N.XR_BUF_SAMPLES)
+        motion = ctx.read(N.XR_BUF_MOTION)
+        gpos = np.ascontiguousarray(ctx.read(N.XR_BUF_GBUF_POS))
+        temporal = ctx.read(N.XR_BUF_TEMPORAL).copy()
+        spatial = ctx.read(N.XR_BUF_SPATIAL).copy()
+        output = ctx.read(N.XR_BUF_OUTPUT).copy()
+        p.passes = N.XR_PASS_TEMPORAL
+        ctx.render(ubo, N.XR_MODE_RESTIR, p)
+        ctx.sync()
+        orc_temporal(W, H, frame, params.temporal_m_cap, samples, motion, gpos, temporal, spatial, output)
+        gt = ctx.read(N.XR_BUF_TEMPORAL)
+        check(bytes_equal_rows(gt, temporal), "temporal reservoirs", gt, temporal)
+        gs = ctx.read(N.XR_BUF_SPATIAL)
+        check(bytes_equal_rows(gs, spatial), "spatial reservoirs (temporal pass)", gs, spatial)
+        go = ctx.read(N.XR_BUF_OUTPUT)
+        assert _max_lsb_diff(go, output) <= 1, "temporal output"
+        # spatial pass
+        output = go.copy()
+        p.passes = N.XR_PASS_SPATIAL
+        ctx.render(ubo, N.XR_MODE_RESTIR, p)
+        ctx.sync()
+        orc.spatial(W, H, frame, 1, params.spatial_samples, params.spatial_radius, params.spatial_m_cap,
+                    params.visibility, samples, gt, spatial, output)
+        gs = ctx.read(N.XR_BUF_SPATIAL)
+        check(bytes_equal_rows(gs, spatial), "spatial reservoirs", gs, spatial)
+        go = ctx.read(N.XR_BUF_OUTPUT)
+        assert _max_lsb_diff(go, output) <= 1, "spatial output"
+    return ctx
+
+
+def _strict(eq, name, g, r):
+    assert eq.all(), describe_mismatch(name, g, r, eq)
+
+
+@pytest.mark.parametrize("bounces", [4, 10])
+def test_restir_cornell_reference_mode(bounces):
+    scene = cornell_scene()
+    eyes = [CORNELL_EYE, (0.52, 0.5, -1.2), (0.54, 0.51, -1.19)]
+    params = N.default_params()
+    _frames(scene, 96, 64, 60.0, eyes, CORNELL_TARGET, bounces, params, _strict)
+
+
+def test_restir_cornell_traced_visibility_config_constants():
+    scene = cornell_scene()
+    eyes = [CORNELL_EYE, (0.51, 0.5, -1.2)]
+    params = N.default_params(temporal_m_cap=20, spatial_samples=5, spatial_radius=30.0, visibility=N.XR_VIS_TRACE)
+    _frames(scene, 80, 60, 60.0, eyes, CORNELL_TARGET, 4, params, _strict)
+
+
+def test_restir_terrain_sky():
+    scene = small_terrain()
+    eyes = [(0.2, 0.4, 0.2), (0.21, 0.4, 0.2)]
+    params = N.default_params(visibility=N.XR_VIS_TRACE)
+    _frames(scene, 64, 48, 45.0, eyes, (0.5, 0.0, 0.5), 4, params, _strict, sky=1)
+
+
+# ---------------------------------------------------------------------------
+# Plain path tracer
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("max_depth", [3, 0])
+def test_plain_accumulation(max_depth):
+    scene = cornell_scene()
+    W, H = 80, 48
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    accum = np.zeros((W * H, 4), np.float32)
+    out = np.zeros(W * H, np.uint32)
+    params = N.default_params(plain_max_depth=max_depth)
+    for frame in range(1, 4):
+        ubo = N.make_ubo(view, view, 60.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
+        say(f"[plain] frame {frame} depth={max_depth}")
+        ctx.render(ubo, N.XR_MODE_PLAIN, params)
+        ctx.sync()
+        orc.plain(N.ubo_to_bytes(ubo), W, H, max_depth, accum, out)
+        ga = ctx.read(N.XR_BUF_ACCUMULATION)
+        eq = bytes_equal_rows(ga, accum)
+        assert eq.all(), describe_mismatch("accumulation", ga, accum, eq)
+        go = ctx.read(N.XR_BUF_OUTPUT)
+        assert _max_lsb_diff(go, out) <= 1
+    assert accum[:, :3].max() > 0  # the light is visible / reached
+
+
+def test_rays_counted():
+    scene = cornell_scene()
+    W, H = 64, 64
+    ctx = N.Context(scene, W, H)
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    ubo = N.make_ubo(view, view, 60.0, W, H, bounces=4, frame=1)
+    ctx.reset_counters()
+    ctx.render(ubo, N.XR_MODE_RESTIR, N.default_params(visibility=N.XR_VIS_TRACE))
+    ctx.sync()
+    ext, shadow = ctx.counters()
+    assert W * H <= ext <= 4 * W * H
+    assert 0 < shadow <= 6 * W * H
+    t = ctx.pass_times()
+    assert len(t) == 3 and all(x >= 0 for x in t)

@@ -1,0 +1,659 @@
+// C ABI of the HIP layer (include/xr_api.h).
+//
+// Owns what the reference's ApplicationRT / ApplicationReSTIRGI own: the
+// scene buffers (AScene, src/Asset/Scene.cpp:16-87), the acceleration
+// structure (ApplicationRT.cpp:55-66) and the per-resolution frame resources
+// (InitReSTIRGITracer, ApplicationReSTIRGI.cpp:304-374), and records the
+// frame that ApplicationReSTIRGI::Render records (:150-275): initial sample,
+// temporal reuse, spatial reuse, in order on one stream, each pass ordered
+// after the previous one by the stream (the reference's memory barriers,
+// :180-215).
+//
+// Deliberate differences from the reference frame, none observable in the
+// image or in the reservoirs:
+//  * the end-of-frame copy samples -> old samples (:272-274) is not made; the
+//    reference binds the old buffer as the sample buffer and never reads the
+//    copy before the next initial pass overwrites it (ReSTIRGIPipeline.cpp:141-143);
+//  * the "prev" G-buffer bindings 16/17 alias bindings 12/13 in the reference
+//    (ReSTIRGIPipeline.cpp:171-199, ApplicationReSTIRGI.cpp:367-372), so they
+//    are one buffer here and the self-copy of InitialSample.rgen:193-194 is
+//    dropped;
+//  * buffers start zeroed (Vulkan leaves them undefined).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/xr_api.h"
+#include "xr_kernels.h"
+
+using namespace xr;
+
+static_assert(sizeof(xr_vertex) == sizeof(AVertex), "xr_vertex layout");
+static_assert(sizeof(xr_material) == sizeof(AMaterial), "xr_material layout");
+static_assert(sizeof(xr_ubo) == sizeof(UniformBufferObject), "xr_ubo layout");
+static_assert(sizeof(xr_restir_sample) == sizeof(ReSTIRGISample), "xr_restir_sample layout");
+static_assert(sizeof(xr_restir_reservoir) == sizeof(ReSTIRGIReservoir), "xr_restir_reservoir layout");
+static_assert(sizeof(xr_ray) == 32 && sizeof(xr_hit) == 32, "ray / hit records are two float4");
+
+struct xr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // scene
+    float* d_vertices = nullptr;
+    uint32_t* d_indices = nullptr;
+    float4* d_materials = nullptr;
+    uint2* d_offsets = nullptr;
+    uint32_t* d_inst_first = nullptr;
+    uint32_t vertex_count = 0, index_count = 0, material_count = 0, instance_count = 0, tri_count = 0;
+
+    // acceleration structure
+    BvhNode* d_nodes = nullptr;
+    BvhTri* d_tris = nullptr;
+    bool built = false;
+    xr_accel_info accel{};
+
+    // frame resources
+    uint32_t width = 0, height = 0;
+    float4* d_samples = nullptr;
+    float4* d_temporal = nullptr;
+    float4* d_spatial = nullptr;
+    float4* d_gpos = nullptr;
+    float4* d_gnrm = nullptr;
+    float4* d_gmatuv = nullptr;
+    float4* d_accum = nullptr;
+    uint32_t* d_motion = nullptr;
+    uint32_t* d_output = nullptr;
+    unsigned long long* d_counters = nullptr;
+
+    // timing of the last frame
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_pass_events = 0;
+
+    // staging for ray batches
+    float4* d_rays = nullptr;
+    float4* d_hits = nullptr;
+    size_t ray_cap = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(xr_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_err = buf;
+    return code;
+}
+
+int hip_code(hipError_t e) { return e == hipErrorOutOfMemory ? XR_ERR_OUT_OF_MEMORY : XR_ERR_DEVICE; }
+
+#define XR_HIP(ctx, expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return set_err((ctx), hip_code(e_), "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+bool fault_inject(const char* what) {
+    const char* v = getenv("XR_FAULT_INJECT");
+    return v && strstr(v, what) != nullptr;
+}
+
+// Device allocation with optional injected failure (XR_FAULT_INJECT=alloc).
+template <typename T>
+hipError_t dev_alloc(T** p, size_t bytes) {
+    *p = nullptr;
+    if (fault_inject("alloc")) return hipErrorOutOfMemory;
+    if (bytes == 0) bytes = 16;
+    return hipMalloc((void**)p, bytes);
+}
+
+template <typename T>
+void dev_free(T** p) {
+    if (*p) hipFree((void*)*p);
+    *p = nullptr;
+}
+
+void free_frame(xr_ctx* c) {
+    dev_free(&c->d_samples);
+    dev_free(&c->d_temporal);
+    dev_free(&c->d_spatial);
+    dev_free(&c->d_gpos);
+    dev_free(&c->d_gnrm);
+    dev_free(&c->d_gmatuv);
+    dev_free(&c->d_accum);
+    dev_free(&c->d_motion);
+    dev_free(&c->d_output);
+    c->width = c->height = 0;
+}
+
+int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
+    free_frame(c);
+    const size_t n = (size_t)w * h;
+    const size_t f4 = sizeof(float4);
+    XR_HIP(c, dev_alloc(&c->d_samples, kPlanes * n * f4));
+    XR_HIP(c, dev_alloc(&c->d_temporal, 2 * kPlanes * n * f4));
+    XR_HIP(c, dev_alloc(&c->d_spatial, 2 * kPlanes * n * f4));
+    XR_HIP(c, dev_alloc(&c->d_gpos, n * f4));
+    XR_HIP(c, dev_alloc(&c->d_gnrm, n * f4));
+    XR_HIP(c, dev_alloc(&c->d_gmatuv, n * f4));
+    XR_HIP(c, dev_alloc(&c->d_accum, n * f4));
+    XR_HIP(c, dev_alloc(&c->d_motion, n * 4));
+    XR_HIP(c, dev_alloc(&c->d_output, n * 4));
+    XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_temporal, 0, 2 * kPlanes * n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_spatial, 0, 2 * kPlanes * n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_gpos, 0, n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_gnrm, 0, n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_gmatuv, 0, n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_accum, 0, n * f4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_motion, 0, n * 4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_output, 0, n * 4, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    c->width = w;
+    c->height = h;
+    return XR_OK;
+}
+
+SceneDev scene_dev(const xr_ctx* c) {
+    SceneDev s;
+    s.vertices = c->d_vertices;
+    s.indices = c->d_indices;
+    s.materials = c->d_materials;
+    s.offsets = c->d_offsets;
+    s.nodes = c->d_nodes;
+    s.tris = c->d_tris;
+    s.material_count = c->material_count;
+    return s;
+}
+
+FrameDev frame_dev(const xr_ctx* c) {
+    FrameDev f;
+    f.width = c->width;
+    f.height = c->height;
+    f.n = c->width * c->height;
+    f.samples = c->d_samples;
+    f.temporal = c->d_temporal;
+    f.spatial = c->d_spatial;
+    f.gpos = c->d_gpos;
+    f.gnrm = c->d_gnrm;
+    f.gmatuv = c->d_gmatuv;
+    f.motion = c->d_motion;
+    f.output = c->d_output;
+    f.accum = c->d_accum;
+    f.counters = c->d_counters;
+    return f;
+}
+
+int check_params(xr_ctx* c, const xr_restir_params* p) {
+    for (int i = 0; i < 9; ++i)
+        if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
+    if (p->visibility > XR_VIS_TRACE) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.visibility %u", p->visibility);
+    if (p->spatial_samples > 64) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.spatial_samples > 64");
+    if (!(p->spatial_radius >= 0.0f && p->spatial_radius <= 4096.0f))
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.spatial_radius out of [0, 4096]");
+    if (p->passes & ~7u) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.passes has unknown bits");
+    return XR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void xr_default_params(xr_restir_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->temporal_m_cap = 30;
+    p->spatial_m_cap = 300;
+    p->spatial_samples = 6;
+    p->spatial_radius = 5.0f;
+    p->visibility = XR_VIS_REFERENCE;
+    p->plain_max_depth = 0;
+    p->passes = 0;
+}
+
+const char* xr_global_last_error(void) { return g_err.c_str(); }
+
+const char* xr_last_error(const xr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+void* xr_stream(xr_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+void xr_destroy(xr_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_frame(c);
+    dev_free(&c->d_vertices);
+    dev_free(&c->d_indices);
+    dev_free(&c->d_materials);
+    dev_free(&c->d_offsets);
+    dev_free(&c->d_inst_first);
+    dev_free(&c->d_nodes);
+    dev_free(&c->d_tris);
+    dev_free(&c->d_counters);
+    dev_free(&c->d_rays);
+    dev_free(&c->d_hits);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** out) {
+    if (!out) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!s) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 28))
+        return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "bad resolution %ux%u", width, height);
+    if (!s->vertices || !s->indices || !s->materials || !s->offsets || s->vertex_count == 0 ||
+        s->index_count == 0 || s->material_count == 0 || s->instance_count == 0)
+        return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "scene arrays must be non-empty");
+    if (s->index_count % 3 != 0) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "index_count %% 3 != 0");
+    if (s->index_count / 3 >= (1u << 27))
+        return set_err(nullptr, XR_ERR_UNSUPPORTED, "more than 2^27 triangles");
+    for (uint32_t m = 0; m < s->material_count; ++m) {
+        if (s->materials[m].base_color_texture_index >= 0)
+            return set_err(nullptr, XR_ERR_UNSUPPORTED, "material %u is textured; texture sampling is not supported", m);
+    }
+    for (uint32_t v = 0; v < s->vertex_count; ++v) {
+        const int32_t mi = s->vertices[v].material_index;
+        if (mi != -1 && (mi < 0 || (uint32_t)mi >= s->material_count))
+            return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "vertex %u material %d out of range", v, mi);
+    }
+    // instance ranges and index bounds
+    std::vector<uint32_t> first(s->instance_count + 1);
+    for (uint32_t i = 0; i < s->instance_count; ++i) {
+        const uint32_t io = s->offsets[2 * i], vo = s->offsets[2 * i + 1];
+        const uint32_t end = (i + 1 < s->instance_count) ? s->offsets[2 * (i + 1)] : s->index_count;
+        if (io % 3 != 0 || end < io || end > s->index_count || (end - io) % 3 != 0)
+            return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "instance %u index range [%u,%u) invalid", i, io, end);
+        if (i == 0 && io != 0) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "instance 0 must start at index 0");
+        for (uint32_t k = io; k < end; ++k) {
+            if ((uint64_t)vo + s->indices[k] >= s->vertex_count)
+                return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "index %u of instance %u out of range", k, i);
+        }
+        first[i] = io / 3;
+    }
+    first[s->instance_count] = s->index_count / 3;
+    for (uint32_t i = 0; i < s->instance_count; ++i)
+        if (first[i + 1] == first[i])
+            return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "instance %u has no triangles", i);
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return set_err(nullptr, XR_ERR_DEVICE, "no HIP device: %s", e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+
+    xr_ctx* c = new (std::nothrow) xr_ctx();
+    if (!c) return set_err(nullptr, XR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    auto fail = [&](int code) {
+        g_err = c->err;
+        xr_destroy(c);
+        return code;
+    };
+    int rc;
+#define XR_CREATE_HIP(expr)                                                                                  \
+    do {                                                                                                     \
+        hipError_t e2 = (expr);                                                                              \
+        if (e2 != hipSuccess) return fail(set_err(c, hip_code(e2), "%s: %s", #expr, hipGetErrorString(e2))); \
+    } while (0)
+    XR_CREATE_HIP(hipSetDevice(0));
+    XR_CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& ev : c->ev) XR_CREATE_HIP(hipEventCreate(&ev));
+    c->vertex_count = s->vertex_count;
+    c->index_count = s->index_count;
+    c->material_count = s->material_count;
+    c->instance_count = s->instance_count;
+    c->tri_count = s->index_count / 3;
+    XR_CREATE_HIP(dev_alloc(&c->d_vertices, sizeof(xr_vertex) * (size_t)s->vertex_count));
+    XR_CREATE_HIP(dev_alloc(&c->d_indices, sizeof(uint32_t) * (size_t)s->index_count));
+    XR_CREATE_HIP(dev_alloc(&c->d_materials, sizeof(xr_material) * (size_t)s->material_count));
+    XR_CREATE_HIP(dev_alloc(&c->d_offsets, sizeof(uint32_t) * 2 * (size_t)s->instance_count));
+    XR_CREATE_HIP(dev_alloc(&c->d_inst_first, sizeof(uint32_t) * first.size()));
+    XR_CREATE_HIP(dev_alloc(&c->d_counters, sizeof(unsigned long long) * 4));
+    XR_CREATE_HIP(hipMemcpyAsync(c->d_vertices, s->vertices, sizeof(xr_vertex) * (size_t)s->vertex_count,
+                                 hipMemcpyHostToDevice, c->stream));
+    XR_CREATE_HIP(hipMemcpyAsync(c->d_indices, s->indices, sizeof(uint32_t) * (size_t)s->index_count,
+                                 hipMemcpyHostToDevice, c->stream));
+    XR_CREATE_HIP(hipMemcpyAsync(c->d_materials, s->materials, sizeof(xr_material) * (size_t)s->material_count,
+                                 hipMemcpyHostToDevice, c->stream));
+    XR_CREATE_HIP(hipMemcpyAsync(c->d_offsets, s->offsets, sizeof(uint32_t) * 2 * (size_t)s->instance_count,
+                                 hipMemcpyHostToDevice, c->stream));
+    XR_CREATE_HIP(hipMemcpyAsync(c->d_inst_first, first.data(), sizeof(uint32_t) * first.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+    XR_CREATE_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    XR_CREATE_HIP(hipStreamSynchronize(c->stream));
+#undef XR_CREATE_HIP
+    rc = alloc_frame(c, width, height);
+    if (rc != XR_OK) return fail(rc);
+    *out = c;
+    return XR_OK;
+}
+
+int xr_build_accel(xr_ctx* c, int builder) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (builder != XR_BUILDER_LBVH) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown builder %d", builder);
+    dev_free(&c->d_nodes);
+    dev_free(&c->d_tris);
+    c->built = false;
+    BvhBuildResult r;
+    char msg[256] = {0};
+    hipError_t e = build_lbvh(c->d_vertices, c->d_indices, c->d_offsets, c->d_inst_first, c->instance_count,
+                              c->tri_count, c->stream, &r, msg, sizeof(msg));
+    if (e != hipSuccess) return set_err(c, XR_ERR_DEVICE, "BVH build failed: %s", msg);
+    c->d_nodes = r.nodes;
+    c->d_tris = r.tris;
+    if (r.max_depth > (uint32_t)kMaxTraversalDepth) {
+        dev_free(&c->d_nodes);
+        dev_free(&c->d_tris);
+        return set_err(c, XR_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack (%d)", r.max_depth,
+                       kMaxTraversalDepth);
+    }
+    c->accel.triangle_count = r.tri_count;
+    c->accel.node_count = r.node_count;
+    c->accel.max_depth = r.max_depth;
+    c->accel.builder = (uint32_t)builder;
+    c->accel.build_ms = r.build_ms;
+    c->accel.node_bytes = (uint64_t)r.node_count * sizeof(BvhNode);
+    c->accel.triangle_bytes = (uint64_t)r.tri_count * sizeof(BvhTri);
+    c->built = true;
+    return XR_OK;
+}
+
+int xr_get_accel_info(const xr_ctx* c, xr_accel_info* info) {
+    if (!c || !info) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->built) return XR_ERR_STATE;
+    *info = c->accel;
+    return XR_OK;
+}
+
+int xr_read_accel(xr_ctx* c, void* nodes, size_t node_bytes, void* tris, size_t tri_bytes) {
+    if (!c || !nodes || !tris) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
+    if (node_bytes != c->accel.node_bytes || tri_bytes != c->accel.triangle_bytes)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "need %llu node bytes and %llu triangle bytes",
+                       (unsigned long long)c->accel.node_bytes, (unsigned long long)c->accel.triangle_bytes);
+    XR_HIP(c, hipMemcpyAsync(nodes, c->d_nodes, node_bytes, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipMemcpyAsync(tris, c->d_tris, tri_bytes, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_resize(xr_ctx* c, uint32_t w, uint32_t h) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (w == 0 || h == 0 || (uint64_t)w * h > (1ull << 28))
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "bad resolution %ux%u", w, h);
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return alloc_frame(c, w, h);
+}
+
+int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* params) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ubo) return set_err(c, XR_ERR_INVALID_ARGUMENT, "ubo is NULL");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
+    xr_restir_params p;
+    if (params) p = *params;
+    else xr_default_params(&p);
+    int rc = check_params(c, &p);
+    if (rc) return rc;
+    PassParams pp;
+    pp.temporal_m_cap = p.temporal_m_cap;
+    pp.spatial_m_cap = p.spatial_m_cap;
+    pp.spatial_samples = p.spatial_samples;
+    pp.spatial_radius = p.spatial_radius;
+    pp.visibility = p.visibility;
+    pp.plain_max_depth = p.plain_max_depth;
+    pp.history = 0;
+    UniformBufferObject u;
+    memcpy(&u, ubo, sizeof(u));
+    const SceneDev sc = scene_dev(c);
+    const FrameDev fr = frame_dev(c);
+    if (fault_inject("launch")) return set_err(c, XR_ERR_DEVICE, "injected launch failure");
+    if (mode == XR_MODE_PLAIN) {
+        XR_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        XR_HIP(c, launch_plain(sc, fr, u, p.plain_max_depth, c->stream));
+        XR_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        c->n_pass_events = 2;
+        return XR_OK;
+    }
+    if (mode != XR_MODE_RESTIR) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown mode %d", mode);
+    const uint32_t passes = p.passes ? p.passes : 7u;
+    XR_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, c->stream));
+    XR_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
+    XR_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+    XR_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    c->n_pass_events = 4;
+    return XR_OK;
+}
+
+int xr_sync(xr_ctx* c) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_pass_times(xr_ctx* c, float* ms, uint32_t n) {
+    if (!c || !ms) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (c->n_pass_events < 2) return 0;
+    XR_HIP(c, hipEventSynchronize(c->ev[c->n_pass_events - 1]));
+    int k = 0;
+    for (int i = 0; i + 1 < c->n_pass_events && (uint32_t)k < n; ++i) {
+        float t = 0.f;
+        XR_HIP(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+        ms[k++] = t;
+    }
+    return k;
+}
+
+int xr_counters(xr_ctx* c, uint64_t* out, uint32_t n) {
+    if (!c || !out) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    unsigned long long h[4];
+    XR_HIP(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n && i < 2; ++i) out[i] = h[i];
+    return XR_OK;
+}
+
+int xr_reset_counters(xr_ctx* c) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    XR_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    return XR_OK;
+}
+
+size_t xr_buffer_bytes(const xr_ctx* c, int which, int layout) {
+    if (!c) return 0;
+    const size_t n = (size_t)c->width * c->height;
+    switch (which) {
+        case XR_BUF_ACCUMULATION:
+        case XR_BUF_GBUF_POS:
+        case XR_BUF_GBUF_NRM:
+        case XR_BUF_GBUF_MATUV: return n * 16;
+        case XR_BUF_OUTPUT:
+        case XR_BUF_MOTION: return n * 4;
+        case XR_BUF_SAMPLES: return layout == XR_LAYOUT_REFERENCE ? n * sizeof(ReSTIRGISample) : n * kPlanes * 16;
+        case XR_BUF_TEMPORAL:
+        case XR_BUF_SPATIAL:
+            return layout == XR_LAYOUT_REFERENCE ? 2 * n * sizeof(ReSTIRGIReservoir) : 2 * n * kPlanes * 16;
+        default: return 0;
+    }
+}
+
+static void* buffer_ptr(xr_ctx* c, int which) {
+    switch (which) {
+        case XR_BUF_ACCUMULATION: return c->d_accum;
+        case XR_BUF_OUTPUT: return c->d_output;
+        case XR_BUF_SAMPLES: return c->d_samples;
+        case XR_BUF_TEMPORAL: return c->d_temporal;
+        case XR_BUF_SPATIAL: return c->d_spatial;
+        case XR_BUF_GBUF_POS: return c->d_gpos;
+        case XR_BUF_GBUF_NRM: return c->d_gnrm;
+        case XR_BUF_GBUF_MATUV: return c->d_gmatuv;
+        case XR_BUF_MOTION: return c->d_motion;
+        default: return nullptr;
+    }
+}
+
+// plane p of entry j (planes stored as [slot][plane][pixel])
+static inline const float* plane_at(const std::vector<float4>& v, size_t n, size_t j, int p) {
+    const size_t slot = j / n, i = j % n;
+    return &v[(slot * kPlanes + p) * n + i].x;
+}
+
+int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
+    if (!c || !dst) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (layout != XR_LAYOUT_REFERENCE && layout != XR_LAYOUT_NATIVE)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    const size_t want = xr_buffer_bytes(c, which, layout);
+    if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
+    if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
+    void* src = buffer_ptr(c, which);
+    const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
+    if (!planar || layout == XR_LAYOUT_NATIVE) {
+        XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        XR_HIP(c, hipStreamSynchronize(c->stream));
+        return XR_OK;
+    }
+    const size_t n = (size_t)c->width * c->height;
+    const size_t entries = which == XR_BUF_SAMPLES ? n : 2 * n;
+    std::vector<float4> tmp;
+    try {
+        tmp.resize(entries * kPlanes);
+    } catch (const std::bad_alloc&) {
+        return set_err(c, XR_ERR_OUT_OF_MEMORY, "host staging allocation failed");
+    }
+    XR_HIP(c, hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t j = 0; j < entries; ++j) {
+        const float* q[kPlanes];
+        for (int p = 0; p < kPlanes; ++p) q[p] = plane_at(tmp, n, j, p);
+        ReSTIRGISample s;
+        memset(&s, 0, sizeof(s));
+        memcpy(s.x_view, q[0], 12);
+        memcpy(&s.materialID, q[0] + 3, 4);
+        memcpy(s.n_view, q[1], 12);
+        memcpy(&s.randSeed, q[1] + 3, 4);
+        memcpy(s.x_sample, q[2], 12);
+        memcpy(&s.p_q, q[2] + 3, 4);
+        memcpy(s.n_sample, q[3], 12);
+        memcpy(s.L_out, q[4], 12);
+        memcpy(s.f, q[5], 12);
+        if (which == XR_BUF_SAMPLES) {
+            memcpy((char*)dst + j * sizeof(ReSTIRGISample), &s, sizeof(s));
+        } else {
+            ReSTIRGIReservoir r;
+            memset(&r, 0, sizeof(r));
+            memcpy(&r.w_sum, q[3] + 3, 4);
+            memcpy(&r.W, q[4] + 3, 4);
+            memcpy(&r.m, q[5] + 3, 4);
+            r.s = s;
+            memcpy((char*)dst + j * sizeof(ReSTIRGIReservoir), &r, sizeof(r));
+        }
+    }
+    return XR_OK;
+}
+
+int xr_write_buffer(xr_ctx* c, int which, const void* src, size_t bytes, int layout) {
+    if (!c || !src) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (layout != XR_LAYOUT_REFERENCE && layout != XR_LAYOUT_NATIVE)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    const size_t want = xr_buffer_bytes(c, which, layout);
+    if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
+    if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
+    void* dst = buffer_ptr(c, which);
+    const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
+    if (!planar || layout == XR_LAYOUT_NATIVE) {
+        XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        XR_HIP(c, hipStreamSynchronize(c->stream));
+        return XR_OK;
+    }
+    const size_t n = (size_t)c->width * c->height;
+    const size_t entries = which == XR_BUF_SAMPLES ? n : 2 * n;
+    std::vector<float4> tmp;
+    try {
+        tmp.resize(entries * kPlanes);
+    } catch (const std::bad_alloc&) {
+        return set_err(c, XR_ERR_OUT_OF_MEMORY, "host staging allocation failed");
+    }
+    for (size_t j = 0; j < entries; ++j) {
+        ReSTIRGISample s;
+        ReSTIRGIReservoir r;
+        memset(&r, 0, sizeof(r));
+        if (which == XR_BUF_SAMPLES) {
+            memcpy(&s, (const char*)src + j * sizeof(s), sizeof(s));
+        } else {
+            memcpy(&r, (const char*)src + j * sizeof(r), sizeof(r));
+            s = r.s;
+        }
+        const size_t slot = j / n, i = j % n;
+        float4* q[kPlanes];
+        for (int p = 0; p < kPlanes; ++p) q[p] = &tmp[(slot * kPlanes + p) * n + i];
+        memcpy(&q[0]->x, s.x_view, 12);
+        memcpy(&q[0]->w, &s.materialID, 4);
+        memcpy(&q[1]->x, s.n_view, 12);
+        memcpy(&q[1]->w, &s.randSeed, 4);
+        memcpy(&q[2]->x, s.x_sample, 12);
+        q[2]->w = s.p_q;
+        memcpy(&q[3]->x, s.n_sample, 12);
+        memcpy(&q[4]->x, s.L_out, 12);
+        memcpy(&q[5]->x, s.f, 12);
+        if (which == XR_BUF_SAMPLES) {
+            q[3]->w = 0.f;
+            q[4]->w = 0.f;
+            q[5]->w = 0.f;
+        } else {
+            q[3]->w = r.w_sum;
+            q[4]->w = r.W;
+            memcpy(&q[5]->w, &r.m, 4);
+        }
+    }
+    XR_HIP(c, hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_read_image(xr_ctx* c, uint32_t* dst, size_t bytes) {
+    return xr_read_buffer(c, XR_BUF_OUTPUT, dst, bytes, XR_LAYOUT_NATIVE);
+}
+
+int xr_trace_rays_device(xr_ctx* c, const xr_ray* d_rays, uint32_t n, xr_hit* d_hits, int any_hit) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (n && (!d_rays || !d_hits)) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL ray or hit array");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
+    XR_HIP(c, launch_trace(scene_dev(c), (const float4*)d_rays, n, (float4*)d_hits, any_hit != 0, c->stream));
+    return XR_OK;
+}
+
+int xr_trace_rays(xr_ctx* c, const xr_ray* rays, uint32_t n, xr_hit* hits, int any_hit) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (n == 0) return XR_OK;
+    if (!rays || !hits) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL ray or hit array");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
+    if (n > c->ray_cap) {
+        dev_free(&c->d_rays);
+        dev_free(&c->d_hits);
+        c->ray_cap = 0;
+        XR_HIP(c, dev_alloc(&c->d_rays, sizeof(xr_ray) * (size_t)n));
+        XR_HIP(c, dev_alloc(&c->d_hits, sizeof(xr_hit) * (size_t)n));
+        c->ray_cap = n;
+    }
+    XR_HIP(c, hipMemcpyAsync(c->d_rays, rays, sizeof(xr_ray) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    XR_HIP(c, launch_trace(scene_dev(c), c->d_rays, n, c->d_hits, any_hit != 0, c->stream));
+    XR_HIP(c, hipMemcpyAsync(hits, c->d_hits, sizeof(xr_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+}  // extern "C"

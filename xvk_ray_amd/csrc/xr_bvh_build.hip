@@ -1,0 +1,352 @@
+// GPU LBVH builder (Karras, "Maximizing Parallelism in the Construction of
+// BVHs, Octrees, and k-d Trees", HPG 2012) for gfx950.
+//
+// Replaces vkCmdBuildAccelerationStructuresKHR of the reference
+// (XVKBLAccelerationStructure.cpp:44-58, XVKTLAccelerationStructure.cpp:49-66),
+// whose BVH arithmetic lives in the Vulkan driver.  All instances carry
+// identity transforms in the reference, so the two-level structure collapses
+// into one BVH over every triangle of every instance.
+//
+// Passes (one kernel each, all on the caller's stream):
+//   1. gather   canonical triangles from the reference buffers, centroid bounds
+//   2. morton   63-bit Morton code of each centroid
+//   3. sort     (code, id) pairs, rocPRIM/hipCUB radix sort
+//   4. karras   internal-node topology from the sorted codes
+//   5. refit    bottom-up boxes, one atomic counter per internal node
+//   6. emit     64-byte two-box nodes and the BVH-ordered triangle array
+#include <hipcub/hipcub.hpp>
+#include <stdio.h>
+#include "xr_kernels.h"
+
+namespace xr {
+
+namespace {
+
+__device__ __forceinline__ int ord_enc(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord_dec(int o) { return __int_as_float(o >= 0 ? o : o ^ 0x7FFFFFFF); }
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restrict__ idx,
+                         const uint2* __restrict__ offsets, const uint32_t* __restrict__ first_tri,
+                         uint32_t ninst, uint32_t ntri, BvhTri* __restrict__ ctris, float4* __restrict__ cent,
+                         int* __restrict__ bounds) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    float c[3] = {0.f, 0.f, 0.f};
+    bool live = g < ntri;
+    if (live) {
+        // instance = last i with first_tri[i] <= g
+        uint32_t lo = 0, hi = ninst;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (first_tri[mid] <= g) lo = mid; else hi = mid;
+        }
+        const uint32_t inst = lo;
+        const uint32_t prim = g - first_tri[inst];
+        const uint2 off = offsets[inst];
+        float p[3][3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = off.y + idx[off.x + prim * 3u + k];
+            const float* q = vtx + (size_t)vi * 9u;
+            p[k][0] = q[0]; p[k][1] = q[1]; p[k][2] = q[2];
+        }
+        BvhTri t;
+        t.v0 = make_float4(p[0][0], p[0][1], p[0][2], __uint_as_float(g));
+        t.v1 = make_float4(p[1][0], p[1][1], p[1][2], __uint_as_float(inst));
+        t.v2 = make_float4(p[2][0], p[2][1], p[2][2], __uint_as_float(prim));
+        ctris[g] = t;
+        for (int a = 0; a < 3; ++a) c[a] = (p[0][a] + p[1][a] + p[2][a]) * (1.0f / 3.0f);
+        cent[g] = make_float4(c[0], c[1], c[2], 0.f);
+    }
+    // block reduction of the centroid bounds, then one atomic per block
+    __shared__ int smin[3][256], smax[3][256];
+    for (int a = 0; a < 3; ++a) {
+        smin[a][threadIdx.x] = live ? ord_enc(c[a]) : 0x7FFFFFFF;
+        smax[a][threadIdx.x] = live ? ord_enc(c[a]) : (int)0x80000000;
+    }
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int a = 0; a < 3; ++a) {
+                smin[a][threadIdx.x] = min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+                smax[a][threadIdx.x] = max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&bounds[a], smin[a][0]);
+            atomicMax(&bounds[3 + a], smax[a][0]);
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long spread21(uint32_t v) {
+    unsigned long long x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(const float4* __restrict__ cent, uint32_t ntri, const int* __restrict__ bounds,
+                         unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ntri) return;
+    const float4 c = cent[g];
+    float lo[3], ext[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = ord_dec(bounds[a]);
+        const float hi = ord_dec(bounds[3 + a]);
+        ext[a] = hi > lo[a] ? hi - lo[a] : 1.0f;
+    }
+    const float cc[3] = {c.x, c.y, c.z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+        float u = (cc[a] - lo[a]) / ext[a];
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        q[a] = (uint32_t)fminf(u * 2097152.0f, 2097151.0f);
+    }
+    keys[g] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+    vals[g] = g;
+}
+
+__device__ __forceinline__ int delta(const unsigned long long* __restrict__ keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const unsigned long long a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Children encoding during the build: value < n-1 internal, else leaf (value - (n-1)).
+__global__ void k_karras(const unsigned long long* __restrict__ keys, int n, int2* __restrict__ child,
+                         int* __restrict__ parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == gamma) ? (n - 1) + gamma : gamma;
+    const int right = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
+    child[i] = make_int2(left, right);
+    parent[left] = i;
+    parent[right] = i;
+}
+
+__device__ __forceinline__ float ld_agent(const float* p) {
+    return __uint_as_float(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Bottom-up boxes.  Node ids: internal 0..n-2, leaf j -> n-1+j.  The second
+// lane to reach an internal node computes its box and walks on.
+__global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
+                        const int2* __restrict__ child, const int* __restrict__ parent, Box* boxes,
+                        int* __restrict__ flags, uint32_t* __restrict__ max_depth) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const BvhTri t = ctris[sorted_ids[j]];
+    Box b;
+    b.lo[0] = fminf(fminf(t.v0.x, t.v1.x), t.v2.x);
+    b.lo[1] = fminf(fminf(t.v0.y, t.v1.y), t.v2.y);
+    b.lo[2] = fminf(fminf(t.v0.z, t.v1.z), t.v2.z);
+    b.hi[0] = fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x);
+    b.hi[1] = fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y);
+    b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z);
+    int node = (n - 1) + j;
+    boxes[node] = b;
+    // Depth of this leaf.  A valid tree reaches the root in < n steps; the
+    // bound keeps a malformed one from looping (it then reports depth n).
+    uint32_t depth = 0;
+    {
+        int p = node;
+        while (p != 0 && n > 1 && depth < (uint32_t)n) { p = parent[p]; ++depth; }
+    }
+    atomicMax(max_depth, depth);
+    if (n == 1) return;
+    for (int steps = 0; node != 0 && steps < n; ++steps) {
+        const int p = parent[node];
+        __threadfence();
+        const int old = atomicAdd(&flags[p], 1);
+        if (old == 0) return;
+        __threadfence();
+        const int2 c = child[p];
+        Box m;
+        for (int a = 0; a < 3; ++a) {
+            m.lo[a] = fminf(ld_agent(&boxes[c.x].lo[a]), ld_agent(&boxes[c.y].lo[a]));
+            m.hi[a] = fmaxf(ld_agent(&boxes[c.x].hi[a]), ld_agent(&boxes[c.y].hi[a]));
+        }
+        boxes[p] = m;
+        node = p;
+    }
+}
+
+__global__ void k_emit(int n, const int2* __restrict__ child, const Box* __restrict__ boxes,
+                       BvhNode* __restrict__ nodes, const BvhTri* __restrict__ ctris,
+                       const uint32_t* __restrict__ sorted_ids, BvhTri* __restrict__ tris) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tris[i] = ctris[sorted_ids[i]];
+    const int ninner = n > 1 ? n - 1 : 1;
+    if (i >= ninner) return;
+    int c0, c1;
+    if (n == 1) {
+        c0 = c1 = 0;  // leaf node id 0 -> both links name leaf 0
+    } else {
+        c0 = child[i].x;
+        c1 = child[i].y;
+    }
+    const int leaf_base = n > 1 ? n - 1 : 0;
+    const Box b0 = boxes[n > 1 ? c0 : 0];
+    const Box b1 = boxes[n > 1 ? c1 : 0];
+    BvhNode nd;
+    nd.a = make_float4(b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]);
+    nd.b = make_float4(b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]);
+    nd.c = make_float4(b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]);
+    auto enc = [&](int c) -> int {
+        if (n == 1) return ~0;
+        return c < leaf_base ? c : ~((c - leaf_base) * 16);
+    };
+    nd.d = make_int4(enc(c0), enc(c1), 0, 0);
+    nodes[i] = nd;
+}
+
+}  // namespace
+
+#define XR_TRY(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            snprintf(err, err_len, "%s failed: %s", #expr, hipGetErrorString(e_));           \
+            goto fail;                                                                       \
+        }                                                                                    \
+    } while (0)
+
+hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len) {
+    hipError_t rc = hipSuccess;
+    const int n = (int)tri_count;
+    const uint32_t ninner = n > 1 ? (uint32_t)(n - 1) : 1u;
+    BvhTri* ctris = nullptr;
+    float4* cent = nullptr;
+    int* bounds = nullptr;
+    unsigned long long *keys = nullptr, *keys2 = nullptr;
+    uint32_t *vals = nullptr, *vals2 = nullptr;
+    int2* child = nullptr;
+    int* parent = nullptr;
+    Box* boxes = nullptr;
+    int* flags = nullptr;
+    uint32_t* dmax = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    BvhNode* nodes = nullptr;
+    BvhTri* tris = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int init_bounds[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
+                                (int)0x80000000};
+    const uint32_t blocks = (tri_count + 255) / 256;
+    uint32_t host_depth = 0;
+    float ms = 0.f;
+    if (tri_count == 0) {
+        snprintf(err, err_len, "scene has no triangles");
+        return hipErrorInvalidValue;
+    }
+    XR_TRY(hipEventCreate(&e0));
+    XR_TRY(hipEventCreate(&e1));
+    XR_TRY(hipEventRecord(e0, st));
+    XR_TRY(hipMalloc(&ctris, sizeof(BvhTri) * tri_count));
+    XR_TRY(hipMalloc(&cent, sizeof(float4) * tri_count));
+    XR_TRY(hipMalloc(&bounds, sizeof(int) * 6));
+    XR_TRY(hipMalloc(&keys, sizeof(unsigned long long) * tri_count));
+    XR_TRY(hipMalloc(&keys2, sizeof(unsigned long long) * tri_count));
+    XR_TRY(hipMalloc(&vals, sizeof(uint32_t) * tri_count));
+    XR_TRY(hipMalloc(&vals2, sizeof(uint32_t) * tri_count));
+    XR_TRY(hipMalloc(&child, sizeof(int2) * ninner));
+    XR_TRY(hipMalloc(&parent, sizeof(int) * (ninner + tri_count)));
+    XR_TRY(hipMalloc(&boxes, sizeof(Box) * (ninner + tri_count)));
+    XR_TRY(hipMalloc(&flags, sizeof(int) * ninner));
+    XR_TRY(hipMalloc(&dmax, sizeof(uint32_t)));
+    XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * ninner));
+    XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * tri_count));
+    XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
+    XR_TRY(hipMemsetAsync(flags, 0, sizeof(int) * ninner, st));
+    XR_TRY(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
+    XR_TRY(hipMemsetAsync(parent, 0, sizeof(int) * (ninner + tri_count), st));
+
+    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
+                       instance_count, tri_count, ctris, cent, bounds);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, st, cent, tri_count, bounds, keys, vals);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
+    XR_TRY(hipMalloc(&temp, temp_bytes));
+    XR_TRY(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, keys2, n, child, parent);
+        XR_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, child, parent, boxes, flags, dmax);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_emit, dim3((tri_count + 255) / 256), dim3(256), 0, st, n, child, boxes, nodes, ctris,
+                       vals2, tris);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipEventRecord(e1, st));
+    XR_TRY(hipMemcpyAsync(&host_depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    XR_TRY(hipEventElapsedTime(&ms, e0, e1));
+
+    out->nodes = nodes;
+    out->tris = tris;
+    out->node_count = ninner;
+    out->tri_count = tri_count;
+    out->max_depth = host_depth;
+    out->build_ms = ms;
+    nodes = nullptr;
+    tris = nullptr;
+    rc = hipSuccess;
+    goto done;
+fail:
+    rc = hipErrorUnknown;
+done:
+    hipFree(ctris);
+    hipFree(cent);
+    hipFree(bounds);
+    hipFree(keys);
+    hipFree(keys2);
+    hipFree(vals);
+    hipFree(vals2);
+    hipFree(child);
+    hipFree(parent);
+    hipFree(boxes);
+    hipFree(flags);
+    hipFree(dmax);
+    hipFree(temp);
+    hipFree(nodes);
+    hipFree(tris);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return rc;
+}
+
+}  // namespace xr

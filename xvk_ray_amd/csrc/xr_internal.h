@@ -1,0 +1,235 @@
+// Device-side views of the scene and of the per-frame buffers, plus the
+// traversal routine shared by every kernel that traces rays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "xr_layout.h"
+#include "xr_math.h"
+#include "xr_bvh.h"
+
+namespace xr {
+
+// Number of float4 planes of a sample / reservoir (see xr_layout.h).
+constexpr int kPlanes = 6;
+
+struct SceneDev {
+    const float* vertices;      // AVertex stream, 9 dwords per vertex (binding 4)
+    const uint32_t* indices;    // binding 5
+    const float4* materials;    // AMaterial as 2 float4 (binding 6)
+    const uint2* offsets;       // (indexOffset, vertexOffset) per instance (binding 7)
+    const BvhNode* nodes;
+    const BvhTri* tris;
+    uint32_t material_count;
+};
+
+struct FrameDev {
+    uint32_t width, height, n;  // n = width*height
+    float4* samples;            // kPlanes planes of n            (binding 9)
+    float4* temporal;           // 2 slots x kPlanes planes of n  (binding 10)
+    float4* spatial;            // 2 slots x kPlanes planes of n  (binding 11)
+    float4* gpos;               // binding 12 (16 aliases it, see ReSTIRGIPipeline.cpp:171-199)
+    float4* gnrm;               // binding 13 (17 aliases it)
+    float4* gmatuv;             // binding 14
+    uint32_t* motion;           // binding 15, RG16F
+    uint32_t* output;           // binding 2, B8G8R8A8_UNORM
+    float4* accum;              // binding 1, RGBA32F
+    unsigned long long* counters;  // [0] extend rays, [1] shadow rays
+};
+
+// Parameters of one ReSTIR GI / plain frame as the kernels see them.
+struct PassParams {
+    uint32_t temporal_m_cap;
+    uint32_t spatial_m_cap;
+    uint32_t spatial_samples;
+    float spatial_radius;
+    uint32_t visibility;       // 0: CheckVisibility returns true (reference), 1: trace
+    uint32_t plain_max_depth;  // 0: unbounded (rgen.hlsl:59)
+    uint32_t history;          // 0: validity read from the current G-buffer (reference aliasing), 1: previous frame's
+};
+
+// Column-major 4x4 times (v0, v1, v2, v3), rows summed left to right
+// (HLSL mul(M, v) on a glm matrix).
+XR_HD void mat_vec(const float* M, float v0, float v1, float v2, float v3, float* out4) {
+    for (int r = 0; r < 4; ++r) out4[r] = M[r] * v0 + M[4 + r] * v1 + M[8 + r] * v2 + M[12 + r] * v3;
+}
+
+// --------------------------------------------------------------------------
+// Traversal.  The stack lives in LDS, one column per lane
+// (stack[level * kBlock + lane] -> consecutive lanes hit consecutive banks);
+// levels beyond kLdsDepth spill to a private array.  The builder rejects trees
+// deeper than kLdsDepth + kSpillDepth, so the stack cannot overflow.
+// --------------------------------------------------------------------------
+constexpr int kLdsDepth = 24;
+constexpr int kSpillDepth = 40;
+constexpr int kMaxTraversalDepth = kLdsDepth + kSpillDepth;
+constexpr uint32_t kMaxNodeVisits = 1u << 24;
+
+template <int kBlock, bool kAnyHit>
+__device__ __forceinline__ bool trace_ray(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
+                                          const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
+                                          Hit* hit) {
+    int spill[kSpillDepth];
+    int sp = 0;
+    int node = 0;
+    float best_t = tmax;
+    uint32_t best_id = 0xFFFFFFFFu;
+    uint32_t best_idx = 0xFFFFFFFFu;
+    float best_u = 0.0f, best_v = 0.0f;
+    const float kInf = __builtin_huge_valf();
+    // A well-formed tree visits each node at most once per ray; the cap only
+    // guarantees termination should a malformed tree ever reach the kernel.
+    uint32_t guard = 0;
+    while (++guard < kMaxNodeVisits) {
+        const float4 na = nodes[node].a;
+        const float4 nb = nodes[node].b;
+        const float4 nc = nodes[node].c;
+        const int4 nd = nodes[node].d;
+        const float tb = best_t * 1.0000003576f;
+        float e0 = slab(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb);
+        float e1 = slab(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb);
+        int c0 = nd.x, c1 = nd.y;
+        bool h0 = e0 != kInf, h1 = e1 != kInf;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const bool hs = side == 0 ? h0 : h1;
+            const int cs = side == 0 ? c0 : c1;
+            if (hs && cs < 0) {
+                const int code = ~cs;
+                const int first = code >> 4;
+                const int cnt = (code & 15) + 1;
+                for (int k = 0; k < cnt; ++k) {
+                    const BvhTri tr = tris[first + k];
+                    float t, u, v;
+                    if (intersect_tri(r, mk3(tr.v0.x, tr.v0.y, tr.v0.z), mk3(tr.v1.x, tr.v1.y, tr.v1.z),
+                                      mk3(tr.v2.x, tr.v2.y, tr.v2.z), tmin, best_t, &t, &u, &v)) {
+                        const uint32_t id = as_uint(tr.v0.w);
+                        if (kAnyHit) {
+                            hit->t = t; hit->u = u; hit->v = v; hit->id = id; hit->idx = first + k;
+                            return true;
+                        }
+                        // intersect_tri guarantees t <= best_t, so this is
+                        // "t < best_t, or t == best_t and a smaller id".
+                        if (t < best_t || id < best_id) {
+                            best_t = t; best_id = id; best_idx = first + k; best_u = u; best_v = v;
+                        }
+                    }
+                }
+                if (side == 0) h0 = false; else h1 = false;
+            }
+        }
+        if (h0 && h1) {
+            if (e1 < e0) { int tmp = c0; c0 = c1; c1 = tmp; }
+            if (sp < kLdsDepth) lds_col[sp * kBlock] = c1;
+            else spill[sp - kLdsDepth] = c1;
+            ++sp;
+            node = c0;
+        } else if (h0) {
+            node = c0;
+        } else if (h1) {
+            node = c1;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            node = sp < kLdsDepth ? lds_col[sp * kBlock] : spill[sp - kLdsDepth];
+        }
+    }
+    if (kAnyHit) return false;
+    hit->t = best_idx == 0xFFFFFFFFu ? -1.0f : best_t;
+    hit->u = best_u;
+    hit->v = best_v;
+    hit->id = best_id;
+    hit->idx = best_idx;
+    return best_idx != 0xFFFFFFFFu;
+}
+
+// Closest-hit attributes (Ray.rchit.hlsl:26-61): world position from
+// barycentric interpolation, twice-normalized interpolated normal, geometric
+// normal normalize(cross(v2-v0, v1-v0)), UV, material of v0.  Instances carry
+// identity transforms (XVKTLAccelerationStructure.cpp:68-93), so the
+// object-to-world products are exact copies and are omitted.
+struct Payload {
+    v3 position;
+    v3 normal;
+    v3 geo_normal;
+    float u, v;
+    int32_t material;
+    float t;
+};
+
+__device__ __forceinline__ void load_vertex(const float* __restrict__ vb, uint32_t i, v3* p, v3* n, float* tu,
+                                            float* tv, int32_t* mat) {
+    const float* q = vb + (size_t)i * 9u;
+    *p = mk3(q[0], q[1], q[2]);
+    *n = mk3(q[3], q[4], q[5]);
+    *tu = q[6];
+    *tv = q[7];
+    *mat = __float_as_int(q[8]);
+}
+
+__device__ __forceinline__ void closest_hit(const SceneDev& sc, const Hit& h, Payload* pl) {
+    const BvhTri tr = sc.tris[h.idx];
+    const uint32_t inst = as_uint(tr.v1.w);
+    const uint32_t prim = as_uint(tr.v2.w);
+    const uint2 off = sc.offsets[inst];
+    const uint32_t base = off.x + prim * 3u;
+    const uint32_t i0 = sc.indices[base + 0], i1 = sc.indices[base + 1], i2 = sc.indices[base + 2];
+    v3 p0, p1, p2, n0, n1, n2;
+    float u0, v0, u1, v1, u2, v2;
+    int32_t m0, m1, m2;
+    load_vertex(sc.vertices, off.y + i0, &p0, &n0, &u0, &v0, &m0);
+    load_vertex(sc.vertices, off.y + i1, &p1, &n1, &u1, &v1, &m1);
+    load_vertex(sc.vertices, off.y + i2, &p2, &n2, &u2, &v2, &m2);
+    const float bx = 1.0f - h.u - h.v, by = h.u, bz = h.v;
+    v3 nrm = add(add(muls(n0, bx), muls(n1, by)), muls(n2, bz));
+    nrm = normalize(normalize(nrm));
+    pl->position = add(add(muls(p0, bx), muls(p1, by)), muls(p2, bz));
+    pl->normal = nrm;
+    pl->u = u0 * bx + u1 * by + u2 * bz;
+    pl->v = v0 * bx + v1 * by + v2 * bz;
+    pl->geo_normal = normalize(cross(sub(p2, p0), sub(p1, p0)));
+    pl->material = m0;
+    pl->t = h.t;
+}
+
+// BranchlessONB (Sampling.hlsl:4-12; Duff et al. 2017).
+XR_HD void onb(v3 n, v3* b1, v3* b2) {
+    const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    *b1 = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    *b2 = mk3(b, sign + n.y * n.y * a, -n.y);
+}
+
+// SampleBSDF (BSDF.hlsl:34-57) with SampleLambertian (:7-25).  Untextured
+// materials only (the scene loader rejects texture indices >= 0).
+__device__ __forceinline__ v3 sample_bsdf(v3 n, const float4 mat0, const float4 mat1, v3* wi_world, float* pdf,
+                                          float* cos_theta, uint32_t* seed) {
+    v3 T, B;
+    onb(n, &T, &B);
+    v3 f = mk3(0.0f, 0.0f, 0.0f);
+    v3 wl = mk3(0.0f, 0.0f, 0.0f);
+    *pdf = 0.0f;
+    *cos_theta = 0.0f;
+    const uint32_t model = __float_as_uint(mat1.w);
+    if (model == kLambertian) {
+        const float xi0 = rand01(seed);
+        const float xi1 = rand01(seed);
+        const float phi = XR_TWO_PI * xi0;
+        const float ct = sqrtf(1.0f - xi1);
+        const float st = sqrtf(xi1);
+        float s, c;
+        sincos_xr(phi, &s, &c);
+        wl = mk3(st * c, st * s, ct);
+        *cos_theta = wl.z;
+        *pdf = wl.z * XR_INV_PI;
+        if (wl.z <= 0.0f) {
+            *pdf = 0.0f;
+        } else {
+            f = mk3(mat0.x * XR_INV_PI, mat0.y * XR_INV_PI, mat0.z * XR_INV_PI);
+        }
+    }
+    *wi_world = mk3(wl.x * T.x + wl.y * B.x + wl.z * n.x, wl.x * T.y + wl.y * B.y + wl.z * n.y,
+                    wl.x * T.z + wl.y * B.z + wl.z * n.z);
+    return f;
+}
+
+}  // namespace xr

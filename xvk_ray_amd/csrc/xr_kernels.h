@@ -1,0 +1,40 @@
+// Host-side launchers of the frame kernels and of the BVH builder.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "xr_internal.h"
+
+namespace xr {
+
+uint32_t image_grid(uint32_t width, uint32_t height);
+
+hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
+                                 hipStream_t st);
+hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
+                           hipStream_t st);
+hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
+                          const PassParams& pp, hipStream_t st);
+hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,
+                        hipStream_t st);
+hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,
+                        hipStream_t st);
+
+// Output of the GPU builder.  All arrays are device allocations owned by the
+// caller after a successful build (free with hipFree).
+struct BvhBuildResult {
+    BvhNode* nodes = nullptr;
+    BvhTri* tris = nullptr;
+    uint32_t node_count = 0;
+    uint32_t tri_count = 0;
+    uint32_t max_depth = 0;   // edges on the longest root-to-leaf path
+    float build_ms = 0.0f;
+};
+
+// LBVH (Karras 2012) over all triangles of the scene.
+//   vertices/indices/offsets: device copies of the reference buffers
+//   inst_first_tri: device array, first canonical triangle of each instance
+//   (instance_count + 1 entries; the last is the total triangle count)
+hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
+
+}  // namespace xr

@@ -1,0 +1,570 @@
+// gfx950 kernels of the ReSTIR GI frame and of the plain path tracer.
+//
+//   k_initial_sample  ReSTIRGIInitialSample.rgen.hlsl:35-214
+//   k_temporal        ReSTIRGITemporalReuse.rgen.hlsl:42-151
+//   k_spatial         ReSTIRGISpatialReuse.rgen.hlsl:64-183
+//   k_plain           rgen.hlsl:31-151 (+ rhit.hlsl, sampling.hlsl)
+//   k_trace           TraceRay on a caller-supplied ray batch
+//
+// Image kernels run 256-lane workgroups on 16x16 pixel tiles; each wave64
+// covers a 16x4 strip.  Tiles are dealt to the eight XCDs in contiguous
+// bands (the hardware hands workgroup i to XCD i % 8), so each XCD's L2 holds
+// one band of the image, its G-buffer/reservoir neighbourhoods and the BVH
+// nodes its rays touch.
+#include "xr_internal.h"
+#include "xr_kernels.h"
+
+namespace xr {
+
+constexpr int kTile = 16;
+constexpr int kBlock = kTile * kTile;
+constexpr int kXcds = 8;
+
+struct TileCoord {
+    uint32_t x, y;
+    bool in_image;
+};
+
+// Logical tile of this workgroup: XCD k (= blockIdx.x % 8) gets tiles
+// [k*per, (k+1)*per), walked in row-major order.
+__device__ __forceinline__ TileCoord tile_pixel(uint32_t width, uint32_t height) {
+    const uint32_t tiles_x = (width + kTile - 1) / kTile;
+    const uint32_t tiles_y = (height + kTile - 1) / kTile;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+    const uint32_t b = blockIdx.x;
+    const uint32_t tile = (b % kXcds) * per + b / kXcds;
+    TileCoord tc;
+    const uint32_t lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+    tc.x = (tile % tiles_x) * kTile + lx;
+    tc.y = (tile / tiles_x) * kTile + ly;
+    tc.in_image = tile < ntiles && tc.x < width && tc.y < height;
+    return tc;
+}
+
+uint32_t image_grid(uint32_t width, uint32_t height) {
+    const uint32_t tiles = ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+    return ((tiles + kXcds - 1) / kXcds) * kXcds;
+}
+
+// One LDS atomic per lane, one global atomic per workgroup.
+__device__ __forceinline__ void count_rays(unsigned long long* counter, uint32_t n, uint32_t* lds_count) {
+    if (threadIdx.x == 0) *lds_count = 0;
+    __syncthreads();
+    if (n) atomicAdd(lds_count, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && *lds_count) atomicAdd(counter, (unsigned long long)*lds_count);
+}
+
+__device__ __forceinline__ v3 xyz(float4 a) { return mk3(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// Camera ray for pixel (x, y) with a jitter drawn from the stream
+// (InitialSample.rgen:43-52, rgen.hlsl:44-54).
+__device__ __forceinline__ void camera_ray(const UniformBufferObject& ubo, uint32_t x, uint32_t y, uint32_t w,
+                                           uint32_t h, uint32_t* rng, v3* org, v3* dir) {
+    const float jx = rand01(rng);
+    const float jy = rand01(rng);
+    const float px = (float)x + jx, py = (float)y + jy;
+    const float uvx = (px / (float)w) * 2.0f - 1.0f;
+    const float uvy = (py / (float)h) * 2.0f - 1.0f;
+    float o4[4], t4[4], d4[4];
+    mat_vec(ubo.viewInverse, 0.0f, 0.0f, 0.0f, 1.0f, o4);
+    mat_vec(ubo.projectionInverse, uvx, uvy, 1.0f, 1.0f, t4);
+    const v3 tn = normalize(mk3(t4[0], t4[1], t4[2]));
+    mat_vec(ubo.viewInverse, tn.x, tn.y, tn.z, 0.0f, d4);
+    *org = mk3(o4[0], o4[1], o4[2]);
+    *dir = normalize(mk3(d4[0], d4[1], d4[2]));
+}
+
+// ---------------------------------------------------------------------------
+// Initial sample pass
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image) {
+        const uint32_t x = tc.x, y = tc.y;
+        const uint32_t pix = y * fr.width + x;
+        uint32_t rng = init_seed(x, y, fr.width, ubo.currentFrame);
+        v3 origin, dir;
+        camera_ray(ubo, x, y, fr.width, fr.height, &rng, &origin, &dir);
+
+        v3 thr = mk3(1.0f, 1.0f, 1.0f);
+        v3 x_view = mk3(0.f, 0.f, 0.f), n_view = x_view, x_s = x_view, n_s = x_view, L = x_view, f0 = x_view;
+        float p_q = 0.0f, uv_u = 0.0f, uv_v = 0.0f;
+        uint32_t mat_idx = 0xFFFFFFFFu, rand_seed = 0;
+        bool valid = false;
+        int* col = stack + threadIdx.x;
+        for (uint32_t bounce = 0;; ++bounce) {
+            if (bounce == ubo.numberOfBounces) break;
+            const RayPre rp = make_ray_pre(origin, dir);
+            Hit hit;
+            ++rays;
+            // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
+            Payload pl;
+            bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+            if (found) {
+                closest_hit(sc, hit, &pl);
+                found = pl.material != -1;
+            }
+            if (!found) {
+                if (ubo.hasSkyBox) L = add(L, thr);
+                if (bounce == 1) {
+                    x_s = add(origin, muls(dir, 1000.0f));
+                    n_s = neg(dir);
+                }
+                break;
+            }
+            const float4 m0 = sc.materials[2 * pl.material];
+            const float4 m1 = sc.materials[2 * pl.material + 1];
+            if (__float_as_uint(m1.w) == kDiffuseLight) {
+                L = add(L, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+                if (bounce == 1) { x_s = pl.position; n_s = pl.normal; }
+                if (bounce == 0) { x_view = pl.position; n_view = pl.normal; }
+                break;
+            }
+            if (bounce == 1) { x_s = pl.position; n_s = pl.normal; }
+            const uint32_t seed_here = rng;
+            v3 wi;
+            float pdf, cos_t;
+            const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
+            if (bounce == 0) {
+                valid = true;
+                x_view = pl.position;
+                n_view = pl.normal;
+                rand_seed = seed_here;
+                mat_idx = (uint32_t)pl.material;
+                uv_u = pl.u;
+                uv_v = pl.v;
+                p_q = pdf;
+                f0 = f;
+            } else {
+                thr = mul(thr, divs(muls(f, cos_t), pdf));
+            }
+            if (bounce > 3) {
+                const float p = maxf(thr.x, maxf(thr.y, thr.z));
+                if (rand01(&rng) > p) break;
+                thr = muls(thr, 1.0f / p);
+            }
+            const v3 hp = add(origin, muls(dir, pl.t));
+            const v3 g = pl.geo_normal;
+            const v3 od = dot(wi, g) > 0.0f ? g : neg(g);
+            origin = add(hp, muls(od, 0.01f));
+            dir = wi;
+        }
+        const uint32_t n = fr.n;
+        float4* S = fr.samples;
+        if (!valid) { L = mk3(0.f, 0.f, 0.f); n_s = L; }
+        S[0 * n + pix] = f4(x_view, __uint_as_float(mat_idx));
+        S[1 * n + pix] = f4(n_view, __uint_as_float(rand_seed));
+        S[2 * n + pix] = f4(x_s, p_q);
+        S[3 * n + pix] = f4(n_s, 0.0f);
+        S[4 * n + pix] = f4(L, 0.0f);
+        S[5 * n + pix] = f4(f0, 0.0f);
+        if (valid) {
+            fr.gpos[pix] = f4(x_view, 1.0f);
+            fr.gnrm[pix] = f4(n_view, 0.0f);
+            fr.gmatuv[pix] = make_float4((float)mat_idx, uv_u, uv_v, 0.0f);
+            const float cu = ((float)x + 0.5f) / (float)fr.width;
+            const float cv = ((float)y + 0.5f) / (float)fr.height;
+            float vp[4], cp[4];
+            mat_vec(ubo.viewPrev, x_view.x, x_view.y, x_view.z, 1.0f, vp);
+            mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
+            const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
+            const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
+            const uint32_t mx = f32_to_f16_bits(cu - pu), my = f32_to_f16_bits(cv - pv);
+            fr.motion[pix] = mx | (my << 16);
+        } else {
+            fr.gpos[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+            fr.gnrm[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+            fr.gmatuv[pix] = make_float4(-1.f, 0.f, 0.f, 0.f);
+            fr.motion[pix] = 0u;
+        }
+    }
+    count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// ---------------------------------------------------------------------------
+// Reservoir helpers (ReSTIRGICommon.hlsl:32-78)
+// ---------------------------------------------------------------------------
+struct Res {
+    float4 p[kPlanes];  // planes as in xr_layout.h (w of 3,4,5 = w_sum, W, M)
+};
+
+__device__ __forceinline__ void res_init(Res* r) {
+    r->p[0] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0xFFFFFFFFu));
+    r->p[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r->p[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r->p[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r->p[4] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r->p[5] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float r_wsum(const Res& r) { return r.p[3].w; }
+__device__ __forceinline__ float r_W(const Res& r) { return r.p[4].w; }
+__device__ __forceinline__ uint32_t r_m(const Res& r) { return __float_as_uint(r.p[5].w); }
+__device__ __forceinline__ void set_wsum(Res* r, float v) { r->p[3].w = v; }
+__device__ __forceinline__ void set_W(Res* r, float v) { r->p[4].w = v; }
+__device__ __forceinline__ void set_m(Res* r, uint32_t v) { r->p[5].w = __uint_as_float(v); }
+// r.s = src (sample planes 0-2 whole, xyz of 3-5)
+__device__ __forceinline__ void take_sample(Res* r, const float4* s) {
+    r->p[0] = s[0];
+    r->p[1] = s[1];
+    r->p[2] = s[2];
+    r->p[3] = make_float4(s[3].x, s[3].y, s[3].z, r->p[3].w);
+    r->p[4] = make_float4(s[4].x, s[4].y, s[4].z, r->p[4].w);
+    r->p[5] = make_float4(s[5].x, s[5].y, s[5].z, r->p[5].w);
+}
+// UpdateReservoir: w_sum += w; m++; one draw; take the sample if xi < w/max(1e-10, w_sum).
+__device__ __forceinline__ void res_update(Res* r, const float4* s, float w, uint32_t* seed) {
+    const float ws = r_wsum(*r) + w;
+    set_wsum(r, ws);
+    set_m(r, r_m(*r) + 1u);
+    if (rand01(seed) < w / maxf(1e-10f, ws)) take_sample(r, s);
+}
+// MergeReservoir: w = p_hat * src.W * src.m; update; m = M0 + src.m.
+__device__ __forceinline__ void res_merge(Res* r, const Res& src, float p_hat, uint32_t* seed) {
+    const uint32_t m0 = r_m(*r);
+    const float w = p_hat * r_W(src) * (float)r_m(src);
+    res_update(r, src.p, w, seed);
+    set_m(r, m0 + r_m(src));
+}
+__device__ __forceinline__ void load_planes(const float4* base, uint32_t n, uint32_t i, float4* out) {
+#pragma unroll
+    for (int k = 0; k < kPlanes; ++k) out[k] = base[(size_t)k * n + i];
+}
+__device__ __forceinline__ void store_planes(float4* base, uint32_t n, uint32_t i, const float4* in) {
+#pragma unroll
+    for (int k = 0; k < kPlanes; ++k) base[(size_t)k * n + i] = in[k];
+}
+
+// EvaluateTargetFunction (Temporal.rgen:33-40): |L| if the sample lies in
+// front of the current surface, else 0.
+__device__ __forceinline__ float target_fn(const float4* s, v3 x_curr, v3 n_curr) {
+    const v3 wi = normalize(sub(xyz(s[2]), x_curr));
+    const float c = dot(n_curr, wi);
+    if (c <= 0.0f) return 0.0f;
+    return length(xyz(s[4]));
+}
+
+// Final shading of a resampled reservoir: L * W * f * max(0, n.wi).
+__device__ __forceinline__ v3 shade_reservoir(const Res& r, v3 x_curr, v3 n_curr) {
+    const v3 wi = normalize(sub(xyz(r.p[2]), x_curr));
+    const float c = maxf(0.0f, dot(n_curr, wi));
+    const float W = r_W(r);
+    const v3 Lo = xyz(r.p[4]), f = xyz(r.p[5]);
+    return mk3(Lo.x * W * f.x * c, Lo.y * W * f.y * c, Lo.z * W * f.z * c);
+}
+
+// ---------------------------------------------------------------------------
+// Temporal reuse pass (HBM bound; no rays)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, PassParams pp,
+                                                  const float4* __restrict__ gpos_hist) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= fr.n) return;
+    const uint32_t n = fr.n, w = fr.width, h = fr.height;
+    const uint32_t x = pix % w, y = pix / w;
+    uint32_t rng = init_seed(x, y, w, frame);
+    const uint32_t ping = frame % 2u;
+    float4* cur = fr.temporal + (size_t)ping * kPlanes * n;
+    const float4* prev = fr.temporal + (size_t)(1u - ping) * kPlanes * n;
+    float4* spat_cur = fr.spatial + (size_t)ping * kPlanes * n;
+
+    float4 s[kPlanes];
+    load_planes(fr.samples, n, pix, s);
+    if (length(xyz(s[3])) == 0.0f) {
+        Res r0;
+        res_init(&r0);
+        store_planes(cur, n, pix, r0.p);
+        store_planes(spat_cur, n, pix, r0.p);
+        return;
+    }
+    bool found = false;
+    Res rp;
+    const uint32_t mbits = fr.motion[pix];
+    const float mx = f16_bits_to_f32((uint16_t)(mbits & 0xFFFFu));
+    const float my = f16_bits_to_f32((uint16_t)(mbits >> 16));
+    const float u = ((float)x + 0.5f) / (float)w;
+    const float v = ((float)y + 0.5f) / (float)h;
+    const float pu = u - mx, pv = v - my;
+    const int px = (int)(pu * (float)w);
+    const int py = (int)(pv * (float)h);
+    if (px >= 0 && px < (int)w && py >= 0 && py < (int)h) {
+        const uint32_t pidx = (uint32_t)py * w + (uint32_t)px;
+        if (gpos_hist[pidx].w != 0.0f) {
+            load_planes(prev, n, pidx, rp.p);
+            found = true;
+        } else {
+            Res r0;
+            res_init(&r0);
+            store_planes(spat_cur, n, pix, r0.p);
+        }
+    }
+    const v3 x_curr = xyz(s[0]);
+    const v3 n_curr = normalize(xyz(s[1]));
+    Res r;
+    res_init(&r);
+    const float tp_new = target_fn(s, x_curr, n_curr);
+    const float w_new = s[2].w > 1e-6f ? tp_new / s[2].w : 0.0f;
+    res_update(&r, s, w_new, &rng);
+    if (found) {
+        const float tp_prev = target_fn(rp.p, x_curr, n_curr);
+        res_merge(&r, rp, tp_prev, &rng);
+    } else {
+        set_m(&r, 1u);
+    }
+    const uint32_t cap = pp.temporal_m_cap;
+    if (r_m(r) > cap) {
+        set_wsum(&r, r_wsum(r) * ((float)cap / (float)r_m(r)));
+        set_m(&r, cap);
+    }
+    const float tp_final = target_fn(r.p, x_curr, n_curr);
+    if (tp_final > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * tp_final));
+    else set_W(&r, 0.0f);
+    store_planes(cur, n, pix, r.p);
+    v3 color = mk3(0.f, 0.f, 0.f);
+    if (r_W(r) > 0.0f) color = shade_reservoir(r, x_curr, xyz(s[1]));
+    fr.output[pix] = resolve_bgra8(color, 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Spatial reuse pass
+// ---------------------------------------------------------------------------
+// IsSimilar (ReSTIRGICommon.hlsl:99-109).
+__device__ __forceinline__ bool similar(float4 cx, float4 cn, float4 hx, float4 hn) {
+    const v3 d = sub(xyz(cx), xyz(hx));
+    if (dot(d, d) > 0.1f) return false;
+    if (dot(xyz(cn), xyz(hn)) < 0.8f) return false;
+    if (__float_as_uint(cx.w) != __float_as_uint(hx.w)) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame, uint32_t reuse,
+                                                    PassParams pp) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image) {
+        const uint32_t n = fr.n, w = fr.width, h = fr.height;
+        const uint32_t x = tc.x, y = tc.y;
+        const uint32_t pix = y * w + x;
+        uint32_t rng = init_seed(x, y, w, frame);
+        const uint32_t ping = frame % 2u;
+        const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
+        float4* wr = fr.spatial + (size_t)(1u - ping) * kPlanes * n;
+        const float4 s0 = fr.samples[pix];
+        const float4 s1 = fr.samples[n + pix];
+        const float4 s3 = fr.samples[3 * n + pix];
+        if (length(xyz(s1)) == 0.0f) {
+            fr.output[pix] = 0xFF000000u;
+        } else if (length(xyz(s3)) == 0.0f) {
+            fr.output[pix] = 0xFFFFFFFFu;
+        } else {
+            Res center;
+            load_planes(rd, n, pix, center.p);
+            Res r;
+            res_init(&r);
+            res_merge(&r, center, length(xyz(center.p[4])), &rng);
+            int* col = stack + threadIdx.x;
+            for (uint32_t i = 0; i < pp.spatial_samples; ++i) {
+                const float ra = rand01(&rng) * 2.0f * XR_PI;
+                const float rr = sqrtf(rand01(&rng)) * pp.spatial_radius;
+                float sa, ca;
+                sincos_xr(ra, &sa, &ca);
+                int nx = (int)x + (int)floorf(ca * rr);
+                int ny = (int)y + (int)floorf(sa * rr);
+                nx = nx < 0 ? 0 : (nx > (int)w - 1 ? (int)w - 1 : nx);
+                ny = ny < 0 ? 0 : (ny > (int)h - 1 ? (int)h - 1 : ny);
+                const uint32_t nidx = (uint32_t)ny * w + (uint32_t)nx;
+                const float4 q0 = fr.samples[nidx];
+                const float4 q1 = fr.samples[n + nidx];
+                if (length(xyz(q1)) == 0.0f || !similar(s0, s1, q0, q1)) continue;
+                Res rn;
+                load_planes(rd, n, nidx, rn.p);
+                const v3 xs = xyz(rn.p[2]);
+                const v3 ns = xyz(rn.p[3]);
+                const v3 vo = sub(xyz(q0), xs);
+                const float d2o = dot(vo, vo);
+                const v3 vc = sub(xyz(s0), xs);
+                const float d2c = dot(vc, vc);
+                float jac = 1.0f;
+                if (d2o > 1e-6f && d2c > 1e-6f) {
+                    const v3 dir_o = muls(vo, 1.0f / sqrtf(d2o));
+                    const v3 dir_c = muls(vc, 1.0f / sqrtf(d2c));
+                    const float co = maxf(0.0f, dot(ns, dir_o));
+                    const float cc = maxf(0.0f, dot(ns, dir_c));
+                    if (co > 1e-6f) jac = (cc * d2o) / (co * d2c);
+                }
+                jac = clampf(jac, 0.1f, 10.0f);
+                float ph = length(xyz(rn.p[4])) * jac;
+                if (pp.visibility) {
+                    // CheckVisibility (Spatial.rgen:44-62) from the currently
+                    // selected sample's visible point, any-hit.
+                    const v3 from = xyz(r.p[0]);
+                    const v3 nf = xyz(r.p[1]);
+                    const v3 dv = sub(xs, from);
+                    const float dist = length(dv);
+                    const v3 o = add(from, muls(nf, 0.01f));
+                    const v3 dn = normalize(dv);
+                    const RayPre rp = make_ray_pre(o, dn);
+                    Hit hh;
+                    ++rays;
+                    if (trace_ray<kBlock, true>(sc.nodes, sc.tris, rp, 0.0f, dist - 0.01f, col, &hh)) ph = 0.0f;
+                }
+                res_merge(&r, rn, ph, &rng);
+                if (ph == 0.0f) set_m(&r, r_m(r) - r_m(rn));
+            }
+            if (r_m(r) > pp.spatial_m_cap) set_m(&r, pp.spatial_m_cap);
+            const float phf = length(xyz(r.p[4]));
+            if (phf > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * phf));
+            else set_W(&r, 0.0f);
+            v3 color = mk3(0.f, 0.f, 0.f);
+            if (r_W(r) > 0.0f) color = shade_reservoir(r, xyz(s0), xyz(s1));
+            store_planes(wr, n, pix, r.p);
+            if (reuse) fr.output[pix] = resolve_bgra8(color, 1.0f);
+        }
+    }
+    count_rays(&fr.counters[1], rays, &lds_count);
+}
+
+// ---------------------------------------------------------------------------
+// Plain path tracer (rgen.hlsl:31-151)
+// ---------------------------------------------------------------------------
+// The plain closest-hit program (rhit.hlsl:29-59) returns the same position,
+// normal, UV and material as the ReSTIR one; it does not compute the
+// geometric normal, which the plain loop does not use.
+__global__ __launch_bounds__(kBlock) void k_plain(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+                                                  uint32_t max_depth) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image) {
+        const uint32_t x = tc.x, y = tc.y;
+        const uint32_t pix = y * fr.width + x;
+        uint32_t rng = init_seed(x, y, fr.width, ubo.currentFrame);
+        v3 pixel = mk3(0.f, 0.f, 0.f);
+        int* col = stack + threadIdx.x;
+        for (uint32_t i = 0; i < ubo.spp; ++i) {
+            v3 origin, dir;
+            camera_ray(ubo, x, y, fr.width, fr.height, &rng, &origin, &dir);
+            v3 color = mk3(0.f, 0.f, 0.f);
+            v3 thr = mk3(1.f, 1.f, 1.f);
+            for (uint32_t bounce = 0;; ++bounce) {
+                if (max_depth != 0u && bounce == max_depth) break;
+                const RayPre rp = make_ray_pre(origin, dir);
+                Hit hit;
+                ++rays;
+                Payload pl;
+                bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+                if (found) {
+                    closest_hit(sc, hit, &pl);
+                    found = pl.material != -1;
+                }
+                if (!found) {
+                    const float t = 0.5f * (dir.y + 1.0f);
+                    // lerp(a, b, t) = a + t*(b - a)
+                    const v3 sky = mk3(1.0f + t * (0.5f - 1.0f), 1.0f + t * (0.7f - 1.0f), 1.0f + t * (1.0f - 1.0f));
+                    if (ubo.hasSkyBox) color = add(color, mul(thr, sky));
+                    break;
+                }
+                const float4 m0 = sc.materials[2 * pl.material];
+                const float4 m1 = sc.materials[2 * pl.material + 1];
+                if (__float_as_uint(m1.w) == kDiffuseLight) {
+                    color = add(color, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+                    break;
+                }
+                v3 wi;
+                float pdf, cos_t;
+                const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
+                if (pdf <= 1e-6f) break;
+                thr = mul(thr, divs(muls(f, cos_t), pdf));
+                if (bounce > 3) {
+                    const float p = maxf(thr.x, maxf(thr.y, thr.z));
+                    if (rand01(&rng) > p) break;
+                    thr = muls(thr, 1.0f / p);
+                }
+                const v3 hp = add(origin, muls(dir, pl.t));
+                const v3 od = dot(wi, pl.normal) > 0.0f ? pl.normal : neg(pl.normal);
+                origin = add(hp, muls(od, 0.001f));
+                dir = wi;
+            }
+            pixel = add(pixel, color);
+        }
+        const bool accumulate = ubo.spp != ubo.totalNumberOfSamples;
+        v3 acc = mk3(0.f, 0.f, 0.f);
+        if (accumulate) acc = xyz(fr.accum[pix]);
+        acc = add(acc, pixel);
+        const v3 fin = divs(acc, (float)ubo.totalNumberOfSamples);
+        fr.accum[pix] = f4(acc, 0.0f);
+        fr.output[pix] = resolve_bgra8(fin, 0.0f);
+    }
+    count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// ---------------------------------------------------------------------------
+// Ray batch tracing (parity and microbenchmarks)
+// ---------------------------------------------------------------------------
+template <bool kAnyHit>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __restrict__ rays, uint32_t nrays,
+                                                  float4* __restrict__ hits) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrays) return;
+    const float4 a = rays[2 * i], b = rays[2 * i + 1];
+    const RayPre rp = make_ray_pre(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z));
+    Hit hit;
+    const bool found = trace_ray<kBlock, kAnyHit>(sc.nodes, sc.tris, rp, a.w, b.w, stack + threadIdx.x, &hit);
+    float4 o0, o1;
+    if (found) {
+        const BvhTri tr = sc.tris[hit.idx];
+        o0 = make_float4(hit.t, hit.u, hit.v, 0.0f);
+        o1 = make_float4(tr.v1.w, tr.v2.w, __uint_as_float(hit.id), 0.0f);
+    } else {
+        o0 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+        o1 = make_float4(__uint_as_float(0xFFFFFFFFu), __uint_as_float(0xFFFFFFFFu), __uint_as_float(0xFFFFFFFFu),
+                         0.0f);
+    }
+    hits[2 * i] = o0;
+    hits[2 * i + 1] = o1;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo);
+    return hipGetLastError();
+}
+hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_temporal, dim3((fr.n + 255) / 256), dim3(256), 0, st, fr, frame, pp, gpos_hist);
+    return hipGetLastError();
+}
+hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
+                          const PassParams& pp, hipStream_t st) {
+    hipLaunchKernelGGL(k_spatial, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, frame, reuse,
+                       pp);
+    return hipGetLastError();
+}
+hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_plain, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
+                       max_depth);
+    return hipGetLastError();
+}
+hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,
+                        hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + kBlock - 1) / kBlock);
+    if (any_hit) hipLaunchKernelGGL(k_trace<true>, grid, dim3(kBlock), 0, st, sc, rays, n, hits);
+    else hipLaunchKernelGGL(k_trace<false>, grid, dim3(kBlock), 0, st, sc, rays, n, hits);
+    return hipGetLastError();
+}
+
+}  // namespace xr

@@ -1,0 +1,504 @@
+"""ctypes bindings of the two native libraries.
+
+libxr.so       -- the HIP layer (include/xr_api.h)
+libxr_host.so  -- the C++ host side: synthetic scenes, camera, frame driver (include/xr_host.h)
+
+Loading fails loudly when a library is missing: there is no Python or CPU
+fallback for any computation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+XR_OK = 0
+XR_ERR_INVALID_ARGUMENT = 1
+XR_ERR_DEVICE = 2
+XR_ERR_OUT_OF_MEMORY = 3
+XR_ERR_STATE = 4
+XR_ERR_UNSUPPORTED = 5
+
+XR_MODE_PLAIN = 1
+XR_MODE_RESTIR = 2
+XR_VIS_REFERENCE = 0
+XR_VIS_TRACE = 1
+XR_PASS_INITIAL, XR_PASS_TEMPORAL, XR_PASS_SPATIAL = 1, 2, 4
+XR_BUILDER_LBVH = 0
+
+XR_BUF_ACCUMULATION = 1
+XR_BUF_OUTPUT = 2
+XR_BUF_SAMPLES = 9
+XR_BUF_TEMPORAL = 10
+XR_BUF_SPATIAL = 11
+XR_BUF_GBUF_POS = 12
+XR_BUF_GBUF_NRM = 13
+XR_BUF_GBUF_MATUV = 14
+XR_BUF_MOTION = 15
+XR_LAYOUT_REFERENCE = 0
+XR_LAYOUT_NATIVE = 1
+
+MATERIAL_LAMBERTIAN = 0
+MATERIAL_DIFFUSE_LIGHT = 4
+
+
+class XrVertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("normal", C.c_float * 3), ("tex_coord", C.c_float * 2),
+                ("material_index", C.c_int32)]
+
+
+class XrMaterial(C.Structure):
+    _fields_ = [("base_color", C.c_float * 4), ("base_color_texture_index", C.c_int32), ("fuzziness", C.c_float),
+                ("refraction_index", C.c_float), ("material_model", C.c_uint32)]
+
+
+class XrUbo(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("view_prev", C.c_float * 16), ("projection", C.c_float * 16),
+                ("projection_prev", C.c_float * 16), ("view_inverse", C.c_float * 16),
+                ("projection_inverse", C.c_float * 16), ("spp", C.c_uint32), ("total_number_of_samples", C.c_uint32),
+                ("number_of_bounces", C.c_uint32), ("random_seed", C.c_uint32), ("has_sky_box", C.c_uint32),
+                ("reuse", C.c_uint32), ("current_frame", C.c_uint32)]
+
+
+class XrSceneDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32), ("indices", C.c_void_p),
+                ("index_count", C.c_uint32), ("materials", C.c_void_p), ("material_count", C.c_uint32),
+                ("offsets", C.c_void_p), ("instance_count", C.c_uint32)]
+
+
+class XrRestirParams(C.Structure):
+    _fields_ = [("temporal_m_cap", C.c_uint32), ("spatial_m_cap", C.c_uint32), ("spatial_samples", C.c_uint32),
+                ("spatial_radius", C.c_float), ("visibility", C.c_uint32), ("plain_max_depth", C.c_uint32),
+                ("passes", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+
+
+class XrAccelInfo(C.Structure):
+    _fields_ = [("triangle_count", C.c_uint32), ("node_count", C.c_uint32), ("max_depth", C.c_uint32),
+                ("builder", C.c_uint32), ("build_ms", C.c_float), ("node_bytes", C.c_uint64),
+                ("triangle_bytes", C.c_uint64)]
+
+
+class XrhSettings(C.Structure):
+    _fields_ = [("rendering_mode", C.c_uint32), ("spp", C.c_uint32), ("number_of_bounces", C.c_uint32),
+                ("max_spp", C.c_uint32), ("fov", C.c_float), ("enable_ray_accumulation", C.c_uint32),
+                ("enable_sky_lighting", C.c_uint32), ("enable_reuse", C.c_uint32)]
+
+
+assert C.sizeof(XrVertex) == 36 and C.sizeof(XrMaterial) == 32 and C.sizeof(XrUbo) == 412
+assert C.sizeof(XrRestirParams) == 64
+
+# numpy views of the reference AoS layouts (ReSTIRGICommon.h:8-32)
+SAMPLE_DTYPE = np.dtype([("x_view", "<f4", 3), ("material_id", "<u4"), ("n_view", "<f4", 3), ("rand_seed", "<u4"),
+                         ("x_sample", "<f4", 3), ("p_q", "<f4"), ("n_sample", "<f4", 3), ("pad1", "<u4"),
+                         ("l_out", "<f4", 3), ("pad2", "<u4"), ("f", "<f4", 3), ("pad3", "<u4")])
+RESERVOIR_DTYPE = np.dtype([("w_sum", "<f4"), ("w", "<f4"), ("m", "<u4"), ("pad", "<f4"), ("s", SAMPLE_DTYPE)])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("pad0", "<u4"), ("instance", "<u4"),
+                      ("primitive", "<u4"), ("triangle", "<u4"), ("pad1", "<u4")])
+VERTEX_DTYPE = np.dtype([("pos", "<f4", 3), ("normal", "<f4", 3), ("tex_coord", "<f4", 2), ("material_index", "<i4")])
+MATERIAL_DTYPE = np.dtype([("base_color", "<f4", 4), ("base_color_texture_index", "<i4"), ("fuzziness", "<f4"),
+                           ("refraction_index", "<f4"), ("material_model", "<u4")])
+assert SAMPLE_DTYPE.itemsize == 96 and RESERVOIR_DTYPE.itemsize == 112
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
+
+XR_API_SYMBOLS = [
+    "xr_default_params", "xr_create", "xr_build_accel", "xr_resize", "xr_render", "xr_sync", "xr_destroy",
+    "xr_read_buffer", "xr_write_buffer", "xr_read_image", "xr_buffer_bytes", "xr_trace_rays",
+    "xr_trace_rays_device", "xr_pass_times", "xr_counters", "xr_reset_counters", "xr_get_accel_info",
+    "xr_read_accel", "xr_stream",
+    "xr_last_error", "xr_global_last_error",
+]
+XR_HOST_SYMBOLS = [
+    "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_desc", "xrh_scene_triangle_count", "xrh_scene_destroy",
+    "xrh_camera_static", "xrh_camera_orbit", "xrh_camera_view", "xrh_camera_destroy", "xrh_perspective",
+    "xrh_inverse", "xrh_app_create", "xrh_app_draw_frame", "xrh_app_last_ubo", "xrh_app_ctx", "xrh_app_last_error",
+    "xrh_app_destroy", "xrh_make_ubo",
+]
+
+_lib = None
+_host = None
+
+
+def _load(name: str) -> C.CDLL:
+    path = LIB_DIR / name
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build the native libraries first (python -m xvk_ray_amd.build)")
+    return C.CDLL(str(path))
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        L = _load("libxr.so")
+        P, U32, I = C.c_void_p, C.c_uint32, C.c_int
+        L.xr_default_params.argtypes = [C.POINTER(XrRestirParams)]
+        L.xr_default_params.restype = None
+        L.xr_create.argtypes = [C.POINTER(XrSceneDesc), U32, U32, C.POINTER(P)]
+        L.xr_build_accel.argtypes = [P, I]
+        L.xr_resize.argtypes = [P, U32, U32]
+        L.xr_render.argtypes = [P, C.POINTER(XrUbo), I, C.POINTER(XrRestirParams)]
+        L.xr_sync.argtypes = [P]
+        L.xr_destroy.argtypes = [P]
+        L.xr_destroy.restype = None
+        L.xr_read_buffer.argtypes = [P, I, P, C.c_size_t, I]
+        L.xr_write_buffer.argtypes = [P, I, P, C.c_size_t, I]
+        L.xr_read_image.argtypes = [P, P, C.c_size_t]
+        L.xr_buffer_bytes.argtypes = [P, I, I]
+        L.xr_buffer_bytes.restype = C.c_size_t
+        L.xr_trace_rays.argtypes = [P, P, U32, P, I]
+        L.xr_trace_rays_device.argtypes = [P, P, U32, P, I]
+        L.xr_pass_times.argtypes = [P, C.POINTER(C.c_float), U32]
+        L.xr_counters.argtypes = [P, C.POINTER(C.c_uint64), U32]
+        L.xr_reset_counters.argtypes = [P]
+        L.xr_get_accel_info.argtypes = [P, C.POINTER(XrAccelInfo)]
+        L.xr_read_accel.argtypes = [P, P, C.c_size_t, P, C.c_size_t]
+        L.xr_stream.argtypes = [P]
+        L.xr_stream.restype = P
+        L.xr_last_error.argtypes = [P]
+        L.xr_last_error.restype = C.c_char_p
+        L.xr_global_last_error.argtypes = []
+        L.xr_global_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def host() -> C.CDLL:
+    global _host
+    if _host is None:
+        lib()  # libxr_host links against libxr
+        H = _load("libxr_host.so")
+        P, U32, U64, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
+        H.xrh_scene_cornell.argtypes = [U64, C.POINTER(P)]
+        H.xrh_scene_terrain.argtypes = [U64, U32, U32, U32, C.POINTER(P)]
+        H.xrh_scene_desc.argtypes = [P, C.POINTER(XrSceneDesc)]
+        H.xrh_scene_triangle_count.argtypes = [P]
+        H.xrh_scene_triangle_count.restype = U32
+        H.xrh_scene_destroy.argtypes = [P]
+        H.xrh_scene_destroy.restype = None
+        H.xrh_camera_static.argtypes = [C.POINTER(F), C.POINTER(F), C.POINTER(P)]
+        H.xrh_camera_orbit.argtypes = [U64, U32, U32, C.POINTER(P)]
+        H.xrh_camera_view.argtypes = [P, U32, C.POINTER(F)]
+        H.xrh_camera_destroy.argtypes = [P]
+        H.xrh_camera_destroy.restype = None
+        H.xrh_perspective.argtypes = [F, F, F, F, C.POINTER(F)]
+        H.xrh_perspective.restype = None
+        H.xrh_inverse.argtypes = [C.POINTER(F), C.POINTER(F)]
+        H.xrh_app_create.argtypes = [C.POINTER(XrhSettings), P, P, U32, U32, C.POINTER(P)]
+        H.xrh_app_draw_frame.argtypes = [P, C.POINTER(XrRestirParams)]
+        H.xrh_app_last_ubo.argtypes = [P, C.POINTER(XrUbo)]
+        H.xrh_app_ctx.argtypes = [P]
+        H.xrh_app_ctx.restype = P
+        H.xrh_app_last_error.argtypes = [P]
+        H.xrh_app_last_error.restype = C.c_char_p
+        H.xrh_app_destroy.argtypes = [P]
+        H.xrh_app_destroy.restype = None
+        H.xrh_make_ubo.argtypes = [C.POINTER(F), C.POINTER(F), F, U32, U32, U32, U32, U32, U32, U32, U32,
+                                   C.POINTER(XrUbo)]
+        H.xrh_make_ubo.restype = None
+        _host = H
+    return _host
+
+
+class XrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"xr error {code}: {msg}")
+        self.code = code
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def default_params(**overrides) -> XrRestirParams:
+    p = XrRestirParams()
+    lib().xr_default_params(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def ubo_to_bytes(u: XrUbo) -> bytes:
+    return bytes(memoryview(u))
+
+
+class Scene:
+    """A synthetic scene built by the C++ host (xrh_scene_*), or one given as arrays."""
+
+    def __init__(self, handle=None, arrays=None):
+        self._h = handle
+        self._arrays = arrays  # keeps numpy arrays alive for array-made scenes
+
+    @classmethod
+    def cornell(cls, seed: int = 1) -> "Scene":
+        h = C.c_void_p()
+        rc = host().xrh_scene_cornell(seed, C.byref(h))
+        if rc:
+            raise XrError(rc, "xrh_scene_cornell")
+        return cls(handle=h)
+
+    @classmethod
+    def terrain(cls, seed: int = 1, grid: int = 1024, spheres: int = 512, lights: int = 64) -> "Scene":
+        h = C.c_void_p()
+        rc = host().xrh_scene_terrain(seed, grid, spheres, lights, C.byref(h))
+        if rc:
+            raise XrError(rc, "xrh_scene_terrain")
+        return cls(handle=h)
+
+    @classmethod
+    def from_arrays(cls, vertices: np.ndarray, indices: np.ndarray, materials: np.ndarray,
+                    offsets: np.ndarray) -> "Scene":
+        v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+        i = np.ascontiguousarray(indices, dtype=np.uint32)
+        m = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
+        o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
+        return cls(arrays=(v, i, m, o))
+
+    def desc(self) -> XrSceneDesc:
+        d = XrSceneDesc()
+        if self._h is not None:
+            rc = host().xrh_scene_desc(self._h, C.byref(d))
+            if rc:
+                raise XrError(rc, "xrh_scene_desc")
+        else:
+            v, i, m, o = self._arrays
+            d.vertices, d.vertex_count = v.ctypes.data, len(v)
+            d.indices, d.index_count = i.ctypes.data, len(i)
+            d.materials, d.material_count = m.ctypes.data, len(m)
+            d.offsets, d.instance_count = o.ctypes.data, len(o) // 2
+        return d
+
+    def arrays(self):
+        """(vertices, indices, materials, offsets) as numpy views/copies."""
+        if self._arrays is not None:
+            return self._arrays
+        d = self.desc()
+        v = np.ctypeslib.as_array((C.c_uint8 * (36 * d.vertex_count)).from_address(d.vertices)).view(VERTEX_DTYPE)
+        i = np.ctypeslib.as_array((C.c_uint32 * d.index_count).from_address(d.indices))
+        m = np.ctypeslib.as_array((C.c_uint8 * (32 * d.material_count)).from_address(d.materials)).view(
+            MATERIAL_DTYPE)
+        o = np.ctypeslib.as_array((C.c_uint32 * (2 * d.instance_count)).from_address(d.offsets))
+        return v.copy(), i.copy(), m.copy(), o.copy()
+
+    @property
+    def triangle_count(self) -> int:
+        return self.desc().index_count // 3
+
+    def close(self):
+        if self._h is not None:
+            host().xrh_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Camera:
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def static(cls, eye, target) -> "Camera":
+        e = np.asarray(eye, np.float32)
+        t = np.asarray(target, np.float32)
+        h = C.c_void_p()
+        rc = host().xrh_camera_static(_fptr(e), _fptr(t), C.byref(h))
+        if rc:
+            raise XrError(rc, "xrh_camera_static")
+        return cls(h)
+
+    @classmethod
+    def orbit(cls, seed: int = 2, n_ctrl: int = 8, frames: int = 128) -> "Camera":
+        h = C.c_void_p()
+        rc = host().xrh_camera_orbit(seed, n_ctrl, frames, C.byref(h))
+        if rc:
+            raise XrError(rc, "xrh_camera_orbit")
+        return cls(h)
+
+    def view(self, k: int) -> np.ndarray:
+        m = np.zeros(16, np.float32)
+        host().xrh_camera_view(self._h, k, _fptr(m))
+        return m
+
+    def close(self):
+        if self._h is not None:
+            host().xrh_camera_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_ubo(view, view_prev, fov, width, height, spp=1, total=1, bounces=4, sky=0, reuse=1, frame=1) -> XrUbo:
+    u = XrUbo()
+    v = np.ascontiguousarray(view, np.float32)
+    vp = np.ascontiguousarray(view_prev, np.float32)
+    host().xrh_make_ubo(_fptr(v), _fptr(vp), fov, width, height, spp, total, bounces, sky, reuse, frame, C.byref(u))
+    return u
+
+
+class Context:
+    """One xr_ctx: scene on the device, frame buffers, acceleration structure."""
+
+    def __init__(self, scene: Scene, width: int, height: int, build: bool = True):
+        self.scene = scene
+        d = scene.desc()
+        h = C.c_void_p()
+        rc = lib().xr_create(C.byref(d), width, height, C.byref(h))
+        if rc:
+            raise XrError(rc, lib().xr_global_last_error().decode())
+        self._h = h
+        self.width, self.height = width, height
+        if build:
+            self.check(lib().xr_build_accel(self._h, XR_BUILDER_LBVH))
+
+    def check(self, rc: int) -> None:
+        if rc:
+            raise XrError(rc, lib().xr_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def render(self, ubo: XrUbo, mode: int = XR_MODE_RESTIR, params: XrRestirParams | None = None) -> None:
+        self.check(lib().xr_render(self._h, C.byref(ubo), mode, C.byref(params) if params is not None else None))
+
+    def sync(self) -> None:
+        self.check(lib().xr_sync(self._h))
+
+    def resize(self, w: int, h: int) -> None:
+        self.check(lib().xr_resize(self._h, w, h))
+        self.width, self.height = w, h
+
+    def buffer_bytes(self, which: int, layout: int) -> int:
+        return lib().xr_buffer_bytes(self._h, which, layout)
+
+    def read(self, which: int, layout: int = XR_LAYOUT_REFERENCE) -> np.ndarray:
+        n = self.buffer_bytes(which, layout)
+        out = np.empty(n, np.uint8)
+        self.check(lib().xr_read_buffer(self._h, which, out.ctypes.data, n, layout))
+        if layout == XR_LAYOUT_REFERENCE:
+            if which == XR_BUF_SAMPLES:
+                return out.view(SAMPLE_DTYPE)
+            if which in (XR_BUF_TEMPORAL, XR_BUF_SPATIAL):
+                return out.view(RESERVOIR_DTYPE)
+        if which in (XR_BUF_OUTPUT, XR_BUF_MOTION):
+            return out.view(np.uint32)
+        if which in (XR_BUF_ACCUMULATION, XR_BUF_GBUF_POS, XR_BUF_GBUF_NRM, XR_BUF_GBUF_MATUV):
+            return out.view(np.float32).reshape(-1, 4)
+        return out.view(np.float32)
+
+    def write(self, which: int, data: np.ndarray, layout: int = XR_LAYOUT_REFERENCE) -> None:
+        a = np.ascontiguousarray(data)
+        self.check(lib().xr_write_buffer(self._h, which, a.ctypes.data, a.nbytes, layout))
+
+    def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        r = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(r), HIT_DTYPE)
+        self.check(lib().xr_trace_rays(self._h, r.ctypes.data, len(r), hits.ctypes.data, int(any_hit)))
+        return hits
+
+    def pass_times(self) -> list[float]:
+        ms = (C.c_float * 4)()
+        n = lib().xr_pass_times(self._h, ms, 4)
+        if n < 0:
+            self.check(n)
+        return [ms[i] for i in range(n)]
+
+    def counters(self) -> tuple[int, int]:
+        c = (C.c_uint64 * 2)()
+        self.check(lib().xr_counters(self._h, c, 2))
+        return int(c[0]), int(c[1])
+
+    def reset_counters(self) -> None:
+        self.check(lib().xr_reset_counters(self._h))
+
+    def accel_info(self) -> XrAccelInfo:
+        info = XrAccelInfo()
+        self.check(lib().xr_get_accel_info(self._h, C.byref(info)))
+        return info
+
+    def read_accel(self):
+        """(nodes, triangles): nodes as (n, 16) float32 with links in columns 12-13 (int32 view),
+        triangles as (m, 12) float32 (w columns hold id / instance / primitive bits)."""
+        info = self.accel_info()
+        nodes = np.zeros((info.node_count, 16), np.float32)
+        tris = np.zeros((info.triangle_count, 12), np.float32)
+        self.check(lib().xr_read_accel(self._h, nodes.ctypes.data, nodes.nbytes, tris.ctypes.data, tris.nbytes))
+        return nodes, tris
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().xr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class App:
+    """The C++ frame driver (xrh_app): PathTracer::DrawFrame without a window."""
+
+    def __init__(self, scene: Scene, camera: Camera, width: int, height: int, mode: int = XR_MODE_RESTIR,
+                 spp: int = 1, bounces: int = 4, max_spp: int = 1000, fov: float = 45.0, accumulate: bool = False,
+                 sky: bool = False, reuse: bool = True):
+        self.scene, self.camera = scene, camera
+        s = XrhSettings(mode, spp, bounces, max_spp, fov, int(accumulate), int(sky), int(reuse))
+        h = C.c_void_p()
+        rc = host().xrh_app_create(C.byref(s), scene._h if scene._h is not None else None, camera._h, width, height,
+                                   C.byref(h))
+        if rc:
+            raise XrError(rc, lib().xr_global_last_error().decode())
+        self._h = h
+        self.width, self.height = width, height
+
+    @property
+    def ctx_handle(self):
+        return host().xrh_app_ctx(self._h)
+
+    def draw_frame(self, params: XrRestirParams | None = None) -> None:
+        rc = host().xrh_app_draw_frame(self._h, C.byref(params) if params is not None else None)
+        if rc:
+            raise XrError(rc, host().xrh_app_last_error(self._h).decode())
+
+    def last_ubo(self) -> XrUbo:
+        u = XrUbo()
+        host().xrh_app_last_ubo(self._h, C.byref(u))
+        return u
+
+    def context(self) -> "BorrowedContext":
+        return BorrowedContext(self.ctx_handle, self.width, self.height)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            host().xrh_app_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BorrowedContext(Context):
+    """A Context view of a context owned by an App (not destroyed by this object)."""
+
+    def __init__(self, handle, width, height):
+        self._h = handle
+        self.width, self.height = width, height
+        self.scene = None
+
+    def close(self):
+        self._h = None
