@@ -233,6 +233,13 @@ int xr_trace_rays_device(xr_ctx* ctx, const xr_ray* d_rays, uint32_t n, xr_hit* 
  * {plain}.  Returns the number written.  Waits for the frame. */
 int xr_pass_times(xr_ctx* ctx, float* ms, uint32_t n);
 
+/* Per-pass milliseconds summed over every frame rendered since the last
+ * xr_reset_pass_times (same pass order), and the number of frames.  Each
+ * frame's events are kept until read, so a run of frames is timed without a
+ * host sync per frame.  Waits for the frames. */
+int xr_pass_time_totals(xr_ctx* ctx, double* ms_sum, uint32_t n, uint32_t* frames);
+int xr_reset_pass_times(xr_ctx* ctx);
+
 /* Device counters since xr_create / the last reset: [0] extend rays
  * (camera + bounce), [1] shadow rays.  Waits for queued work. */
 int xr_counters(xr_ctx* ctx, uint64_t* out, uint32_t n);

@@ -91,6 +91,8 @@ class OracleScene:
         self.o = np.ascontiguousarray(offsets, np.uint32).reshape(-1)
         self.h = self.L.orc_scene_create(self.v.ctypes.data, len(self.v), self.i.ctypes.data, len(self.i),
                                          self.m.ctypes.data, len(self.m), self.o.ctypes.data, len(self.o) // 2)
+        if not self.h:
+            raise ValueError("oracle rejected the scene: an index is out of range for its instance")
 
     def close(self):
         if self.h:

@@ -336,6 +336,11 @@ struct Scene {
     };
     std::vector<Node> nodes;
     std::vector<uint32_t> order;
+    // Box padding: the FP32 watertight test may report a hit a few ulps
+    // outside a triangle's exact box, so every box is widened by
+    // max(2^-16 * extent, 2^-20 * max|coordinate|) (far above that error for
+    // origins inside the scene); brute force needs no boxes at all.
+    double pad = 0.0;
 };
 
 void tri_verts(const Scene& s, uint32_t g, float3* p) {
@@ -361,6 +366,10 @@ int build_node(Scene& s, std::vector<double>& cent, int begin, int end) {
             clo[a] = std::min(clo[a], cent[3 * (size_t)g + a]);
             chi[a] = std::max(chi[a], cent[3 * (size_t)g + a]);
         }
+    }
+    for (int a = 0; a < 3; ++a) {
+        nd.lo[a] -= s.pad;
+        nd.hi[a] += s.pad;
     }
     nd.left = nd.right = -1;
     nd.begin = begin;
@@ -982,6 +991,12 @@ void orc_merge_reservoir(void* r, const void* src, float p_hat_, uint32_t* seed)
 // ---- scene ----
 void* orc_scene_create(const void* vertices, uint32_t vcount, const uint32_t* indices, uint32_t icount,
                        const void* materials, uint32_t mcount, const uint32_t* offsets, uint32_t ninst) {
+    // reject what the reference would read out of bounds
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t end = i + 1 < ninst ? offsets[2 * (i + 1)] : icount;
+        for (uint32_t k = offsets[2 * i]; k < end; ++k)
+            if ((uint64_t)offsets[2 * i + 1] + indices[k] >= vcount) return nullptr;
+    }
     Scene* s = new Scene();
     s->vertices.assign((const Vertex*)vertices, (const Vertex*)vertices + vcount);
     s->indices.assign(indices, indices + icount);
@@ -1008,6 +1023,19 @@ void* orc_scene_create(const void* vertices, uint32_t vcount, const uint32_t* in
                 cent[3 * (size_t)g + a] = ((double)s->tv[9 * (size_t)g + a] + s->tv[9 * (size_t)g + 3 + a] +
                                            s->tv[9 * (size_t)g + 6 + a]) / 3.0;
         }
+    }
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (size_t k = 0; k < s->tv.size(); ++k) {
+            lo[k % 3] = std::min(lo[k % 3], (double)s->tv[k]);
+            hi[k % 3] = std::max(hi[k % 3], (double)s->tv[k]);
+        }
+        double ext = 0, mag = 0;
+        for (int a = 0; a < 3; ++a) {
+            ext = std::max(ext, hi[a] - lo[a]);
+            mag = std::max(mag, std::max(fabs(lo[a]), fabs(hi[a])));
+        }
+        s->pad = std::max(ext / 65536.0, mag / 1048576.0);
     }
     s->order.resize(ntri);
     for (uint32_t g = 0; g < ntri; ++g) s->order[g] = g;

@@ -72,9 +72,21 @@ struct xr_ctx {
     uint32_t* d_output = nullptr;
     unsigned long long* d_counters = nullptr;
 
-    // timing of the last frame
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int n_pass_events = 0;
+    // Per-pass timing: a ring of event sets, one per recorded frame.  A set is
+    // harvested into the running totals when it is about to be reused or when
+    // the totals are read, so timing a run of frames needs no host sync per
+    // frame.
+    struct EventSet {
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        int n = 0;           // events recorded (2: plain, 4: ReSTIR)
+        bool pending = false;
+    };
+    static constexpr int kRing = 8;
+    EventSet ring[kRing];
+    int ring_cursor = 0;
+    int last_set = -1;
+    double pass_sum_ms[3] = {0, 0, 0};
+    uint32_t pass_frames = 0;
 
     // staging for ray batches
     float4* d_rays = nullptr;
@@ -196,6 +208,20 @@ FrameDev frame_dev(const xr_ctx* c) {
     return f;
 }
 
+// Folds a recorded event set into the per-pass totals (waits for it).
+int harvest(xr_ctx* c, xr_ctx::EventSet& set) {
+    if (!set.pending) return XR_OK;
+    XR_HIP(c, hipEventSynchronize(set.ev[set.n - 1]));
+    for (int i = 0; i + 1 < set.n; ++i) {
+        float t = 0.f;
+        XR_HIP(c, hipEventElapsedTime(&t, set.ev[i], set.ev[i + 1]));
+        c->pass_sum_ms[i] += t;
+    }
+    c->pass_frames++;
+    set.pending = false;
+    return XR_OK;
+}
+
 int check_params(xr_ctx* c, const xr_restir_params* p) {
     for (int i = 0; i < 9; ++i)
         if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
@@ -243,8 +269,9 @@ void xr_destroy(xr_ctx* c) {
     dev_free(&c->d_counters);
     dev_free(&c->d_rays);
     dev_free(&c->d_hits);
-    for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
+    for (auto& set : c->ring)
+        for (auto& e : set.ev)
+            if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -309,7 +336,8 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     } while (0)
     XR_CREATE_HIP(hipSetDevice(0));
     XR_CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto& ev : c->ev) XR_CREATE_HIP(hipEventCreate(&ev));
+    for (auto& set : c->ring)
+        for (auto& ev : set.ev) XR_CREATE_HIP(hipEventCreate(&ev));
     c->vertex_count = s->vertex_count;
     c->index_count = s->index_count;
     c->material_count = s->material_count;
@@ -419,23 +447,31 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
     const SceneDev sc = scene_dev(c);
     const FrameDev fr = frame_dev(c);
     if (fault_inject("launch")) return set_err(c, XR_ERR_DEVICE, "injected launch failure");
+    if (mode != XR_MODE_PLAIN && mode != XR_MODE_RESTIR)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown mode %d", mode);
+    xr_ctx::EventSet& set = c->ring[c->ring_cursor];
+    rc = harvest(c, set);
+    if (rc) return rc;
+    hipEvent_t* ev = set.ev;
     if (mode == XR_MODE_PLAIN) {
-        XR_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        XR_HIP(c, hipEventRecord(ev[0], c->stream));
         XR_HIP(c, launch_plain(sc, fr, u, p.plain_max_depth, c->stream));
-        XR_HIP(c, hipEventRecord(c->ev[1], c->stream));
-        c->n_pass_events = 2;
-        return XR_OK;
+        XR_HIP(c, hipEventRecord(ev[1], c->stream));
+        set.n = 2;
+    } else {
+        const uint32_t passes = p.passes ? p.passes : 7u;
+        XR_HIP(c, hipEventRecord(ev[0], c->stream));
+        if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, c->stream));
+        XR_HIP(c, hipEventRecord(ev[1], c->stream));
+        if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
+        XR_HIP(c, hipEventRecord(ev[2], c->stream));
+        if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+        XR_HIP(c, hipEventRecord(ev[3], c->stream));
+        set.n = 4;
     }
-    if (mode != XR_MODE_RESTIR) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown mode %d", mode);
-    const uint32_t passes = p.passes ? p.passes : 7u;
-    XR_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, c->stream));
-    XR_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
-    XR_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
-    XR_HIP(c, hipEventRecord(c->ev[3], c->stream));
-    c->n_pass_events = 4;
+    set.pending = true;
+    c->last_set = c->ring_cursor;
+    c->ring_cursor = (c->ring_cursor + 1) % xr_ctx::kRing;
     return XR_OK;
 }
 
@@ -447,15 +483,38 @@ int xr_sync(xr_ctx* c) {
 
 int xr_pass_times(xr_ctx* c, float* ms, uint32_t n) {
     if (!c || !ms) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (c->n_pass_events < 2) return 0;
-    XR_HIP(c, hipEventSynchronize(c->ev[c->n_pass_events - 1]));
+    if (c->last_set < 0) return 0;
+    const xr_ctx::EventSet& set = c->ring[c->last_set];
+    XR_HIP(c, hipEventSynchronize(set.ev[set.n - 1]));
     int k = 0;
-    for (int i = 0; i + 1 < c->n_pass_events && (uint32_t)k < n; ++i) {
+    for (int i = 0; i + 1 < set.n && (uint32_t)k < n; ++i) {
         float t = 0.f;
-        XR_HIP(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+        XR_HIP(c, hipEventElapsedTime(&t, set.ev[i], set.ev[i + 1]));
         ms[k++] = t;
     }
     return k;
+}
+
+int xr_pass_time_totals(xr_ctx* c, double* ms_sum, uint32_t n, uint32_t* frames) {
+    if (!c || !ms_sum) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (auto& set : c->ring) {
+        int rc = harvest(c, set);
+        if (rc) return rc;
+    }
+    for (uint32_t i = 0; i < n && i < 3; ++i) ms_sum[i] = c->pass_sum_ms[i];
+    if (frames) *frames = c->pass_frames;
+    return XR_OK;
+}
+
+int xr_reset_pass_times(xr_ctx* c) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    for (auto& set : c->ring) {
+        int rc = harvest(c, set);
+        if (rc) return rc;
+    }
+    c->pass_sum_ms[0] = c->pass_sum_ms[1] = c->pass_sum_ms[2] = 0.0;
+    c->pass_frames = 0;
+    return XR_OK;
 }
 
 int xr_counters(xr_ctx* c, uint64_t* out, uint32_t n) {

@@ -8,11 +8,11 @@
 // into one BVH over every triangle of every instance.
 //
 // Passes (one kernel each, all on the caller's stream):
-//   1. gather   canonical triangles from the reference buffers, centroid bounds
+//   1. gather   canonical triangles from the reference buffers, centroid and vertex bounds
 //   2. morton   63-bit Morton code of each centroid
 //   3. sort     (code, id) pairs, rocPRIM/hipCUB radix sort
 //   4. karras   internal-node topology from the sorted codes
-//   5. refit    bottom-up boxes, one atomic counter per internal node
+//   5. refit    bottom-up boxes (padded, see box_pad), one atomic counter per internal node
 //   6. emit     64-byte two-box nodes and the BVH-ordered triangle array
 #include <hipcub/hipcub.hpp>
 #include <stdio.h>
@@ -32,12 +32,14 @@ struct Box {
     float lo[3], hi[3];
 };
 
+// bounds[0..2] / [3..5]: centroid min / max; bounds[6..8] / [9..11]: vertex min / max
+// (order-preserving int encodings of the floats).
 __global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restrict__ idx,
                          const uint2* __restrict__ offsets, const uint32_t* __restrict__ first_tri,
                          uint32_t ninst, uint32_t ntri, BvhTri* __restrict__ ctris, float4* __restrict__ cent,
                          int* __restrict__ bounds) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    float c[3] = {0.f, 0.f, 0.f};
+    float c[3] = {0.f, 0.f, 0.f}, vlo[3] = {0.f, 0.f, 0.f}, vhi[3] = {0.f, 0.f, 0.f};
     bool live = g < ntri;
     if (live) {
         // instance = last i with first_tri[i] <= g
@@ -60,19 +62,25 @@ __global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restri
         t.v1 = make_float4(p[1][0], p[1][1], p[1][2], __uint_as_float(inst));
         t.v2 = make_float4(p[2][0], p[2][1], p[2][2], __uint_as_float(prim));
         ctris[g] = t;
-        for (int a = 0; a < 3; ++a) c[a] = (p[0][a] + p[1][a] + p[2][a]) * (1.0f / 3.0f);
+        for (int a = 0; a < 3; ++a) {
+            c[a] = (p[0][a] + p[1][a] + p[2][a]) * (1.0f / 3.0f);
+            vlo[a] = fminf(fminf(p[0][a], p[1][a]), p[2][a]);
+            vhi[a] = fmaxf(fmaxf(p[0][a], p[1][a]), p[2][a]);
+        }
         cent[g] = make_float4(c[0], c[1], c[2], 0.f);
     }
-    // block reduction of the centroid bounds, then one atomic per block
-    __shared__ int smin[3][256], smax[3][256];
+    // block reduction of the bounds, then one atomic per block and value
+    __shared__ int smin[6][256], smax[6][256];
     for (int a = 0; a < 3; ++a) {
         smin[a][threadIdx.x] = live ? ord_enc(c[a]) : 0x7FFFFFFF;
         smax[a][threadIdx.x] = live ? ord_enc(c[a]) : (int)0x80000000;
+        smin[3 + a][threadIdx.x] = live ? ord_enc(vlo[a]) : 0x7FFFFFFF;
+        smax[3 + a][threadIdx.x] = live ? ord_enc(vhi[a]) : (int)0x80000000;
     }
     __syncthreads();
     for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            for (int a = 0; a < 3; ++a) {
+            for (int a = 0; a < 6; ++a) {
                 smin[a][threadIdx.x] = min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
                 smax[a][threadIdx.x] = max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
             }
@@ -83,8 +91,26 @@ __global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restri
         for (int a = 0; a < 3; ++a) {
             atomicMin(&bounds[a], smin[a][0]);
             atomicMax(&bounds[3 + a], smax[a][0]);
+            atomicMin(&bounds[6 + a], smin[3 + a][0]);
+            atomicMax(&bounds[9 + a], smax[3 + a][0]);
         }
     }
+}
+
+// Box padding.  The watertight FP32 test is consistent across shared edges
+// but not exact: it may report a hit up to a few ulps of |vertex - origin|
+// outside the triangle, i.e. outside its exact box.  Every box is therefore
+// widened by pad = max(2^-16 * scene extent, 2^-20 * max |coordinate|), well
+// above that error for ray origins within the scene, so no hit the triangle
+// test would report is culled by a box.
+__device__ __forceinline__ float box_pad(const int* __restrict__ bounds) {
+    float ext = 0.0f, mag = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float lo = ord_dec(bounds[6 + a]), hi = ord_dec(bounds[9 + a]);
+        ext = fmaxf(ext, hi - lo);
+        mag = fmaxf(mag, fmaxf(fabsf(lo), fabsf(hi)));
+    }
+    return fmaxf(ext * (1.0f / 65536.0f), mag * (1.0f / 1048576.0f));
 }
 
 __device__ __forceinline__ unsigned long long spread21(uint32_t v) {
@@ -163,17 +189,18 @@ __device__ __forceinline__ float ld_agent(const float* p) {
 // lane to reach an internal node computes its box and walks on.
 __global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
                         const int2* __restrict__ child, const int* __restrict__ parent, Box* boxes,
-                        int* __restrict__ flags, uint32_t* __restrict__ max_depth) {
+                        int* __restrict__ flags, uint32_t* __restrict__ max_depth, const int* __restrict__ bounds) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const BvhTri t = ctris[sorted_ids[j]];
+    const float pad = box_pad(bounds);
     Box b;
-    b.lo[0] = fminf(fminf(t.v0.x, t.v1.x), t.v2.x);
-    b.lo[1] = fminf(fminf(t.v0.y, t.v1.y), t.v2.y);
-    b.lo[2] = fminf(fminf(t.v0.z, t.v1.z), t.v2.z);
-    b.hi[0] = fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x);
-    b.hi[1] = fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y);
-    b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z);
+    b.lo[0] = fminf(fminf(t.v0.x, t.v1.x), t.v2.x) - pad;
+    b.lo[1] = fminf(fminf(t.v0.y, t.v1.y), t.v2.y) - pad;
+    b.lo[2] = fminf(fminf(t.v0.z, t.v1.z), t.v2.z) - pad;
+    b.hi[0] = fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x) + pad;
+    b.hi[1] = fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y) + pad;
+    b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z) + pad;
     int node = (n - 1) + j;
     boxes[node] = b;
     // Depth of this leaf.  A valid tree reaches the root in < n steps; the
@@ -263,8 +290,9 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     BvhNode* nodes = nullptr;
     BvhTri* tris = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const int init_bounds[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
-                                (int)0x80000000};
+    const int init_bounds[12] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
+                                 (int)0x80000000, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000,
+                                 (int)0x80000000, (int)0x80000000};
     const uint32_t blocks = (tri_count + 255) / 256;
     uint32_t host_depth = 0;
     float ms = 0.f;
@@ -277,7 +305,7 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     XR_TRY(hipEventRecord(e0, st));
     XR_TRY(hipMalloc(&ctris, sizeof(BvhTri) * tri_count));
     XR_TRY(hipMalloc(&cent, sizeof(float4) * tri_count));
-    XR_TRY(hipMalloc(&bounds, sizeof(int) * 6));
+    XR_TRY(hipMalloc(&bounds, sizeof(int) * 12));
     XR_TRY(hipMalloc(&keys, sizeof(unsigned long long) * tri_count));
     XR_TRY(hipMalloc(&keys2, sizeof(unsigned long long) * tri_count));
     XR_TRY(hipMalloc(&vals, sizeof(uint32_t) * tri_count));
@@ -306,7 +334,8 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, keys2, n, child, parent);
         XR_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, child, parent, boxes, flags, dmax);
+    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, child, parent, boxes, flags, dmax,
+                       bounds);
     XR_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_emit, dim3((tri_count + 255) / 256), dim3(256), 0, st, n, child, boxes, nodes, ctris,
                        vals2, tris);

@@ -107,7 +107,8 @@ assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
 XR_API_SYMBOLS = [
     "xr_default_params", "xr_create", "xr_build_accel", "xr_resize", "xr_render", "xr_sync", "xr_destroy",
     "xr_read_buffer", "xr_write_buffer", "xr_read_image", "xr_buffer_bytes", "xr_trace_rays",
-    "xr_trace_rays_device", "xr_pass_times", "xr_counters", "xr_reset_counters", "xr_get_accel_info",
+    "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
+    "xr_reset_counters", "xr_get_accel_info",
     "xr_read_accel", "xr_stream",
     "xr_last_error", "xr_global_last_error",
 ]
@@ -151,6 +152,8 @@ def lib() -> C.CDLL:
         L.xr_trace_rays.argtypes = [P, P, U32, P, I]
         L.xr_trace_rays_device.argtypes = [P, P, U32, P, I]
         L.xr_pass_times.argtypes = [P, C.POINTER(C.c_float), U32]
+        L.xr_pass_time_totals.argtypes = [P, C.POINTER(C.c_double), U32, C.POINTER(U32)]
+        L.xr_reset_pass_times.argtypes = [P]
         L.xr_counters.argtypes = [P, C.POINTER(C.c_uint64), U32]
         L.xr_reset_counters.argtypes = [P]
         L.xr_get_accel_info.argtypes = [P, C.POINTER(XrAccelInfo)]
@@ -412,6 +415,15 @@ class Context:
         if n < 0:
             self.check(n)
         return [ms[i] for i in range(n)]
+
+    def pass_time_totals(self) -> tuple[list[float], int]:
+        ms = (C.c_double * 3)()
+        frames = C.c_uint32()
+        self.check(lib().xr_pass_time_totals(self._h, ms, 3, C.byref(frames)))
+        return [ms[i] for i in range(3)], int(frames.value)
+
+    def reset_pass_times(self) -> None:
+        self.check(lib().xr_reset_pass_times(self._h))
 
     def counters(self) -> tuple[int, int]:
         c = (C.c_uint64 * 2)()
