@@ -90,18 +90,34 @@ def cpu_baseline(scene, app, ctx, target_s: float) -> dict:
 
     dt, _, _ = run(4096)
     npix = int(min(n, max(4096, 4096 * target_s / max(dt, 1e-3))))
-    dt, rays, parts = run(npix)
-    ms_frame = dt * 1e3 * n / npix
+    # Below a full frame: one run over npix random pixels.  A full frame that
+    # finishes early is repeated until the target time is reached.
+    total, rays, reps, parts = 0.0, 0, 0, [0.0, 0.0, 0.0]
+    while True:
+        dt, r, p = run(npix)
+        total += dt
+        rays += r
+        reps += 1
+        parts = [a + b for a, b in zip(parts, p)]
+        if npix < n or total >= target_s or reps >= 64:
+            break
+    ms_frame = total * 1e3 * n / (npix * reps)
+    what = (f"{npix} random pixels of one 1920x1080 frame" if npix < n
+            else f"{reps} full 1920x1080 frames")
     return {
-        "value": round(rays / dt / 1e6, 3),
+        "value": round(rays / total / 1e6, 3),
         "unit": "Mrays/s",
         "ms_per_frame": round(ms_frame, 1),
         "cores": threads,
         "kind": "port",
-        "sample": f"{npix} random pixels of one 1920x1080 frame (all three passes, traced visibility), "
-                  f"{dt:.1f} s; C++ restatement of the shaders, not lavapipe",
+        "sample": f"{what} (all three passes, traced visibility), {total:.1f} s; "
+                  f"C++ restatement of the shaders, not lavapipe",
         "pass_seconds": [round(x, 3) for x in parts],
     }
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -122,10 +138,13 @@ def main():
     params = N.default_params(visibility=N.XR_VIS_TRACE, **PARAMS)
     info = ctx.accel_info()
     setup_s = time.perf_counter() - t_setup
+    log(f"scene {info.triangle_count} tris, BVH {info.node_count} nodes depth {info.max_depth} "
+        f"built in {info.build_ms:.1f} ms; setup {setup_s:.1f} s")
 
     for _ in range(args.warmup):
         app.draw_frame(params)
     ctx.sync()
+    log(f"{args.warmup} warm-up frames done")
     ctx.reset_counters()
     ctx.reset_pass_times()
     ctx.sync()
@@ -139,6 +158,7 @@ def main():
     assert frames == args.steps, (frames, args.steps)
 
     ms_per_step = wall * 1e3 / args.steps
+    log(f"{args.steps} timed frames: {ms_per_step:.3f} ms/frame, passes {[round(x / frames, 3) for x in pass_ms]}")
     rays = ext + shadow
     mrays = rays / wall / 1e6
     names = ["initial", "temporal", "spatial"]
@@ -191,6 +211,7 @@ def main():
         "setup_s": round(setup_s, 2),
     }
     if not args.no_cpu_baseline:
+        log("CPU baseline (oracle on a pixel sample)")
         result["cpu_baseline"] = cpu_baseline(scene, app, ctx, args.cpu_seconds)
     print(json.dumps(result))
 

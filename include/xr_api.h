@@ -142,8 +142,14 @@ typedef struct xr_restir_params {
     uint32_t visibility;       /* XR_VIS_* (reference XR_VIS_REFERENCE) */
     uint32_t plain_max_depth;  /* mode 1 bounce cap, 0 = unbounded as rgen.hlsl:59 */
     uint32_t passes;           /* XR_PASS_* mask for mode 2, 0 = all three */
-    uint32_t reserved[9];      /* must be zero */
+    uint32_t flags;            /* XR_FLAG_* */
+    uint32_t reserved[8];      /* must be zero */
 } xr_restir_params;
+
+/* flags: also write the linear (pre-tone-map) colour of every output pixel to
+ * XR_BUF_RADIANCE (0 for the black / white early-out pixels of the spatial
+ * pass), for error measurements against the converged image. */
+#define XR_FLAG_RADIANCE 0x1u
 
 /* Fills p with the reference's compile-time constants (30 / 300 / 6 / 5.0,
  * always-visible, unbounded plain depth, all passes). */
@@ -187,6 +193,8 @@ void xr_destroy(xr_ctx* ctx);
 #define XR_BUF_GBUF_NRM 13    /* binding 13 (= 17), RGBA32F */
 #define XR_BUF_GBUF_MATUV 14  /* binding 14, RGBA32F */
 #define XR_BUF_MOTION 15      /* binding 15, RG16F, 4 B/px */
+#define XR_BUF_RADIANCE 16    /* no binding: linear colour behind the output, RGBA32F (XR_FLAG_RADIANCE) */
+#define XR_BUF_CONVERGED 17   /* no binding: FP64 RGB sums of xr_converge, 24 B/px */
 
 #define XR_LAYOUT_REFERENCE 0 /* AoS exactly as the reference binds it (96 / 112-byte structs) */
 #define XR_LAYOUT_NATIVE 1    /* as stored on the device: six float4 planes per entry */
@@ -266,6 +274,24 @@ int xr_read_accel(xr_ctx* ctx, void* nodes, size_t node_bytes, void* triangles, 
 
 /* HIP stream the context records on (a hipStream_t). */
 void* xr_stream(xr_ctx* ctx);
+
+/* ---- converged image and error (no reference counterpart) ---- */
+
+/* Adds spp samples per pixel of the quantity the ReSTIR output estimates (the
+ * initial sample's one-sample RIS estimate L_out * f * cos / p_q at the
+ * primary hit, 0 where the primary ray misses or sees an emitter) into FP64
+ * per-pixel sums.  Sample s uses frame number frame_base + s for its RNG seed,
+ * so successive calls with disjoint ranges accumulate one progressive image.
+ * ubo supplies camera and bounce count.  Asynchronous. */
+int xr_converge(xr_ctx* ctx, const xr_ubo* ubo, uint32_t spp, uint32_t frame_base);
+/* Clears the sums. */
+int xr_converge_reset(xr_ctx* ctx);
+/* Samples per pixel accumulated so far. */
+uint64_t xr_converged_spp(const xr_ctx* ctx);
+/* Root-mean-square error over pixels and RGB channels of XR_BUF_RADIANCE (the
+ * last frame rendered with XR_FLAG_RADIANCE) against the converged mean,
+ * reduced in FP64 on the device.  Synchronous. */
+int xr_rmse(xr_ctx* ctx, double* rmse);
 
 /* ---- errors ---- */
 const char* xr_last_error(const xr_ctx* ctx);

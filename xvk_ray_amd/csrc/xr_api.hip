@@ -70,7 +70,13 @@ struct xr_ctx {
     float4* d_accum = nullptr;
     uint32_t* d_motion = nullptr;
     uint32_t* d_output = nullptr;
+    float4* d_radiance = nullptr;
     unsigned long long* d_counters = nullptr;
+
+    // converged image: FP64 RGB sums per pixel, samples accumulated, error scratch
+    double* d_conv = nullptr;
+    uint64_t conv_spp = 0;
+    double* d_err = nullptr;
 
     // Per-pass timing: a ring of event sets, one per recorded frame.  A set is
     // harvested into the running totals when it is about to be reused or when
@@ -147,6 +153,9 @@ void free_frame(xr_ctx* c) {
     dev_free(&c->d_accum);
     dev_free(&c->d_motion);
     dev_free(&c->d_output);
+    dev_free(&c->d_radiance);
+    dev_free(&c->d_conv);
+    c->conv_spp = 0;
     c->width = c->height = 0;
 }
 
@@ -163,6 +172,8 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_accum, n * f4));
     XR_HIP(c, dev_alloc(&c->d_motion, n * 4));
     XR_HIP(c, dev_alloc(&c->d_output, n * 4));
+    XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
+    XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_temporal, 0, 2 * kPlanes * n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_spatial, 0, 2 * kPlanes * n * f4, c->stream));
@@ -204,6 +215,7 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.motion = c->d_motion;
     f.output = c->d_output;
     f.accum = c->d_accum;
+    f.radiance = nullptr;
     f.counters = c->d_counters;
     return f;
 }
@@ -223,8 +235,9 @@ int harvest(xr_ctx* c, xr_ctx::EventSet& set) {
 }
 
 int check_params(xr_ctx* c, const xr_restir_params* p) {
-    for (int i = 0; i < 9; ++i)
+    for (int i = 0; i < 8; ++i)
         if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
+    if (p->flags & ~XR_FLAG_RADIANCE) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.flags has unknown bits");
     if (p->visibility > XR_VIS_TRACE) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.visibility %u", p->visibility);
     if (p->spatial_samples > 64) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.spatial_samples > 64");
     if (!(p->spatial_radius >= 0.0f && p->spatial_radius <= 4096.0f))
@@ -269,6 +282,7 @@ void xr_destroy(xr_ctx* c) {
     dev_free(&c->d_counters);
     dev_free(&c->d_rays);
     dev_free(&c->d_hits);
+    dev_free(&c->d_err);
     for (auto& set : c->ring)
         for (auto& e : set.ev)
             if (e) hipEventDestroy(e);
@@ -445,7 +459,8 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
     UniformBufferObject u;
     memcpy(&u, ubo, sizeof(u));
     const SceneDev sc = scene_dev(c);
-    const FrameDev fr = frame_dev(c);
+    FrameDev fr = frame_dev(c);
+    if (p.flags & XR_FLAG_RADIANCE) fr.radiance = c->d_radiance;
     if (fault_inject("launch")) return set_err(c, XR_ERR_DEVICE, "injected launch failure");
     if (mode != XR_MODE_PLAIN && mode != XR_MODE_RESTIR)
         return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown mode %d", mode);
@@ -539,7 +554,9 @@ size_t xr_buffer_bytes(const xr_ctx* c, int which, int layout) {
         case XR_BUF_ACCUMULATION:
         case XR_BUF_GBUF_POS:
         case XR_BUF_GBUF_NRM:
-        case XR_BUF_GBUF_MATUV: return n * 16;
+        case XR_BUF_GBUF_MATUV:
+        case XR_BUF_RADIANCE: return n * 16;
+        case XR_BUF_CONVERGED: return n * 24;
         case XR_BUF_OUTPUT:
         case XR_BUF_MOTION: return n * 4;
         case XR_BUF_SAMPLES: return layout == XR_LAYOUT_REFERENCE ? n * sizeof(ReSTIRGISample) : n * kPlanes * 16;
@@ -561,8 +578,61 @@ static void* buffer_ptr(xr_ctx* c, int which) {
         case XR_BUF_GBUF_NRM: return c->d_gnrm;
         case XR_BUF_GBUF_MATUV: return c->d_gmatuv;
         case XR_BUF_MOTION: return c->d_motion;
+        case XR_BUF_RADIANCE: return c->d_radiance;
+        case XR_BUF_CONVERGED: return c->d_conv;
         default: return nullptr;
     }
+}
+
+static int ensure_conv(xr_ctx* c) {
+    if (c->d_conv) return XR_OK;
+    const size_t bytes = (size_t)c->width * c->height * 3 * sizeof(double);
+    XR_HIP(c, dev_alloc(&c->d_conv, bytes));
+    XR_HIP(c, hipMemsetAsync(c->d_conv, 0, bytes, c->stream));
+    c->conv_spp = 0;
+    return XR_OK;
+}
+
+int xr_converge(xr_ctx* c, const xr_ubo* ubo, uint32_t spp, uint32_t frame_base) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ubo) return set_err(c, XR_ERR_INVALID_ARGUMENT, "ubo is NULL");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
+    int rc = ensure_conv(c);
+    if (rc) return rc;
+    UniformBufferObject u;
+    memcpy(&u, ubo, sizeof(u));
+    // Launch in slices of at most 64 samples so no single kernel runs long.
+    for (uint32_t done = 0; done < spp;) {
+        const uint32_t k = spp - done < 64u ? spp - done : 64u;
+        XR_HIP(c, launch_converge(scene_dev(c), frame_dev(c), u, frame_base + done, k, c->d_conv, c->stream));
+        done += k;
+    }
+    c->conv_spp += spp;
+    return XR_OK;
+}
+
+int xr_converge_reset(xr_ctx* c) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (c->d_conv)
+        XR_HIP(c, hipMemsetAsync(c->d_conv, 0, (size_t)c->width * c->height * 3 * sizeof(double), c->stream));
+    c->conv_spp = 0;
+    return XR_OK;
+}
+
+uint64_t xr_converged_spp(const xr_ctx* c) { return c ? c->conv_spp : 0; }
+
+int xr_rmse(xr_ctx* c, double* rmse) {
+    if (!c || !rmse) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->d_conv || c->conv_spp == 0) return set_err(c, XR_ERR_STATE, "no converged samples (xr_converge)");
+    if (!c->d_err) XR_HIP(c, dev_alloc(&c->d_err, sizeof(double)));
+    const uint32_t n = c->width * c->height;
+    XR_HIP(c, hipMemsetAsync(c->d_err, 0, sizeof(double), c->stream));
+    XR_HIP(c, launch_sq_error(c->d_radiance, c->d_conv, 1.0 / (double)c->conv_spp, n, c->d_err, c->stream));
+    double sum = 0.0;
+    XR_HIP(c, hipMemcpyAsync(&sum, c->d_err, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    *rmse = sqrt(sum / (3.0 * (double)n));
+    return XR_OK;
 }
 
 // plane p of entry j (planes stored as [slot][plane][pixel])
@@ -579,6 +649,7 @@ int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
     if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
     if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
     void* src = buffer_ptr(c, which);
+    if (!src) return set_err(c, XR_ERR_STATE, "buffer %d has not been created yet", which);
     const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
     if (!planar || layout == XR_LAYOUT_NATIVE) {
         XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -632,6 +703,7 @@ int xr_write_buffer(xr_ctx* c, int which, const void* src, size_t bytes, int lay
     if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
     if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
     void* dst = buffer_ptr(c, which);
+    if (!dst) return set_err(c, XR_ERR_STATE, "buffer %d has not been created yet", which);
     const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
     if (!planar || layout == XR_LAYOUT_NATIVE) {
         XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));

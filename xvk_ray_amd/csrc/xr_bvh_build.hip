@@ -153,8 +153,9 @@ __device__ __forceinline__ int delta(const unsigned long long* __restrict__ keys
 }
 
 // Children encoding during the build: value < n-1 internal, else leaf (value - (n-1)).
+// range[i] = the contiguous sorted-triangle range [lo, hi] below internal node i.
 __global__ void k_karras(const unsigned long long* __restrict__ keys, int n, int2* __restrict__ child,
-                         int* __restrict__ parent) {
+                         int* __restrict__ parent, int2* __restrict__ range) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -177,19 +178,36 @@ __global__ void k_karras(const unsigned long long* __restrict__ keys, int n, int
     const int left = (lo == gamma) ? (n - 1) + gamma : gamma;
     const int right = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
     child[i] = make_int2(left, right);
+    range[i] = make_int2(lo, hi);
     parent[left] = i;
     parent[right] = i;
+}
+
+// Surface-area-heuristic constants of the leaf collapse: relative cost of
+// visiting a node (two slab tests, one 64-byte fetch) and of one triangle
+// test (one 48-byte fetch, the watertight test).
+constexpr float kSahNode = 1.0f;
+constexpr float kSahTri = 1.0f;
+constexpr int kMaxLeafTris = 8;
+
+__device__ __forceinline__ float box_area(const Box& b) {
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
 __device__ __forceinline__ float ld_agent(const float* p) {
     return __uint_as_float(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Bottom-up boxes.  Node ids: internal 0..n-2, leaf j -> n-1+j.  The second
-// lane to reach an internal node computes its box and walks on.
+// Bottom-up boxes and SAH costs.  Node ids: internal 0..n-2, leaf j -> n-1+j.
+// The second lane to reach an internal node computes its box and cost and
+// walks on.  cost(leaf of k tris) = kSahTri*k*A; cost(split) = kSahNode*A +
+// cost(left) + cost(right); a non-root node whose subtree holds at most
+// kMaxLeafTris triangles becomes one leaf when that is cheaper (collapse[p]).
 __global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
                         const int2* __restrict__ child, const int* __restrict__ parent, Box* boxes,
-                        int* __restrict__ flags, uint32_t* __restrict__ max_depth, const int* __restrict__ bounds) {
+                        int* __restrict__ flags, uint32_t* __restrict__ max_depth, const int* __restrict__ bounds,
+                        const int2* __restrict__ range, float* cost, int* __restrict__ collapse) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const BvhTri t = ctris[sorted_ids[j]];
@@ -203,6 +221,7 @@ __global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __rest
     b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z) + pad;
     int node = (n - 1) + j;
     boxes[node] = b;
+    cost[node] = kSahTri * box_area(b);
     // Depth of this leaf.  A valid tree reaches the root in < n steps; the
     // bound keeps a malformed one from looping (it then reports depth n).
     uint32_t depth = 0;
@@ -225,13 +244,23 @@ __global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __rest
             m.hi[a] = fmaxf(ld_agent(&boxes[c.x].hi[a]), ld_agent(&boxes[c.y].hi[a]));
         }
         boxes[p] = m;
+        const float area = box_area(m);
+        const float split = kSahNode * area + ld_agent(&cost[c.x]) + ld_agent(&cost[c.y]);
+        const int2 r = range[p];
+        const int cnt = r.y - r.x + 1;
+        const float as_leaf = kSahTri * (float)cnt * area;
+        const bool leaf = p != 0 && cnt <= kMaxLeafTris && as_leaf <= split;
+        collapse[p] = leaf ? 1 : 0;
+        cost[p] = leaf ? as_leaf : split;
         node = p;
     }
 }
 
+// Internal nodes below a collapsed node are emitted too but never linked.
 __global__ void k_emit(int n, const int2* __restrict__ child, const Box* __restrict__ boxes,
                        BvhNode* __restrict__ nodes, const BvhTri* __restrict__ ctris,
-                       const uint32_t* __restrict__ sorted_ids, BvhTri* __restrict__ tris) {
+                       const uint32_t* __restrict__ sorted_ids, BvhTri* __restrict__ tris,
+                       const int2* __restrict__ range, const int* __restrict__ collapse) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) tris[i] = ctris[sorted_ids[i]];
     const int ninner = n > 1 ? n - 1 : 1;
@@ -252,7 +281,12 @@ __global__ void k_emit(int n, const int2* __restrict__ child, const Box* __restr
     nd.c = make_float4(b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]);
     auto enc = [&](int c) -> int {
         if (n == 1) return ~0;
-        return c < leaf_base ? c : ~((c - leaf_base) * 16);
+        if (c >= leaf_base) return ~((c - leaf_base) * 16);
+        if (collapse[c]) {
+            const int2 r = range[c];
+            return ~(r.x * 16 + (r.y - r.x));
+        }
+        return c;
     };
     nd.d = make_int4(enc(c0), enc(c1), 0, 0);
     nodes[i] = nd;
@@ -284,6 +318,9 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     int* parent = nullptr;
     Box* boxes = nullptr;
     int* flags = nullptr;
+    int2* range = nullptr;
+    float* cost = nullptr;
+    int* collapse = nullptr;
     uint32_t* dmax = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
@@ -314,6 +351,10 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     XR_TRY(hipMalloc(&parent, sizeof(int) * (ninner + tri_count)));
     XR_TRY(hipMalloc(&boxes, sizeof(Box) * (ninner + tri_count)));
     XR_TRY(hipMalloc(&flags, sizeof(int) * ninner));
+    XR_TRY(hipMalloc(&range, sizeof(int2) * ninner));
+    XR_TRY(hipMalloc(&cost, sizeof(float) * (ninner + tri_count)));
+    XR_TRY(hipMalloc(&collapse, sizeof(int) * ninner));
+    XR_TRY(hipMemsetAsync(collapse, 0, sizeof(int) * ninner, st));
     XR_TRY(hipMalloc(&dmax, sizeof(uint32_t)));
     XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * ninner));
     XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * tri_count));
@@ -331,14 +372,14 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     XR_TRY(hipMalloc(&temp, temp_bytes));
     XR_TRY(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
     if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, keys2, n, child, parent);
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, keys2, n, child, parent, range);
         XR_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, child, parent, boxes, flags, dmax,
-                       bounds);
+                       bounds, range, cost, collapse);
     XR_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_emit, dim3((tri_count + 255) / 256), dim3(256), 0, st, n, child, boxes, nodes, ctris,
-                       vals2, tris);
+                       vals2, tris, range, collapse);
     XR_TRY(hipGetLastError());
     XR_TRY(hipEventRecord(e1, st));
     XR_TRY(hipMemcpyAsync(&host_depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -369,6 +410,9 @@ done:
     hipFree(parent);
     hipFree(boxes);
     hipFree(flags);
+    hipFree(range);
+    hipFree(cost);
+    hipFree(collapse);
     hipFree(dmax);
     hipFree(temp);
     hipFree(nodes);

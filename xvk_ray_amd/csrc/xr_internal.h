@@ -32,6 +32,7 @@ struct FrameDev {
     uint32_t* motion;           // binding 15, RG16F
     uint32_t* output;           // binding 2, B8G8R8A8_UNORM
     float4* accum;              // binding 1, RGBA32F
+    float4* radiance;           // linear colour behind the output (no reference binding), or null
     unsigned long long* counters;  // [0] extend rays, [1] shadow rays
 };
 

@@ -15,6 +15,10 @@ hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame
                           const PassParams& pp, hipStream_t st);
 hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,
                         hipStream_t st);
+hipError_t launch_converge(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
+                           uint32_t frame_base, uint32_t spp, double* acc, hipStream_t st);
+hipError_t launch_sq_error(const float4* img, const double* acc, double inv_spp, uint32_t n, double* out,
+                           hipStream_t st);
 hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,
                         hipStream_t st);
 

@@ -80,6 +80,90 @@ __device__ __forceinline__ void camera_ray(const UniformBufferObject& ubo, uint3
 // ---------------------------------------------------------------------------
 // Initial sample pass
 // ---------------------------------------------------------------------------
+// What the bounce loop of InitialSample.rgen:72-181 leaves for one pixel.
+struct PathOut {
+    v3 x_view, n_view, x_s, n_s, L, f0;
+    float p_q, uv_u, uv_v;
+    uint32_t mat_idx, rand_seed;
+    bool valid;  // primitiveHitValid
+};
+
+// Camera ray + bounce loop of the initial sample pass (InitialSample.rgen:41-181)
+// for pixel (x, y) of frame `frame`; returns the number of rays traced.
+__device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const UniformBufferObject& ubo, uint32_t x,
+                                               uint32_t y, uint32_t w, uint32_t h, uint32_t frame,
+                                               uint32_t bounces, int* col, PathOut* o) {
+    uint32_t rays = 0;
+    uint32_t rng = init_seed(x, y, w, frame);
+    v3 origin, dir;
+    camera_ray(ubo, x, y, w, h, &rng, &origin, &dir);
+    v3 thr = mk3(1.0f, 1.0f, 1.0f);
+    const v3 zero = mk3(0.f, 0.f, 0.f);
+    o->x_view = zero; o->n_view = zero; o->x_s = zero; o->n_s = zero; o->L = zero; o->f0 = zero;
+    o->p_q = 0.0f; o->uv_u = 0.0f; o->uv_v = 0.0f;
+    o->mat_idx = 0xFFFFFFFFu; o->rand_seed = 0;
+    o->valid = false;
+    for (uint32_t bounce = 0;; ++bounce) {
+        if (bounce == bounces) break;
+        const RayPre rp = make_ray_pre(origin, dir);
+        Hit hit;
+        ++rays;
+        // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
+        Payload pl;
+        bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+        if (found) {
+            closest_hit(sc, hit, &pl);
+            found = pl.material != -1;
+        }
+        if (!found) {
+            if (ubo.hasSkyBox) o->L = add(o->L, thr);
+            if (bounce == 1) {
+                o->x_s = add(origin, muls(dir, 1000.0f));
+                o->n_s = neg(dir);
+            }
+            break;
+        }
+        const float4 m0 = sc.materials[2 * pl.material];
+        const float4 m1 = sc.materials[2 * pl.material + 1];
+        if (__float_as_uint(m1.w) == kDiffuseLight) {
+            o->L = add(o->L, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+            if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
+            if (bounce == 0) { o->x_view = pl.position; o->n_view = pl.normal; }
+            break;
+        }
+        if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
+        const uint32_t seed_here = rng;
+        v3 wi;
+        float pdf, cos_t;
+        const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
+        if (bounce == 0) {
+            o->valid = true;
+            o->x_view = pl.position;
+            o->n_view = pl.normal;
+            o->rand_seed = seed_here;
+            o->mat_idx = (uint32_t)pl.material;
+            o->uv_u = pl.u;
+            o->uv_v = pl.v;
+            o->p_q = pdf;
+            o->f0 = f;
+        } else {
+            thr = mul(thr, divs(muls(f, cos_t), pdf));
+        }
+        if (bounce > 3) {
+            const float p = maxf(thr.x, maxf(thr.y, thr.z));
+            if (rand01(&rng) > p) break;
+            thr = muls(thr, 1.0f / p);
+        }
+        const v3 hp = add(origin, muls(dir, pl.t));
+        const v3 g = pl.geo_normal;
+        const v3 od = dot(wi, g) > 0.0f ? g : neg(g);
+        origin = add(hp, muls(od, 0.01f));
+        dir = wi;
+    }
+    if (!o->valid) { o->L = zero; o->n_s = zero; }
+    return rays;
+}
+
 __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
@@ -88,90 +172,25 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
     if (tc.in_image) {
         const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * fr.width + x;
-        uint32_t rng = init_seed(x, y, fr.width, ubo.currentFrame);
-        v3 origin, dir;
-        camera_ray(ubo, x, y, fr.width, fr.height, &rng, &origin, &dir);
-
-        v3 thr = mk3(1.0f, 1.0f, 1.0f);
-        v3 x_view = mk3(0.f, 0.f, 0.f), n_view = x_view, x_s = x_view, n_s = x_view, L = x_view, f0 = x_view;
-        float p_q = 0.0f, uv_u = 0.0f, uv_v = 0.0f;
-        uint32_t mat_idx = 0xFFFFFFFFu, rand_seed = 0;
-        bool valid = false;
-        int* col = stack + threadIdx.x;
-        for (uint32_t bounce = 0;; ++bounce) {
-            if (bounce == ubo.numberOfBounces) break;
-            const RayPre rp = make_ray_pre(origin, dir);
-            Hit hit;
-            ++rays;
-            // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
-            Payload pl;
-            bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
-            if (found) {
-                closest_hit(sc, hit, &pl);
-                found = pl.material != -1;
-            }
-            if (!found) {
-                if (ubo.hasSkyBox) L = add(L, thr);
-                if (bounce == 1) {
-                    x_s = add(origin, muls(dir, 1000.0f));
-                    n_s = neg(dir);
-                }
-                break;
-            }
-            const float4 m0 = sc.materials[2 * pl.material];
-            const float4 m1 = sc.materials[2 * pl.material + 1];
-            if (__float_as_uint(m1.w) == kDiffuseLight) {
-                L = add(L, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
-                if (bounce == 1) { x_s = pl.position; n_s = pl.normal; }
-                if (bounce == 0) { x_view = pl.position; n_view = pl.normal; }
-                break;
-            }
-            if (bounce == 1) { x_s = pl.position; n_s = pl.normal; }
-            const uint32_t seed_here = rng;
-            v3 wi;
-            float pdf, cos_t;
-            const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
-            if (bounce == 0) {
-                valid = true;
-                x_view = pl.position;
-                n_view = pl.normal;
-                rand_seed = seed_here;
-                mat_idx = (uint32_t)pl.material;
-                uv_u = pl.u;
-                uv_v = pl.v;
-                p_q = pdf;
-                f0 = f;
-            } else {
-                thr = mul(thr, divs(muls(f, cos_t), pdf));
-            }
-            if (bounce > 3) {
-                const float p = maxf(thr.x, maxf(thr.y, thr.z));
-                if (rand01(&rng) > p) break;
-                thr = muls(thr, 1.0f / p);
-            }
-            const v3 hp = add(origin, muls(dir, pl.t));
-            const v3 g = pl.geo_normal;
-            const v3 od = dot(wi, g) > 0.0f ? g : neg(g);
-            origin = add(hp, muls(od, 0.01f));
-            dir = wi;
-        }
+        PathOut o;
+        rays = trace_path(sc, ubo, x, y, fr.width, fr.height, ubo.currentFrame, ubo.numberOfBounces,
+                          stack + threadIdx.x, &o);
         const uint32_t n = fr.n;
         float4* S = fr.samples;
-        if (!valid) { L = mk3(0.f, 0.f, 0.f); n_s = L; }
-        S[0 * n + pix] = f4(x_view, __uint_as_float(mat_idx));
-        S[1 * n + pix] = f4(n_view, __uint_as_float(rand_seed));
-        S[2 * n + pix] = f4(x_s, p_q);
-        S[3 * n + pix] = f4(n_s, 0.0f);
-        S[4 * n + pix] = f4(L, 0.0f);
-        S[5 * n + pix] = f4(f0, 0.0f);
-        if (valid) {
-            fr.gpos[pix] = f4(x_view, 1.0f);
-            fr.gnrm[pix] = f4(n_view, 0.0f);
-            fr.gmatuv[pix] = make_float4((float)mat_idx, uv_u, uv_v, 0.0f);
+        S[0 * n + pix] = f4(o.x_view, __uint_as_float(o.mat_idx));
+        S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
+        S[2 * n + pix] = f4(o.x_s, o.p_q);
+        S[3 * n + pix] = f4(o.n_s, 0.0f);
+        S[4 * n + pix] = f4(o.L, 0.0f);
+        S[5 * n + pix] = f4(o.f0, 0.0f);
+        if (o.valid) {
+            fr.gpos[pix] = f4(o.x_view, 1.0f);
+            fr.gnrm[pix] = f4(o.n_view, 0.0f);
+            fr.gmatuv[pix] = make_float4((float)o.mat_idx, o.uv_u, o.uv_v, 0.0f);
             const float cu = ((float)x + 0.5f) / (float)fr.width;
             const float cv = ((float)y + 0.5f) / (float)fr.height;
             float vp[4], cp[4];
-            mat_vec(ubo.viewPrev, x_view.x, x_view.y, x_view.z, 1.0f, vp);
+            mat_vec(ubo.viewPrev, o.x_view.x, o.x_view.y, o.x_view.z, 1.0f, vp);
             mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
             const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
             const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
@@ -185,6 +204,67 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
         }
     }
     count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// ---------------------------------------------------------------------------
+// Converged image (FP64 progressive accumulation; no reference counterpart)
+// ---------------------------------------------------------------------------
+// One sample of the quantity the ReSTIR output estimates: the initial
+// sample's own RIS estimate with M = 1, W = 1/p_q, i.e.
+// L_out * (1/p_q) * f * max(0, n_v . normalize(x_s - x_v)) for pixels with a
+// valid primary hit and 0 elsewhere (the black / white early-out pixels of
+// the spatial pass).  Sample s of pixel p uses frame number frame_base + s.
+// Each pixel adds its spp samples into three FP64 sums.
+__global__ __launch_bounds__(kBlock) void k_converge(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+                                                     uint32_t frame_base, uint32_t spp, double* __restrict__ acc) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image) {
+        const uint32_t pix = tc.y * fr.width + tc.x;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (uint32_t s = 0; s < spp; ++s) {
+            PathOut o;
+            rays += trace_path(sc, ubo, tc.x, tc.y, fr.width, fr.height, frame_base + s, ubo.numberOfBounces,
+                               stack + threadIdx.x, &o);
+            if (o.valid && o.p_q > 1e-6f) {
+                const v3 wi = normalize(sub(o.x_s, o.x_view));
+                const float c = maxf(0.0f, dot(o.n_view, wi));
+                const float W = 1.0f / o.p_q;
+                sx += (double)(o.L.x * W * o.f0.x * c);
+                sy += (double)(o.L.y * W * o.f0.y * c);
+                sz += (double)(o.L.z * W * o.f0.z * c);
+            }
+        }
+        acc[3 * (size_t)pix + 0] += sx;
+        acc[3 * (size_t)pix + 1] += sy;
+        acc[3 * (size_t)pix + 2] += sz;
+    }
+    count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// Sum over pixels and channels of (img - acc/spp)^2 in FP64; one double
+// atomic per workgroup.
+__global__ __launch_bounds__(256) void k_sq_error(const float4* __restrict__ img, const double* __restrict__ acc,
+                                                  double inv_spp, uint32_t n, double* __restrict__ out) {
+    __shared__ double part[256];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    double e = 0.0;
+    if (i < n) {
+        const float4 v = img[i];
+        const double dx = (double)v.x - acc[3 * (size_t)i] * inv_spp;
+        const double dy = (double)v.y - acc[3 * (size_t)i + 1] * inv_spp;
+        const double dz = (double)v.z - acc[3 * (size_t)i + 2] * inv_spp;
+        e = dx * dx + dy * dy + dz * dz;
+    }
+    part[threadIdx.x] = e;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -280,6 +360,7 @@ __global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, P
         res_init(&r0);
         store_planes(cur, n, pix, r0.p);
         store_planes(spat_cur, n, pix, r0.p);
+        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     bool found = false;
@@ -328,6 +409,7 @@ __global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, P
     v3 color = mk3(0.f, 0.f, 0.f);
     if (r_W(r) > 0.0f) color = shade_reservoir(r, x_curr, xyz(s[1]));
     fr.output[pix] = resolve_bgra8(color, 1.0f);
+    if (fr.radiance) fr.radiance[pix] = f4(color, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,8 +443,10 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, ui
         const float4 s3 = fr.samples[3 * n + pix];
         if (length(xyz(s1)) == 0.0f) {
             fr.output[pix] = 0xFF000000u;
+            if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else if (length(xyz(s3)) == 0.0f) {
             fr.output[pix] = 0xFFFFFFFFu;
+            if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
             Res center;
             load_planes(rd, n, pix, center.p);
@@ -401,9 +485,12 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, ui
                 }
                 jac = clampf(jac, 0.1f, 10.0f);
                 float ph = length(xyz(rn.p[4])) * jac;
-                if (pp.visibility) {
-                    // CheckVisibility (Spatial.rgen:44-62) from the currently
-                    // selected sample's visible point, any-hit.
+                // CheckVisibility (Spatial.rgen:44-62) from the currently
+                // selected sample's visible point, any-hit.  Its only effect
+                // is to zero ph, so it is traced only when ph is not already
+                // zero (same result, fewer rays: samples that found no light
+                // carry L_out = 0).
+                if (pp.visibility && ph != 0.0f) {
                     const v3 from = xyz(r.p[0]);
                     const v3 nf = xyz(r.p[1]);
                     const v3 dv = sub(xs, from);
@@ -425,7 +512,10 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, ui
             v3 color = mk3(0.f, 0.f, 0.f);
             if (r_W(r) > 0.0f) color = shade_reservoir(r, xyz(s0), xyz(s1));
             store_planes(wr, n, pix, r.p);
-            if (reuse) fr.output[pix] = resolve_bgra8(color, 1.0f);
+            if (reuse) {
+                fr.output[pix] = resolve_bgra8(color, 1.0f);
+                if (fr.radiance) fr.radiance[pix] = f4(color, 0.0f);
+            }
         }
     }
     count_rays(&fr.counters[1], rays, &lds_count);
@@ -556,6 +646,17 @@ hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBuf
                         hipStream_t st) {
     hipLaunchKernelGGL(k_plain, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        max_depth);
+    return hipGetLastError();
+}
+hipError_t launch_converge(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
+                           uint32_t frame_base, uint32_t spp, double* acc, hipStream_t st) {
+    hipLaunchKernelGGL(k_converge, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
+                       frame_base, spp, acc);
+    return hipGetLastError();
+}
+hipError_t launch_sq_error(const float4* img, const double* acc, double inv_spp, uint32_t n, double* out,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_sq_error, dim3((n + 255) / 256), dim3(256), 0, st, img, acc, inv_spp, n, out);
     return hipGetLastError();
 }
 hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,

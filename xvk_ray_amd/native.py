@@ -38,6 +38,9 @@ XR_BUF_GBUF_POS = 12
 XR_BUF_GBUF_NRM = 13
 XR_BUF_GBUF_MATUV = 14
 XR_BUF_MOTION = 15
+XR_BUF_RADIANCE = 16
+XR_BUF_CONVERGED = 17
+XR_FLAG_RADIANCE = 1
 XR_LAYOUT_REFERENCE = 0
 XR_LAYOUT_NATIVE = 1
 
@@ -72,7 +75,7 @@ class XrSceneDesc(C.Structure):
 class XrRestirParams(C.Structure):
     _fields_ = [("temporal_m_cap", C.c_uint32), ("spatial_m_cap", C.c_uint32), ("spatial_samples", C.c_uint32),
                 ("spatial_radius", C.c_float), ("visibility", C.c_uint32), ("plain_max_depth", C.c_uint32),
-                ("passes", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+                ("passes", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 8)]
 
 
 class XrAccelInfo(C.Structure):
@@ -109,7 +112,7 @@ XR_API_SYMBOLS = [
     "xr_read_buffer", "xr_write_buffer", "xr_read_image", "xr_buffer_bytes", "xr_trace_rays",
     "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
     "xr_reset_counters", "xr_get_accel_info",
-    "xr_read_accel", "xr_stream",
+    "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
     "xr_last_error", "xr_global_last_error",
 ]
 XR_HOST_SYMBOLS = [
@@ -152,6 +155,11 @@ def lib() -> C.CDLL:
         L.xr_trace_rays.argtypes = [P, P, U32, P, I]
         L.xr_trace_rays_device.argtypes = [P, P, U32, P, I]
         L.xr_pass_times.argtypes = [P, C.POINTER(C.c_float), U32]
+        L.xr_converge.argtypes = [P, C.POINTER(XrUbo), U32, U32]
+        L.xr_converge_reset.argtypes = [P]
+        L.xr_converged_spp.argtypes = [P]
+        L.xr_converged_spp.restype = C.c_uint64
+        L.xr_rmse.argtypes = [P, C.POINTER(C.c_double)]
         L.xr_pass_time_totals.argtypes = [P, C.POINTER(C.c_double), U32, C.POINTER(U32)]
         L.xr_reset_pass_times.argtypes = [P]
         L.xr_counters.argtypes = [P, C.POINTER(C.c_uint64), U32]
@@ -395,7 +403,7 @@ class Context:
                 return out.view(RESERVOIR_DTYPE)
         if which in (XR_BUF_OUTPUT, XR_BUF_MOTION):
             return out.view(np.uint32)
-        if which in (XR_BUF_ACCUMULATION, XR_BUF_GBUF_POS, XR_BUF_GBUF_NRM, XR_BUF_GBUF_MATUV):
+        if which in (XR_BUF_ACCUMULATION, XR_BUF_GBUF_POS, XR_BUF_GBUF_NRM, XR_BUF_GBUF_MATUV, XR_BUF_RADIANCE):
             return out.view(np.float32).reshape(-1, 4)
         return out.view(np.float32)
 
@@ -415,6 +423,27 @@ class Context:
         if n < 0:
             self.check(n)
         return [ms[i] for i in range(n)]
+
+    def converge(self, ubo: XrUbo, spp: int, frame_base: int) -> None:
+        self.check(lib().xr_converge(self._h, C.byref(ubo), spp, frame_base))
+
+    def converge_reset(self) -> None:
+        self.check(lib().xr_converge_reset(self._h))
+
+    def converged_spp(self) -> int:
+        return int(lib().xr_converged_spp(self._h))
+
+    def converged_mean(self) -> np.ndarray:
+        """(n, 3) float64 mean of the converged image."""
+        n = self.buffer_bytes(XR_BUF_CONVERGED, XR_LAYOUT_NATIVE)
+        out = np.empty(n // 8, np.float64)
+        self.check(lib().xr_read_buffer(self._h, XR_BUF_CONVERGED, out.ctypes.data, n, XR_LAYOUT_NATIVE))
+        return out.reshape(-1, 3) / max(1, self.converged_spp())
+
+    def rmse(self) -> float:
+        r = C.c_double()
+        self.check(lib().xr_rmse(self._h, C.byref(r)))
+        return float(r.value)
 
     def pass_time_totals(self) -> tuple[list[float], int]:
         ms = (C.c_double * 3)()
