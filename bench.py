@@ -153,7 +153,7 @@ def main():
         app.draw_frame(params)
     ctx.sync()
     wall = time.perf_counter() - t0
-    ext, shadow = ctx.counters()
+    ext, shadow, valid_px = ctx.counters3()
     pass_ms, frames = ctx.pass_time_totals()
     assert frames == args.steps, (frames, args.steps)
 
@@ -205,6 +205,7 @@ def main():
         "rays_per_frame": rays // args.steps,
         "extend_rays_per_frame": ext // args.steps,
         "shadow_rays_per_frame": shadow // args.steps,
+        "valid_primary_frac": round(valid_px / (args.steps * W * H), 4),
         "passes": per_pass,
         "temporal_plus_spatial_ms": round(per_pass["temporal"]["ms"] + per_pass["spatial"]["ms"], 4),
         "roofline": roofline,

@@ -33,7 +33,8 @@ int xrh_scene_cornell(uint64_t seed, xrh_scene** out);
 /* Terrain: grid x grid vertex heightfield over [0,1]^2 with 4-octave fBm
  * heights (amplitude 0.1), n_spheres icospheres (1,280 triangles each, one
  * instance each) resting on it with radii U[0.02,0.1] and albedo U[0.1,0.9],
- * and n_lights downward-facing quad lights of radiance U[5,50] above it (one
+ * and n_lights downward-facing quad lights (side U[0.03,0.08], height
+ * U[0.6,0.9]) of radiance U[5,50] above it (one
  * instance). */
 int xrh_scene_terrain(uint64_t seed, uint32_t grid, uint32_t n_spheres, uint32_t n_lights, xrh_scene** out);
 
@@ -48,7 +49,8 @@ typedef struct xrh_camera_path xrh_camera_path;
 /* Static camera at eye looking at target (config-1 style). */
 int xrh_camera_static(const float eye[3], const float target[3], xrh_camera_path** out);
 /* Closed Catmull-Rom path through n_ctrl seeded control points over the
- * terrain (eye 0.25-0.45 above the ground, looking at a second seeded path
+ * terrain (eye at heights 0.32-0.45: above every sphere, below every light;
+ * looking at a second seeded path
  * near the ground), sampled at `frames` evenly spaced parameters. */
 int xrh_camera_orbit(uint64_t seed, uint32_t n_ctrl, uint32_t frames, xrh_camera_path** out);
 /* View matrix (column-major) of path frame k (k taken modulo the frame count). */

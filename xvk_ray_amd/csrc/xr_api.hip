@@ -537,7 +537,7 @@ int xr_counters(xr_ctx* c, uint64_t* out, uint32_t n) {
     unsigned long long h[4];
     XR_HIP(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
-    for (uint32_t i = 0; i < n && i < 2; ++i) out[i] = h[i];
+    for (uint32_t i = 0; i < n && i < 3; ++i) out[i] = h[i];
     return XR_OK;
 }
 

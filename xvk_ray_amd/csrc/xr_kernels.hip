@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
-    uint32_t rays = 0;
+    uint32_t rays = 0, valid = 0;
     if (tc.in_image) {
         const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * fr.width + x;
@@ -202,8 +202,10 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
             fr.gmatuv[pix] = make_float4(-1.f, 0.f, 0.f, 0.f);
             fr.motion[pix] = 0u;
         }
+        valid = o.valid ? 1u : 0u;
     }
     count_rays(&fr.counters[0], rays, &lds_count);
+    count_rays(&fr.counters[2], valid, &lds_count);
 }
 
 // ---------------------------------------------------------------------------

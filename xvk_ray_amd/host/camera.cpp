@@ -117,7 +117,9 @@ int xrh_camera_orbit(uint64_t seed, uint32_t n_ctrl, uint32_t frames, xrh_camera
         const double th = 2.0 * M_PI * (k + rng.uni(-0.2, 0.2)) / n_ctrl;
         const double rho = rng.uni(0.25, 0.45);
         ce[3 * k] = 0.5 + rho * cos(th);
-        ce[3 * k + 1] = rng.uni(0.25, 0.45);
+        // above every sphere top of xrh_scene_terrain (<= 0.1 + 2 * 0.1) and
+        // below its lights (0.6-0.9), so the eye never sits inside geometry
+        ce[3 * k + 1] = rng.uni(0.32, 0.45);
         ce[3 * k + 2] = 0.5 + rho * sin(th);
         const double tr = rng.uni(0.0, 0.2);
         ct[3 * k] = 0.5 + tr * cos(th + M_PI);

@@ -293,7 +293,8 @@ int xrh_scene_terrain(uint64_t seed, uint32_t grid, uint32_t n_spheres, uint32_t
             const double down[3] = {0, -1, 0};
             for (uint32_t k = 0; k < n_lights; ++k) {
                 const double size = rng.uni(0.03f, 0.08f);
-                const double x = rng.uni(0.0f, 1.0f), z = rng.uni(0.0f, 1.0f), y = rng.uni(0.3f, 0.5f);
+                // above the camera band of xrh_camera_orbit (eye heights 0.32-0.45)
+                const double x = rng.uni(0.0f, 1.0f), z = rng.uni(0.0f, 1.0f), y = rng.uni(0.6f, 0.9f);
                 s->materials.push_back(light(rng.uni(5.0f, 50.0f)));
                 const int mat = (int)s->materials.size() - 1;
                 const double c[3] = {x - 0.5 * size, y, z - 0.5 * size}, e1[3] = {0, 0, size}, e2[3] = {size, 0, 0};

@@ -455,9 +455,14 @@ class Context:
         self.check(lib().xr_reset_pass_times(self._h))
 
     def counters(self) -> tuple[int, int]:
-        c = (C.c_uint64 * 2)()
-        self.check(lib().xr_counters(self._h, c, 2))
-        return int(c[0]), int(c[1])
+        """(extend rays, shadow rays) since the last reset."""
+        return self.counters3()[:2]
+
+    def counters3(self) -> tuple[int, int, int]:
+        """(extend rays, shadow rays, valid primary hits) since the last reset."""
+        c = (C.c_uint64 * 3)()
+        self.check(lib().xr_counters(self._h, c, 3))
+        return int(c[0]), int(c[1]), int(c[2])
 
     def reset_counters(self) -> None:
         self.check(lib().xr_reset_counters(self._h))
