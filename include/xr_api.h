@@ -143,13 +143,25 @@ typedef struct xr_restir_params {
     uint32_t plain_max_depth;  /* mode 1 bounce cap, 0 = unbounded as rgen.hlsl:59 */
     uint32_t passes;           /* XR_PASS_* mask for mode 2, 0 = all three */
     uint32_t flags;            /* XR_FLAG_* */
-    uint32_t reserved[8];      /* must be zero */
+    float mlp_spread;          /* cache: stop at x_s when its spread > mlp_spread * primary footprint (0.01) */
+    float mlp_lr;              /* cache: Adam learning rate (1e-3) */
+    uint32_t mlp_train_pixels; /* cache: training samples per frame (65,536) */
+    float mlp_loss_scale;      /* cache: loss scale of the BF16 backward pass (128) */
+    uint32_t reserved[4];      /* must be zero */
 } xr_restir_params;
 
 /* flags: also write the linear (pre-tone-map) colour of every output pixel to
  * XR_BUF_RADIANCE (0 for the black / white early-out pixels of the spatial
  * pass), for error measurements against the converged image. */
 #define XR_FLAG_RADIANCE 0x1u
+/* flags: MLP radiance cache (no reference counterpart).  In mode 2 the
+ * initial-sample pass ends most paths at their first diffuse vertex x_s and
+ * takes L_out from a 96-64x5-3 MLP evaluated there; every
+ * (pixels / mlp_train_pixels)-th pixel keeps tracing and its path-suffix
+ * radiance trains the network online (one Adam step per frame, BF16 MFMA,
+ * FP32 accumulation).  Paths whose x_s is too close to their primary footprint
+ * fall back to plain tracing. */
+#define XR_FLAG_MLP_CACHE 0x2u
 
 /* Fills p with the reference's compile-time constants (30 / 300 / 6 / 5.0,
  * always-visible, unbounded plain depth, all passes). */
@@ -237,8 +249,9 @@ int xr_trace_rays_device(xr_ctx* ctx, const xr_ray* d_rays, uint32_t n, xr_hit* 
 /* ---- measurement ---- */
 
 /* Milliseconds of the last xr_render's kernels (HIP events on the context's
- * stream), in pass order: mode 2 -> {initial, temporal, spatial}, mode 1 ->
- * {plain}.  Returns the number written.  Waits for the frame. */
+ * stream): mode 2 -> {initial, temporal, spatial, cache (inference + training
+ * + Adam; two adjacent events without XR_FLAG_MLP_CACHE)}, mode 1 -> {plain}.  Returns the
+ * number written.  Waits for the frame. */
 int xr_pass_times(xr_ctx* ctx, float* ms, uint32_t n);
 
 /* Per-pass milliseconds summed over every frame rendered since the last
@@ -294,6 +307,27 @@ uint64_t xr_converged_spp(const xr_ctx* ctx);
  * last frame rendered with XR_FLAG_RADIANCE) against the converged mean,
  * reduced in FP64 on the device.  Synchronous. */
 int xr_rmse(xr_ctx* ctx, double* rmse);
+
+/* ---- MLP radiance cache (no reference counterpart) ---- */
+
+#define XR_MLP_INPUTS 96
+#define XR_MLP_PARAMS 23043
+
+/* (Re)initialises the network (He-uniform weights from seed, zero biases and
+ * Adam state).  Runs implicitly on the first use with seed 1. */
+int xr_mlp_reset(xr_ctx* ctx, uint64_t seed);
+/* Forward pass on rows x 96 inputs (host arrays) -> rows x 3 outputs, on the
+ * same MFMA path the cache uses.  Synchronous. */
+int xr_mlp_eval(xr_ctx* ctx, const float* inputs, uint32_t rows, float* outputs);
+/* Gradient of the relative-L2 loss mean((p - y)^2 / (p^2 + 0.01)) over rows x 3
+ * on the given data (BF16 backward, FP32 accumulation), without a step. */
+int xr_mlp_gradient(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float* grad_out);
+/* One Adam step (lr; loss scale as in the frame) on the given data. */
+int xr_mlp_train_rows(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float lr);
+/* Copies the FP32 master parameters (XR_MLP_PARAMS floats; per layer W[in][out]
+ * row-major then bias) to params; if not NULL, writes them from params first
+ * when set != 0. */
+int xr_mlp_params(xr_ctx* ctx, float* params, int set);
 
 /* ---- errors ---- */
 const char* xr_last_error(const xr_ctx* ctx);

@@ -66,6 +66,21 @@ class Oracle:
             L.orc_trace64.restype = None
             L.orc_initial_sample.argtypes = [P, P, U32, U32, P, U32, P, P, P, P, P]
             L.orc_initial_sample.restype = U64
+            L.orc_initial_sample_cache.argtypes = [P, P, U32, U32, P, U32, P, P, P, P, P, U32, U32, F]
+            L.orc_initial_sample_cache.restype = U64
+            L.orc_mlp_param_count.restype = I
+            L.orc_mlp_layer_offset.argtypes = [I, I]
+            L.orc_mlp_layer_offset.restype = I
+            L.orc_bf16_round.argtypes = [P, U32, P]
+            L.orc_bf16_round.restype = None
+            L.orc_mlp_encode.argtypes = [P, P, P, P, P, P, P]
+            L.orc_mlp_encode.restype = None
+            L.orc_mlp_forward.argtypes = [P, P, U32, I, P]
+            L.orc_mlp_forward.restype = None
+            L.orc_mlp_loss_grad.argtypes = [P, P, P, U32, I, P]
+            L.orc_mlp_loss_grad.restype = C.c_double
+            L.orc_adam_step.argtypes = [P, P, P, P, U32, C.c_double, C.c_double, C.c_double, C.c_double, U32]
+            L.orc_adam_step.restype = None
             L.orc_temporal.argtypes = [U32, U32, U32, U32, P, U32, P, P, P, P, P, P]
             L.orc_temporal.restype = None
             L.orc_spatial.argtypes = [P, U32, U32, U32, U32, U32, F, U32, I, P, U32, P, P, P, P]
@@ -124,7 +139,8 @@ class OracleScene:
         self.L.orc_trace64(self.h, r.ctypes.data, n, ids.ctypes.data, ts.ctypes.data)
         return ids, ts
 
-    def initial_sample(self, ubo_bytes: bytes, W: int, H: int, pixels=None):
+    def initial_sample(self, ubo_bytes: bytes, W: int, H: int, pixels=None, cache=None):
+        """cache: None, or (stride, offset, spread_c) of the radiance-cache path termination."""
         from xvk_ray_amd.native import SAMPLE_DTYPE
         n = W * H
         samples = np.zeros(n, SAMPLE_DTYPE)
@@ -134,9 +150,10 @@ class OracleScene:
         motion = np.zeros(n, np.uint32)
         ub = np.frombuffer(ubo_bytes, np.uint8).copy()
         px = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)
-        rays = self.L.orc_initial_sample(self.h, ub.ctypes.data, W, H, _ptr(px), 0 if px is None else len(px),
-                                         samples.ctypes.data, gpos.ctypes.data, gnrm.ctypes.data,
-                                         gmatuv.ctypes.data, motion.ctypes.data)
+        stride, offset, spread = cache if cache is not None else (0, 0, 0.0)
+        rays = self.L.orc_initial_sample_cache(self.h, ub.ctypes.data, W, H, _ptr(px), 0 if px is None else len(px),
+                                               samples.ctypes.data, gpos.ctypes.data, gnrm.ctypes.data,
+                                               gmatuv.ctypes.data, motion.ctypes.data, stride, offset, spread)
         return dict(samples=samples, gpos=gpos, gnrm=gnrm, gmatuv=gmatuv, motion=motion, rays=rays)
 
     def spatial(self, W, H, frame, reuse, num_samples, radius, m_max, visibility, samples, temporal, spatial, output,
@@ -160,3 +177,55 @@ def temporal(W, H, frame, m_max, samples, motion, gpos_prev, temporal_buf, spati
     L.orc_temporal(W, H, frame, m_max, _ptr(px), 0 if px is None else len(px), samples.ctypes.data,
                    motion.ctypes.data, gpos_prev.ctypes.data, temporal_buf.ctypes.data, spatial_buf.ctypes.data,
                    output.ctypes.data)
+
+
+# ---- MLP radiance cache ----
+MLP_INPUTS = 96
+MLP_PARAMS = 23043
+
+
+def bf16_round(x) -> np.ndarray:
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(a)
+    Oracle.lib().orc_bf16_round(a.ctypes.data, a.size, out.ctypes.data)
+    return out
+
+
+def mlp_layer_offsets(layer: int) -> tuple[int, int]:
+    """(weight offset, bias offset) of a layer in the parameter vector."""
+    L = Oracle.lib()
+    return L.orc_mlp_layer_offset(layer, 0), L.orc_mlp_layer_offset(layer, 1)
+
+
+def mlp_encode(p, w, n, a, lo, inv_ext) -> np.ndarray:
+    args = [np.ascontiguousarray(v, np.float32) for v in (p, w, n, a, lo, inv_ext)]
+    out = np.zeros(MLP_INPUTS, np.float32)
+    Oracle.lib().orc_mlp_encode(*[v.ctypes.data for v in args], out.ctypes.data)
+    return out
+
+
+def mlp_forward(params, x, mode: int = 1) -> np.ndarray:
+    """mode 0: exact FP64 network; mode 1: the kernels' BF16 rounding points."""
+    w = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+    out = np.zeros((len(xx), 3), np.float64)
+    Oracle.lib().orc_mlp_forward(w.ctypes.data, xx.ctypes.data, len(xx), mode, out.ctypes.data)
+    return out
+
+
+def mlp_loss_grad(params, x, y, mode: int = 1, grad: bool = True):
+    """(loss, gradient or None) of the relative-L2 loss."""
+    w = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+    yy = np.ascontiguousarray(y, np.float32).reshape(-1, 3)
+    g = np.zeros(MLP_PARAMS, np.float64) if grad else None
+    loss = Oracle.lib().orc_mlp_loss_grad(w.ctypes.data, xx.ctypes.data, yy.ctypes.data, len(xx), mode, _ptr(g))
+    return float(loss), g
+
+
+def adam_step(w, m, v, g, lr, beta1, beta2, eps, t) -> None:
+    """In place on float64 arrays."""
+    for a in (w, m, v, g):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    Oracle.lib().orc_adam_step(w.ctypes.data, m.ctypes.data, v.ctypes.data, g.ctypes.data, len(w), lr, beta1, beta2,
+                               eps, t)

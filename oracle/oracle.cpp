@@ -610,8 +610,22 @@ struct GOut {
     float pos[4], nrm[4], matuv[4];
     uint32_t motion;
 };
+// Radiance-cache path termination (no reference counterpart; Mueller et al.
+// 2021, "Real-time Neural Radiance Caching", section 4): at the first
+// secondary vertex x_s the path stops into the cache when the spread
+// t1^2 / (p_q |cos theta_s|) exceeds c times the primary footprint
+// t0^2 / (4 pi |cos theta_v|); pixel p keeps tracing as a training sample
+// when p % stride == offset.  Recorded in the spare words: pad1 = material
+// at x_s, pad2 = 1 (query) or 2 (training).
+struct CacheOpt {
+    bool enabled = false;
+    uint32_t stride = 1, offset = 0;
+    float spread_c = 0.0f;
+};
 uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32_t y, uint32_t W, uint32_t H,
-                              ReSTIRGISample& out, GOut& g) {
+                              ReSTIRGISample& out, GOut& g, const CacheOpt& cache = CacheOpt()) {
+    uint32_t cache_kind = 0, xs_material = 0;
+    float footprint0 = 0.0f;
     uint32_t rays = 0;
     uint32_t rngState = InitRandomSeed(x, y, W, ubo.currentFrame);
     float3 origin, direction;
@@ -646,6 +660,20 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
         float pdf = 0, cos_theta = 0;
         float3 wi = F3(0, 0, 0);
         if (bounce == 1) { x_sample = pl.Position; n_sample = pl.Normal; }
+        if (cache.enabled) {
+            const float cos_hit = std::fabs(dot(pl.Normal, direction));
+            if (bounce == 0) {
+                footprint0 = pl.HitT * pl.HitT / (4.0f * PI * cos_hit);
+            } else if (bounce == 1) {
+                xs_material = (uint32_t)pl.MaterialIndex;
+                if ((y * W + x) % cache.stride == cache.offset) {
+                    cache_kind = 2;
+                } else if (pl.HitT * pl.HitT / (p_q * cos_hit) > cache.spread_c * footprint0) {
+                    cache_kind = 1;
+                    break;
+                }
+            }
+        }
         uint32_t seedForThisBounce = rngState;
         float3 f = SampleBSDF(pl.Normal, mat, wi, pdf, cos_theta, rngState);
         if (bounce == 0) {
@@ -680,7 +708,9 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
     out.randSeed = rand_seed;
     out.p_q = p_q;
     out.f = f_;
-    out.pad1 = out.pad2 = out.pad3 = 0;
+    out.pad1 = cache_kind ? xs_material : 0u;
+    out.pad2 = cache_kind;
+    out.pad3 = 0;
     if (primitiveHitValid) {
         float pos[4] = {x_view.x, x_view.y, x_view.z, 1.0f};
         float nrm[4] = {n_view.x, n_view.y, n_view.z, 0.0f};
@@ -1102,9 +1132,14 @@ void orc_trace64(const void* h, const double* rays, uint32_t n, uint32_t* ids, d
 
 // ---- passes over whole images (reference layouts) ----
 // pixels: list of pixel indices to compute (NULL = all W*H).
-uint64_t orc_initial_sample(const void* h, const void* ubo, uint32_t W, uint32_t H, const uint32_t* pixels,
-                            uint32_t npix, void* samples, float* gpos, float* gnrm, float* gmatuv,
-                            uint32_t* motion) {
+uint64_t orc_initial_sample_cache(const void* h, const void* ubo, uint32_t W, uint32_t H, const uint32_t* pixels,
+                                  uint32_t npix, void* samples, float* gpos, float* gnrm, float* gmatuv,
+                                  uint32_t* motion, uint32_t cache_stride, uint32_t cache_offset, float spread_c) {
+    CacheOpt cache;
+    cache.enabled = cache_stride != 0;
+    cache.stride = cache_stride ? cache_stride : 1;
+    cache.offset = cache_offset;
+    cache.spread_c = spread_c;
     const Scene& s = *(const Scene*)h;
     const UBO& u = *(const UBO*)ubo;
     ReSTIRGISample* out = (ReSTIRGISample*)samples;
@@ -1115,13 +1150,19 @@ uint64_t orc_initial_sample(const void* h, const void* ubo, uint32_t W, uint32_t
         uint32_t p = pixels ? pixels[k] : (uint32_t)k;
         uint32_t x = p % W, y = p / W;
         GOut g;
-        rays += initial_sample_pixel(s, u, x, y, W, H, out[p], g);
+        rays += initial_sample_pixel(s, u, x, y, W, H, out[p], g, cache);
         memcpy(gpos + 4 * (size_t)p, g.pos, 16);
         memcpy(gnrm + 4 * (size_t)p, g.nrm, 16);
         memcpy(gmatuv + 4 * (size_t)p, g.matuv, 16);
         motion[p] = g.motion;
     }
     return rays;
+}
+
+uint64_t orc_initial_sample(const void* h, const void* ubo, uint32_t W, uint32_t H, const uint32_t* pixels,
+                            uint32_t npix, void* samples, float* gpos, float* gnrm, float* gmatuv,
+                            uint32_t* motion) {
+    return orc_initial_sample_cache(h, ubo, W, H, pixels, npix, samples, gpos, gnrm, gmatuv, motion, 0, 0, 0.0f);
 }
 
 void orc_temporal(uint32_t W, uint32_t H, uint32_t frame, uint32_t m_max, const uint32_t* pixels, uint32_t npix,

@@ -1,0 +1,270 @@
+"""MLP radiance cache on the GPU vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Tolerances: the kernels and the oracle's BF16 mode round at the same points
+(inputs, hidden weights, hidden activations, backpropagated pre-activation
+gradients to BF16; FP32 output layer), but the MFMA sums in an order of its
+own, so a value near a BF16 rounding boundary can land one BF16 ulp (2^-8
+relative) apart and carry that through later layers.  Forward outputs are
+compared to 1e-2 of the output scale, gradients per layer to 5e-2 in relative
+L2 norm with cosine similarity above 0.998.  The Adam step is pinned by its
+closed form; path termination and training-pixel choice are compared bit for
+bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from tests.helpers import CORNELL_EYE, CORNELL_TARGET, bytes_equal_rows, describe_mismatch, look_view, oracle_scene, say
+from tests.test_cpu_oracle_mlp import he_params, random_inputs
+from xvk_ray_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(W=32, H=32):
+    return N.Context(N.Scene.cornell(1), W, H)
+
+
+def _layers():
+    for l in range(6):
+        wo, _ = B.mlp_layer_offsets(l)
+        end = B.mlp_layer_offsets(l + 1)[0] if l < 5 else B.MLP_PARAMS
+        yield l, wo, end
+
+
+def _positive_params(seed):
+    w = he_params(seed)
+    w[B.mlp_layer_offsets(5)[1]:] = 5.0  # predictions away from 0 (see the oracle tests)
+    return w
+
+
+def test_params_roundtrip_and_init():
+    ctx = _ctx()
+    p0 = ctx.mlp_params()
+    assert p0.shape == (B.MLP_PARAMS,) and np.isfinite(p0).all()
+    for l, wo, end in _layers():
+        bo = B.mlp_layer_offsets(l)[1]
+        fan_in = 96 if l == 0 else 64
+        a = np.sqrt(6.0 / fan_in)
+        assert np.abs(p0[wo:bo]).max() <= a and np.abs(p0[wo:bo]).max() > 0.9 * a
+        assert np.all(p0[bo:end] == 0.0)
+    w = he_params(3)
+    assert np.array_equal(ctx.mlp_params(w), w)
+    ctx.mlp_reset(1)
+    assert np.array_equal(ctx.mlp_params(), p0)
+    ctx.mlp_reset(2)
+    assert not np.array_equal(ctx.mlp_params(), p0)
+
+
+@pytest.mark.parametrize("rows", [1, 63, 64, 1000, 20000])
+def test_forward_matches_oracle(rows):
+    ctx = _ctx()
+    w = he_params(5)
+    ctx.mlp_params(w)
+    x = random_inputs(rows, 6 + rows)
+    say(f"[mlp-eval] rows={rows}")
+    gpu = ctx.mlp_eval(x).astype(np.float64)
+    ref = B.mlp_forward(w, x, mode=1)
+    exact = B.mlp_forward(w, x, mode=0)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(gpu - ref).max() <= 1e-2 * scale, float(np.abs(gpu - ref).max())
+    assert np.abs(gpu - exact).max() <= 5e-2 * scale
+    # most rows agree much closer than the bound
+    assert np.median(np.abs(gpu - ref)) <= 1e-3 * scale
+
+
+def test_forward_encoded_features():
+    """Inputs of the shape the cache sees: encoded position, unit vectors, albedo."""
+    ctx = _ctx()
+    w = he_params(8)
+    ctx.mlp_params(w)
+    rng = np.random.default_rng(9)
+    rows = 4096
+    x = np.zeros((rows, 96), np.float32)
+    lo = np.zeros(3, np.float32)
+    inv = np.ones(3, np.float32)
+    for r in range(rows):
+        d = rng.normal(size=3)
+        nn = rng.normal(size=3)
+        x[r] = B.mlp_encode(rng.uniform(0, 1, 3), d / np.linalg.norm(d), nn / np.linalg.norm(nn),
+                            rng.uniform(0, 1, 3), lo, inv)
+    gpu = ctx.mlp_eval(x).astype(np.float64)
+    ref = B.mlp_forward(w, x, mode=1)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(gpu - ref).max() <= 1e-2 * scale
+
+
+@pytest.mark.parametrize("rows", [64, 1000, 8192])
+def test_gradient_matches_oracle(rows):
+    ctx = _ctx()
+    w = _positive_params(12)
+    ctx.mlp_params(w)
+    x = random_inputs(rows, 13)
+    y = np.random.default_rng(14).uniform(0.0, 8.0, (rows, 3)).astype(np.float32)
+    say(f"[mlp-grad] rows={rows}")
+    g = ctx.mlp_gradient(x, y).astype(np.float64)
+    _, ref = B.mlp_loss_grad(w, x, y, mode=1)
+    for l, wo, end in _layers():
+        a, b = g[wo:end], ref[wo:end]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert rel < 5e-2 and cos > 0.998, (l, rel, cos)
+    # a gradient call takes no step
+    assert np.array_equal(ctx.mlp_params(), w)
+
+
+def test_adam_first_steps_closed_form():
+    """Step 1 of bias-corrected Adam moves each weight by -lr * g / (|g| + eps);
+    step 2 follows the oracle's Adam on the kernels' gradients."""
+    ctx = _ctx()
+    w0 = _positive_params(21)
+    ctx.mlp_reset(1)  # zero Adam state and step count ...
+    ctx.mlp_params(w0)  # ... with these weights
+    rows = 4096
+    x = random_inputs(rows, 22)
+    y = np.random.default_rng(23).uniform(0.0, 8.0, (rows, 3)).astype(np.float32)
+    lr = 1e-3
+    g1 = ctx.mlp_gradient(x, y).astype(np.float64)
+    ctx.mlp_train_rows(x, y, lr)
+    w1 = ctx.mlp_params().astype(np.float64)
+    step = w1 - w0.astype(np.float64)
+    big = np.abs(g1) > 1e-3 * np.abs(g1).max()
+    assert big.sum() > 1000
+    assert np.allclose(step[big], -lr * np.sign(g1[big]), rtol=2e-2, atol=1e-6), \
+        float(np.abs(step[big] + lr * np.sign(g1[big])).max())
+    assert np.abs(step).max() <= lr * 1.001  # FP32 rounding of the weight (bias 5: ulp 4.8e-7)
+    # second step against the oracle's Adam
+    x2 = random_inputs(rows, 24)
+    g2 = ctx.mlp_gradient(x2, y).astype(np.float64)
+    ctx.mlp_train_rows(x2, y, lr)
+    w2 = ctx.mlp_params().astype(np.float64)
+    wr, m, v = w0.astype(np.float64), np.zeros(B.MLP_PARAMS), np.zeros(B.MLP_PARAMS)
+    B.adam_step(wr, m, v, g1.copy(), lr, 0.9, 0.99, 1e-8, 1)
+    B.adam_step(wr, m, v, g2.copy(), lr, 0.9, 0.99, 1e-8, 2)
+    sel = (np.abs(g1) > 1e-3 * np.abs(g1).max()) & (np.abs(g2) > 1e-3 * np.abs(g2).max())
+    d_gpu, d_ref = (w2 - w0)[sel], (wr - w0)[sel]
+    assert np.abs(d_gpu - d_ref).max() <= 0.05 * lr
+
+
+def test_training_reduces_loss():
+    """A smooth radiance-like target of the encoded inputs is learned: the
+    oracle's loss of the GPU-trained weights falls well below the start."""
+    ctx = _ctx()
+    ctx.mlp_reset(3)
+    rng = np.random.default_rng(30)
+    rows = 8192
+    lo, inv = np.zeros(3, np.float32), np.ones(3, np.float32)
+    pos = rng.uniform(0, 1, (rows, 3)).astype(np.float32)
+    x = np.zeros((rows, 96), np.float32)
+    nrm = np.array([0, 1, 0], np.float32)
+    for r in range(rows):
+        x[r] = B.mlp_encode(pos[r], nrm, nrm, np.full(3, 0.5, np.float32), lo, inv)
+    y = np.stack([1.0 + np.sin(3 * pos[:, 0]), 0.5 + pos[:, 1], 2.0 * pos[:, 2] ** 2], 1).astype(np.float32)
+    w_start = ctx.mlp_params()
+    loss0, _ = B.mlp_loss_grad(w_start, x, y, mode=1, grad=False)
+    say("[mlp-train] 300 steps")
+    for _ in range(300):
+        ctx.mlp_train_rows(x, y, 1e-3)
+    w_end = ctx.mlp_params()
+    loss1, _ = B.mlp_loss_grad(w_end, x, y, mode=1, grad=False)
+    say(f"[mlp-train] loss {loss0:.4f} -> {loss1:.4f}")
+    assert np.isfinite(w_end).all()
+    assert loss1 < 0.01 * loss0 and loss1 < 0.1
+
+
+def _cache_frame(ctx, W, H, frame, params):
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    ubo = N.make_ubo(view, view, 45.0, W, H, bounces=4, frame=frame)
+    ctx.render(ubo, N.XR_MODE_RESTIR, params)
+    ctx.sync()
+    return ubo
+
+
+def _features(samples, mats, lo, inv, idx):
+    f32 = np.float32
+    x = np.zeros((len(idx), 96), np.float32)
+    for r, p in enumerate(idx):
+        s = samples[p]
+        d = (s["x_view"] - s["x_sample"]).astype(np.float32)
+        dd = f32(f32(d[0] * d[0]) + f32(d[1] * d[1])) + f32(d[2] * d[2])
+        wv = (d * (f32(1.0) / np.sqrt(f32(dd)))).astype(np.float32)
+        alb = mats["base_color"][s["pad1"]][:3]
+        x[r] = B.mlp_encode(s["x_sample"], wv, s["n_sample"], alb, lo, inv)
+    return x
+
+
+def test_cache_frame_matches_oracle():
+    """One cache frame: termination flags and every traced field bit-exact vs
+    the oracle; query pixels carry max(0, network(features)) computed with the
+    weights before the frame's training step."""
+    scene = N.Scene.cornell(1)
+    W, H = 64, 48
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    verts, _, mats, _ = scene.arrays()
+    lo = verts["pos"].min(0).astype(np.float32)
+    inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
+    stride = 8
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, mlp_train_pixels=W * H // stride, mlp_spread=0.01)
+    # train a few frames first so the network is not at its initial state
+    for frame in range(1, 4):
+        _cache_frame(ctx, W, H, frame, params)
+    w_before = ctx.mlp_params()
+    frame = 4
+    say(f"[cache-frame] {W}x{H} frame {frame}")
+    ubo = _cache_frame(ctx, W, H, frame, params)
+    w_after = ctx.mlp_params()
+    assert not np.array_equal(w_before, w_after)  # the frame trained the network
+    gpu = ctx.read(N.XR_BUF_SAMPLES)
+    offset = ((frame * 7919) & 0xFFFFFFFF) % stride
+    ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H, cache=(stride, offset, 0.01))["samples"]
+    kind = ref["pad2"]
+    assert np.array_equal(gpu["pad2"], kind) and np.array_equal(gpu["pad1"], ref["pad1"])
+    query = kind == 1
+    assert query.sum() > 0.1 * W * H and (kind == 2).sum() > 0
+    # everything but the cached radiance is bit-exact
+    g2, r2 = gpu.copy(), ref.copy()
+    g2["l_out"][query] = 0
+    eq = bytes_equal_rows(g2, r2)
+    assert eq.all(), describe_mismatch("cache-frame samples", g2, r2, eq)
+    idx = np.nonzero(query)[0]
+    pred = np.maximum(B.mlp_forward(w_before, _features(ref, mats, lo, inv, idx), mode=1), 0.0)
+    got = gpu["l_out"][idx].astype(np.float64)
+    scale = max(1.0, np.abs(pred).max())
+    assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())
+
+
+def test_cache_gain_over_untrained_and_ray_savings():
+    """What training buys: with the same path termination, the trained cache's
+    per-frame radiance is closer to the converged image than the untrained
+    cache's (lr 0), and both trace fewer extension rays than plain ReSTIR GI.
+    The RMSE ratio against plain ReSTIR GI is printed (the equal-RMSE time
+    study lives in the benchmark)."""
+    scene = N.Scene.cornell(1)
+    W, H = 96, 72
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    frames = 48
+
+    def run(flags, lr):
+        ctx = N.Context(scene, W, H)
+        ctx.converge(N.make_ubo(view, view, 45.0, W, H, bounces=4, frame=1), 512, 1 << 20)
+        p = N.default_params(flags=N.XR_FLAG_RADIANCE | flags, mlp_train_pixels=W * H // 16, mlp_lr=lr)
+        errs = []
+        ctx.reset_counters()
+        for f in range(1, frames + 1):
+            ctx.render(N.make_ubo(view, view, 45.0, W, H, bounces=4, frame=f), N.XR_MODE_RESTIR, p)
+            ctx.sync()
+            errs.append(ctx.rmse())
+        ext, _, _ = ctx.counters3()
+        return float(np.mean(errs[-8:])), ext
+
+    say("[cache-gain] plain / untrained / trained")
+    e_plain, r_plain = run(0, 1e-3)
+    e_untrained, r_untrained = run(N.XR_FLAG_MLP_CACHE, 0.0)
+    e_trained, r_trained = run(N.XR_FLAG_MLP_CACHE, 1e-3)
+    say(f"[cache-gain] rmse plain {e_plain:.4f} untrained {e_untrained:.4f} trained {e_trained:.4f}; "
+        f"extend rays {r_plain} {r_untrained} {r_trained}")
+    assert e_trained < 0.7 * e_untrained
+    assert r_trained < r_plain and r_untrained < r_plain

@@ -205,4 +205,5 @@ def test_rays_counted():
     assert W * H <= ext <= 4 * W * H
     assert 0 < shadow <= 6 * W * H
     t = ctx.pass_times()
-    assert len(t) == 3 and all(x >= 0 for x in t)
+    assert len(t) == 4 and all(x >= 0 for x in t)  # initial, temporal, spatial, cache
+    assert t[3] < 0.05  # no cache work without the cache flag (two adjacent events)

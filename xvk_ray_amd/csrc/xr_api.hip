@@ -30,6 +30,7 @@
 
 #include "../../include/xr_api.h"
 #include "xr_kernels.h"
+#include "xr_mlp.h"
 
 using namespace xr;
 
@@ -82,17 +83,27 @@ struct xr_ctx {
     // harvested into the running totals when it is about to be reused or when
     // the totals are read, so timing a run of frames needs no host sync per
     // frame.
+    // ReSTIR sets record e0 initial e1 cache e2 temporal e3 spatial e4.
     struct EventSet {
-        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-        int n = 0;           // events recorded (2: plain, 4: ReSTIR)
+        hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        int n = 0;           // events recorded (2: plain, 5: ReSTIR)
         bool pending = false;
     };
     static constexpr int kRing = 8;
     EventSet ring[kRing];
     int ring_cursor = 0;
     int last_set = -1;
-    double pass_sum_ms[3] = {0, 0, 0};
+    double pass_sum_ms[4] = {0, 0, 0, 0};  // initial (or plain), temporal, spatial, cache
     uint32_t pass_frames = 0;
+
+    // scene box (position normalisation of the cache inputs)
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};
+
+    // MLP radiance cache
+    void* d_mlp = nullptr;
+    MlpState mlp;
+    float* d_mlp_io = nullptr;  // staging for xr_mlp_eval / train_rows
+    size_t mlp_io_cap = 0;
 
     // staging for ray batches
     float4* d_rays = nullptr;
@@ -220,24 +231,91 @@ FrameDev frame_dev(const xr_ctx* c) {
     return f;
 }
 
+// Milliseconds per pass of one recorded set, in reporting order
+// (initial, temporal, spatial, cache); *count = number of passes.
+int set_times(xr_ctx* c, const xr_ctx::EventSet& set, float* ms, int* count) {
+    XR_HIP(c, hipEventSynchronize(set.ev[set.n - 1]));
+    if (set.n == 2) {
+        XR_HIP(c, hipEventElapsedTime(&ms[0], set.ev[0], set.ev[1]));
+        *count = 1;
+        return XR_OK;
+    }
+    static const int slot[4] = {0, 3, 1, 2};  // interval i -> reporting slot
+    for (int i = 0; i < 4; ++i) XR_HIP(c, hipEventElapsedTime(&ms[slot[i]], set.ev[i], set.ev[i + 1]));
+    *count = 4;
+    return XR_OK;
+}
+
 // Folds a recorded event set into the per-pass totals (waits for it).
 int harvest(xr_ctx* c, xr_ctx::EventSet& set) {
     if (!set.pending) return XR_OK;
-    XR_HIP(c, hipEventSynchronize(set.ev[set.n - 1]));
-    for (int i = 0; i + 1 < set.n; ++i) {
-        float t = 0.f;
-        XR_HIP(c, hipEventElapsedTime(&t, set.ev[i], set.ev[i + 1]));
-        c->pass_sum_ms[i] += t;
-    }
+    float ms[4] = {0, 0, 0, 0};
+    int k = 0;
+    const int rc = set_times(c, set, ms, &k);
+    if (rc) return rc;
+    for (int i = 0; i < k; ++i) c->pass_sum_ms[i] += ms[i];
     c->pass_frames++;
     set.pending = false;
     return XR_OK;
 }
 
+// He-uniform initial weights (U(-a, a), a = sqrt(6 / fan_in)) from a
+// splitmix64 stream, zero biases, zero Adam state; then the BF16 copies.
+int mlp_init(xr_ctx* c, uint64_t seed) {
+    std::vector<float> w(kMlpParams, 0.0f);
+    uint64_t s = seed;
+    auto next = [&]() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    for (int l = 0; l < kMlpLayers; ++l) {
+        const float a = sqrtf(6.0f / (float)mlp_in(l));
+        for (int i = 0; i < mlp_in(l) * mlp_out(l); ++i) {
+            const double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+            w[mlp_woff(l) + i] = (float)((2.0 * u - 1.0) * a);
+        }
+    }
+    XR_HIP(c, hipMemsetAsync(c->d_mlp, 0, mlp_bytes(), c->stream));
+    XR_HIP(c, hipMemcpyAsync(c->mlp.w, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c->stream));
+    c->mlp.steps = 0;
+    XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int ensure_mlp(xr_ctx* c) {
+    if (c->d_mlp) return XR_OK;
+    XR_HIP(c, dev_alloc(&c->d_mlp, mlp_bytes()));
+    mlp_bind(&c->mlp, c->d_mlp);
+    for (int a = 0; a < 3; ++a) {
+        const float ext = c->scene_hi[a] - c->scene_lo[a];
+        c->mlp.lo[a] = c->scene_lo[a];
+        c->mlp.inv_ext[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    return mlp_init(c, 1);
+}
+
+int ensure_mlp_io(xr_ctx* c, size_t floats) {
+    if (floats <= c->mlp_io_cap) return XR_OK;
+    dev_free(&c->d_mlp_io);
+    c->mlp_io_cap = 0;
+    XR_HIP(c, dev_alloc(&c->d_mlp_io, floats * sizeof(float)));
+    c->mlp_io_cap = floats;
+    return XR_OK;
+}
+
 int check_params(xr_ctx* c, const xr_restir_params* p) {
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 4; ++i)
         if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
-    if (p->flags & ~XR_FLAG_RADIANCE) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.flags has unknown bits");
+    if (p->flags & ~(XR_FLAG_RADIANCE | XR_FLAG_MLP_CACHE))
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.flags has unknown bits");
+    if (p->flags & XR_FLAG_MLP_CACHE) {
+        if (!(p->mlp_spread >= 0.0f) || !(p->mlp_lr >= 0.0f && p->mlp_lr < 1.0f) || p->mlp_train_pixels == 0 ||
+            !(p->mlp_loss_scale > 0.0f))
+            return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.mlp_* out of range");
+    }
     if (p->visibility > XR_VIS_TRACE) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.visibility %u", p->visibility);
     if (p->spatial_samples > 64) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.spatial_samples > 64");
     if (!(p->spatial_radius >= 0.0f && p->spatial_radius <= 4096.0f))
@@ -260,6 +338,10 @@ void xr_default_params(xr_restir_params* p) {
     p->visibility = XR_VIS_REFERENCE;
     p->plain_max_depth = 0;
     p->passes = 0;
+    p->mlp_spread = 0.01f;
+    p->mlp_lr = 1e-3f;
+    p->mlp_train_pixels = 65536;
+    p->mlp_loss_scale = 128.0f;
 }
 
 const char* xr_global_last_error(void) { return g_err.c_str(); }
@@ -283,6 +365,8 @@ void xr_destroy(xr_ctx* c) {
     dev_free(&c->d_rays);
     dev_free(&c->d_hits);
     dev_free(&c->d_err);
+    dev_free(&c->d_mlp);
+    dev_free(&c->d_mlp_io);
     for (auto& set : c->ring)
         for (auto& e : set.ev)
             if (e) hipEventDestroy(e);
@@ -326,6 +410,12 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
         first[i] = io / 3;
     }
     first[s->instance_count] = s->index_count / 3;
+    float box_lo[3] = {3e38f, 3e38f, 3e38f}, box_hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (uint32_t v = 0; v < s->vertex_count; ++v)
+        for (int a = 0; a < 3; ++a) {
+            box_lo[a] = fminf(box_lo[a], s->vertices[v].pos[a]);
+            box_hi[a] = fmaxf(box_hi[a], s->vertices[v].pos[a]);
+        }
     for (uint32_t i = 0; i < s->instance_count; ++i)
         if (first[i + 1] == first[i])
             return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "instance %u has no triangles", i);
@@ -357,6 +447,10 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     c->material_count = s->material_count;
     c->instance_count = s->instance_count;
     c->tri_count = s->index_count / 3;
+    for (int a = 0; a < 3; ++a) {
+        c->scene_lo[a] = box_lo[a];
+        c->scene_hi[a] = box_hi[a];
+    }
     XR_CREATE_HIP(dev_alloc(&c->d_vertices, sizeof(xr_vertex) * (size_t)s->vertex_count));
     XR_CREATE_HIP(dev_alloc(&c->d_indices, sizeof(uint32_t) * (size_t)s->index_count));
     XR_CREATE_HIP(dev_alloc(&c->d_materials, sizeof(xr_material) * (size_t)s->material_count));
@@ -475,14 +569,36 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
         set.n = 2;
     } else {
         const uint32_t passes = p.passes ? p.passes : 7u;
+        const bool cache_on = (p.flags & XR_FLAG_MLP_CACHE) != 0;
+        CacheCfg cache = {0u, 1u, 0u, 0.0f};
+        if (cache_on) {
+            rc = ensure_mlp(c);
+            if (rc) return rc;
+            const uint32_t n = fr.n;
+            cache.enabled = 1u;
+            cache.stride = p.mlp_train_pixels >= n ? 1u : n / p.mlp_train_pixels;
+            cache.offset = mlp_train_offset(u.currentFrame, cache.stride);
+            cache.spread_c = p.mlp_spread;
+        }
         XR_HIP(c, hipEventRecord(ev[0], c->stream));
-        if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, c->stream));
+        if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, cache, c->stream));
         XR_HIP(c, hipEventRecord(ev[1], c->stream));
-        if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
+        if (cache_on && (passes & XR_PASS_INITIAL)) {
+            // infer with the current weights, then one training step on this
+            // frame's path suffixes
+            XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
+            const uint32_t nslots = (fr.n + cache.stride - 1) / cache.stride;
+            XR_HIP(c, launch_mlp_train_frame(c->mlp, fr, c->d_materials, cache.stride, cache.offset, nslots,
+                                             p.mlp_loss_scale, c->stream));
+            c->mlp.steps++;
+            XR_HIP(c, launch_mlp_adam(c->mlp, p.mlp_lr, 1.0f / p.mlp_loss_scale, true, c->stream));
+        }
         XR_HIP(c, hipEventRecord(ev[2], c->stream));
-        if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+        if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
         XR_HIP(c, hipEventRecord(ev[3], c->stream));
-        set.n = 4;
+        if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+        XR_HIP(c, hipEventRecord(ev[4], c->stream));
+        set.n = 5;
     }
     set.pending = true;
     c->last_set = c->ring_cursor;
@@ -499,15 +615,13 @@ int xr_sync(xr_ctx* c) {
 int xr_pass_times(xr_ctx* c, float* ms, uint32_t n) {
     if (!c || !ms) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (c->last_set < 0) return 0;
-    const xr_ctx::EventSet& set = c->ring[c->last_set];
-    XR_HIP(c, hipEventSynchronize(set.ev[set.n - 1]));
+    float t[4] = {0, 0, 0, 0};
     int k = 0;
-    for (int i = 0; i + 1 < set.n && (uint32_t)k < n; ++i) {
-        float t = 0.f;
-        XR_HIP(c, hipEventElapsedTime(&t, set.ev[i], set.ev[i + 1]));
-        ms[k++] = t;
-    }
-    return k;
+    const int rc = set_times(c, c->ring[c->last_set], t, &k);
+    if (rc) return -rc;
+    int w = 0;
+    for (; w < k && (uint32_t)w < n; ++w) ms[w] = t[w];
+    return w;
 }
 
 int xr_pass_time_totals(xr_ctx* c, double* ms_sum, uint32_t n, uint32_t* frames) {
@@ -516,7 +630,7 @@ int xr_pass_time_totals(xr_ctx* c, double* ms_sum, uint32_t n, uint32_t* frames)
         int rc = harvest(c, set);
         if (rc) return rc;
     }
-    for (uint32_t i = 0; i < n && i < 3; ++i) ms_sum[i] = c->pass_sum_ms[i];
+    for (uint32_t i = 0; i < n && i < 4; ++i) ms_sum[i] = c->pass_sum_ms[i];
     if (frames) *frames = c->pass_frames;
     return XR_OK;
 }
@@ -527,7 +641,7 @@ int xr_reset_pass_times(xr_ctx* c) {
         int rc = harvest(c, set);
         if (rc) return rc;
     }
-    c->pass_sum_ms[0] = c->pass_sum_ms[1] = c->pass_sum_ms[2] = 0.0;
+    c->pass_sum_ms[0] = c->pass_sum_ms[1] = c->pass_sum_ms[2] = c->pass_sum_ms[3] = 0.0;
     c->pass_frames = 0;
     return XR_OK;
 }
@@ -681,6 +795,11 @@ int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
         memcpy(s.L_out, q[4], 12);
         memcpy(s.f, q[5], 12);
         if (which == XR_BUF_SAMPLES) {
+            // spare words: 0 on the reference path; with the radiance cache,
+            // pad1 = material at x_s and pad2 = 1 (query) / 2 (training)
+            memcpy(&s.pad1, q[3] + 3, 4);
+            memcpy(&s.pad2, q[4] + 3, 4);
+            memcpy(&s.pad3, q[5] + 3, 4);
             memcpy((char*)dst + j * sizeof(ReSTIRGISample), &s, sizeof(s));
         } else {
             ReSTIRGIReservoir r;
@@ -741,9 +860,9 @@ int xr_write_buffer(xr_ctx* c, int which, const void* src, size_t bytes, int lay
         memcpy(&q[4]->x, s.L_out, 12);
         memcpy(&q[5]->x, s.f, 12);
         if (which == XR_BUF_SAMPLES) {
-            q[3]->w = 0.f;
-            q[4]->w = 0.f;
-            q[5]->w = 0.f;
+            memcpy(&q[3]->w, &s.pad1, 4);
+            memcpy(&q[4]->w, &s.pad2, 4);
+            memcpy(&q[5]->w, &s.pad3, 4);
         } else {
             q[3]->w = r.w_sum;
             q[4]->w = r.W;
@@ -783,6 +902,98 @@ int xr_trace_rays(xr_ctx* c, const xr_ray* rays, uint32_t n, xr_hit* hits, int a
     XR_HIP(c, hipMemcpyAsync(c->d_rays, rays, sizeof(xr_ray) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     XR_HIP(c, launch_trace(scene_dev(c), c->d_rays, n, c->d_hits, any_hit != 0, c->stream));
     XR_HIP(c, hipMemcpyAsync(hits, c->d_hits, sizeof(xr_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// MLP radiance cache entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int xr_mlp_reset(xr_ctx* c, uint64_t seed) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!c->d_mlp) {
+        const int rc = ensure_mlp(c);
+        if (rc) return rc;
+    }
+    return mlp_init(c, seed);
+}
+
+int xr_mlp_eval(xr_ctx* c, const float* inputs, uint32_t rows, float* outputs) {
+    if (!c || (rows && (!inputs || !outputs))) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (rows == 0) return XR_OK;
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    rc = ensure_mlp_io(c, (size_t)rows * (kMlpIn + 3));
+    if (rc) return rc;
+    float* din = c->d_mlp_io;
+    float* dout = c->d_mlp_io + (size_t)rows * kMlpIn;
+    XR_HIP(c, hipMemcpyAsync(din, inputs, sizeof(float) * rows * kMlpIn, hipMemcpyHostToDevice, c->stream));
+    XR_HIP(c, launch_mlp_eval(c->mlp, din, rows, dout, c->stream));
+    XR_HIP(c, hipMemcpyAsync(outputs, dout, sizeof(float) * rows * 3, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+static int stage_rows(xr_ctx* c, const float* inputs, const float* targets, uint32_t rows) {
+    int rc = ensure_mlp_io(c, (size_t)rows * (kMlpIn + 3));
+    if (rc) return rc;
+    XR_HIP(c, hipMemcpyAsync(c->d_mlp_io, inputs, sizeof(float) * rows * kMlpIn, hipMemcpyHostToDevice, c->stream));
+    XR_HIP(c, hipMemcpyAsync(c->d_mlp_io + (size_t)rows * kMlpIn, targets, sizeof(float) * rows * 3,
+                             hipMemcpyHostToDevice, c->stream));
+    return XR_OK;
+}
+
+int xr_mlp_gradient(xr_ctx* c, const float* inputs, const float* targets, uint32_t rows, float* grad_out) {
+    if (!c || !inputs || !targets || !grad_out || rows == 0)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument or no rows");
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    rc = stage_rows(c, inputs, targets, rows);
+    if (rc) return rc;
+    const size_t gbytes = sizeof(float) * (size_t)kMlpParams * 32;
+    XR_HIP(c, hipMemsetAsync(c->mlp.grad, 0, gbytes, c->stream));
+    XR_HIP(c, launch_mlp_train_rows(c->mlp, c->d_mlp_io, c->d_mlp_io + (size_t)rows * kMlpIn, rows, 1.0f, c->stream));
+    std::vector<float> parts((size_t)kMlpParams * 32);
+    XR_HIP(c, hipMemcpyAsync(parts.data(), c->mlp.grad, gbytes, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->mlp.grad, 0, gbytes, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < kMlpParams; ++i) {
+        double g = 0.0;
+        for (int p = 0; p < 32; ++p) g += parts[(size_t)p * kMlpParams + i];
+        grad_out[i] = (float)g;
+    }
+    return XR_OK;
+}
+
+int xr_mlp_train_rows(xr_ctx* c, const float* inputs, const float* targets, uint32_t rows, float lr) {
+    if (!c || !inputs || !targets || rows == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument or no rows");
+    if (!(lr >= 0.0f && lr < 1.0f)) return set_err(c, XR_ERR_INVALID_ARGUMENT, "lr out of range");
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    rc = stage_rows(c, inputs, targets, rows);
+    if (rc) return rc;
+    const float scale = 128.0f;
+    XR_HIP(c, launch_mlp_train_rows(c->mlp, c->d_mlp_io, c->d_mlp_io + (size_t)rows * kMlpIn, rows, scale,
+                                    c->stream));
+    c->mlp.steps++;
+    XR_HIP(c, launch_mlp_adam(c->mlp, lr, 1.0f / scale, true, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_mlp_params(xr_ctx* c, float* params, int set) {
+    if (!c || !params) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    if (set) {
+        XR_HIP(c, hipMemcpyAsync(c->mlp.w, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c->stream));
+        XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
+    }
+    XR_HIP(c, hipMemcpyAsync(params, c->mlp.w, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
     return XR_OK;
 }

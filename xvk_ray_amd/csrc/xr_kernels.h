@@ -7,8 +7,9 @@ namespace xr {
 
 uint32_t image_grid(uint32_t width, uint32_t height);
 
+struct CacheCfg;
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
-                                 hipStream_t st);
+                                 const CacheCfg& cache, hipStream_t st);
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
                            hipStream_t st);
 hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
@@ -21,6 +22,37 @@ hipError_t launch_sq_error(const float4* img, const double* acc, double inv_spp,
                            hipStream_t st);
 hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,
                         hipStream_t st);
+
+// MLP radiance cache state: one device allocation of mlp_bytes() bytes carved
+// by mlp_bind into 256-byte-aligned regions.
+struct MlpState {
+    float* w = nullptr;         // FP32 master parameters (23,043)
+    float* m = nullptr;         // Adam first moment
+    float* v = nullptr;         // Adam second moment
+    float* grad = nullptr;      // 32 partial gradient copies
+    uint16_t* wt_bf = nullptr;  // BF16 W^T of the hidden layers (forward)
+    uint16_t* w_bf = nullptr;   // BF16 W of hidden layers 1..4 (backward)
+    float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};  // position normalisation
+    uint32_t steps = 0;         // Adam steps taken
+};
+size_t mlp_bytes();
+size_t mlp_grad_offset_bytes();
+hipError_t mlp_bind(MlpState* st, void* base);
+hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool step, hipStream_t s);
+hipError_t launch_mlp_infer(const MlpState& st, const FrameDev& fr, const float4* materials, hipStream_t s);
+hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows, float* out, hipStream_t s);
+hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, uint32_t stride,
+                                  uint32_t offset, uint32_t nslots, float loss_scale, hipStream_t s);
+hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const float* targets, uint32_t rows,
+                                 float loss_scale, hipStream_t s);
+
+// Cache termination of the initial-sample pass (MLP mode).
+struct CacheCfg {
+    uint32_t enabled;
+    uint32_t stride;   // training pixels: pix % stride == offset
+    uint32_t offset;
+    float spread_c;    // terminate at x_s when its spread exceeds spread_c * primary footprint
+};
 
 // Output of the GPU builder.  All arrays are device allocations owned by the
 // caller after a successful build (free with hipFree).

@@ -85,14 +85,22 @@ struct PathOut {
     v3 x_view, n_view, x_s, n_s, L, f0;
     float p_q, uv_u, uv_v;
     uint32_t mat_idx, rand_seed;
-    bool valid;  // primitiveHitValid
+    uint32_t cache;   // 0: plain path, 1: ends in a cache query at x_s, 2: training path
+    uint32_t xs_mat;  // material at x_s (cache modes)
+    bool valid;       // primitiveHitValid
 };
 
 // Camera ray + bounce loop of the initial sample pass (InitialSample.rgen:41-181)
 // for pixel (x, y) of frame `frame`; returns the number of rays traced.
+// With the radiance cache enabled (no reference counterpart) the path stops
+// at its first diffuse vertex x_s when that vertex's spread
+// t1^2 / (p_q |cos theta_s|) exceeds spread_c times the primary footprint
+// t0^2 / (4 pi |cos theta_v|) (Mueller et al. 2021, "Real-time Neural
+// Radiance Caching"), leaving L_out to the cache; every stride-th pixel keeps
+// tracing to full depth and becomes a training sample.
 __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const UniformBufferObject& ubo, uint32_t x,
                                                uint32_t y, uint32_t w, uint32_t h, uint32_t frame,
-                                               uint32_t bounces, int* col, PathOut* o) {
+                                               uint32_t bounces, int* col, PathOut* o, const CacheCfg& cache) {
     uint32_t rays = 0;
     uint32_t rng = init_seed(x, y, w, frame);
     v3 origin, dir;
@@ -102,7 +110,9 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
     o->x_view = zero; o->n_view = zero; o->x_s = zero; o->n_s = zero; o->L = zero; o->f0 = zero;
     o->p_q = 0.0f; o->uv_u = 0.0f; o->uv_v = 0.0f;
     o->mat_idx = 0xFFFFFFFFu; o->rand_seed = 0;
+    o->cache = 0u; o->xs_mat = 0u;
     o->valid = false;
+    float a0 = 0.0f;  // primary footprint (cache mode)
     for (uint32_t bounce = 0;; ++bounce) {
         if (bounce == bounces) break;
         const RayPre rp = make_ray_pre(origin, dir);
@@ -132,6 +142,22 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
             break;
         }
         if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
+        if (cache.enabled) {
+            if (bounce == 0) {
+                a0 = pl.t * pl.t / (4.0f * XR_PI * fabsf(dot(pl.normal, dir)));
+            } else if (bounce == 1) {
+                o->xs_mat = (uint32_t)pl.material;
+                if ((y * w + x) % cache.stride == cache.offset) {
+                    o->cache = 2u;
+                } else {
+                    const float a1 = pl.t * pl.t / (o->p_q * fabsf(dot(pl.normal, dir)));
+                    if (a1 > cache.spread_c * a0) {
+                        o->cache = 1u;
+                        break;
+                    }
+                }
+            }
+        }
         const uint32_t seed_here = rng;
         v3 wi;
         float pdf, cos_t;
@@ -164,7 +190,8 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
     return rays;
 }
 
-__global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo) {
+__global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+                                                           CacheCfg cache) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
@@ -174,14 +201,16 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
         const uint32_t pix = y * fr.width + x;
         PathOut o;
         rays = trace_path(sc, ubo, x, y, fr.width, fr.height, ubo.currentFrame, ubo.numberOfBounces,
-                          stack + threadIdx.x, &o);
+                          stack + threadIdx.x, &o, cache);
         const uint32_t n = fr.n;
         float4* S = fr.samples;
+        // Spare words (zero without the cache): plane 3 .w = material at x_s,
+        // plane 4 .w = 1 cache query / 2 training sample.
         S[0 * n + pix] = f4(o.x_view, __uint_as_float(o.mat_idx));
         S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
         S[2 * n + pix] = f4(o.x_s, o.p_q);
-        S[3 * n + pix] = f4(o.n_s, 0.0f);
-        S[4 * n + pix] = f4(o.L, 0.0f);
+        S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
+        S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
         S[5 * n + pix] = f4(o.f0, 0.0f);
         if (o.valid) {
             fr.gpos[pix] = f4(o.x_view, 1.0f);
@@ -228,8 +257,9 @@ __global__ __launch_bounds__(kBlock) void k_converge(SceneDev sc, FrameDev fr, U
         double sx = 0.0, sy = 0.0, sz = 0.0;
         for (uint32_t s = 0; s < spp; ++s) {
             PathOut o;
+            const CacheCfg plain = {0u, 1u, 0u, 0.0f};
             rays += trace_path(sc, ubo, tc.x, tc.y, fr.width, fr.height, frame_base + s, ubo.numberOfBounces,
-                               stack + threadIdx.x, &o);
+                               stack + threadIdx.x, &o, plain);
             if (o.valid && o.p_q > 1e-6f) {
                 const v3 wi = normalize(sub(o.x_s, o.x_view));
                 const float c = maxf(0.0f, dot(o.n_view, wi));
@@ -629,8 +659,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __r
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
-                                 hipStream_t st) {
-    hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo);
+                                 const CacheCfg& cache, hipStream_t st) {
+    hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
+                       cache);
     return hipGetLastError();
 }
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,

@@ -1,0 +1,502 @@
+// MLP radiance cache kernels for gfx950 (no counterpart in the reference).
+//
+// Every GEMM of the 96-64-64-64-64-64-3 network runs on the CDNA4 MFMA
+// v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32 accumulation): one wave64
+// owns a tile of 64 rows (samples) and computes each 64x64 layer as 2x2
+// output tiles of 32x32, K in steps of 16.  Fragment layouts (verified on the
+// device): A lane l holds row l%32, k = 8*(l/32)..+7; B lane l holds
+// k = 8*(l/32)..+7 of column l%32; D element j of lane l is row
+// 8*(j/4) + 4*(l/32) + j%4, column l%32.  Activations live in LDS as BF16
+// rows padded to 16-byte-aligned strides (104 / 72 elements) so the 16-byte
+// fragment loads of consecutive lanes hit distinct banks.  The 64->3 output
+// layer is 192 FMAs per row on the VALU.
+//
+//   k_mlp_infer   cache queries of the initial-sample pass -> L_out (persistent,
+//                 weights in LDS, 4 waves per workgroup)
+//   k_mlp_train   forward + backward on the frame's training samples, FP32
+//                 gradients accumulated with atomics into 32 partial copies
+//   k_mlp_adam    sums the partials, removes the loss scale, Adam step on the
+//                 FP32 master weights, repacks the BF16 copies
+//   k_mlp_eval    forward on caller-given inputs (testing)
+#include "xr_internal.h"
+#include "xr_kernels.h"
+#include "xr_mlp.h"
+
+#include <cstddef>
+
+namespace xr {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kStrideIn = kMlpIn + 8;       // 104 (input rows)
+constexpr int kStrideHid = kMlpHidden + 8;  // 72  (hidden rows)
+// BF16 W^T copies (forward): layer 0 [64][96], layers 1..4 [64][64].
+constexpr int kWtSize = 64 * 96 + 4 * 64 * 64;
+XR_HD int wt_off(int l) { return l == 0 ? 0 : 64 * 96 + (l - 1) * 64 * 64; }
+// BF16 W copies (backward input gradients): layers 1..4 [64][64] row-major.
+constexpr int kWSize = 4 * 64 * 64;
+constexpr int kGradCopies = 32;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bf16x8 frag16(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const u32x4*)p); }
+
+__device__ __forceinline__ bf16x8 frag_gather(const uint16_t* base, int stride) {
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = __builtin_bit_cast(__bf16, base[i * stride]);
+    return r;
+}
+
+// out = relu(in . W + b) for one hidden layer on a 64-row tile.
+//   in  [64][sin] bf16 (LDS), K columns used
+//   wt  W^T [64][wstride] bf16 (LDS or global)
+//   out [64][sout] bf16 (LDS)
+template <int K>
+__device__ __forceinline__ void fwd_layer(const uint16_t* in, int sin, const uint16_t* wt, int wstride,
+                                          const float* bias, uint16_t* out, int sout, int lane) {
+    const int lr = lane & 31, lk = 8 * (lane >> 5);
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < K / 16; ++ks) {
+                const bf16x8 a = frag16(in + (tr * 32 + lr) * sin + ks * 16 + lk);
+                const bf16x8 b = frag16(wt + (tc * 32 + lr) * wstride + ks * 16 + lk);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+            const int col = tc * 32 + lr;
+            const float bj = bias[col];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                out[row * sout + col] = bf16_bits(fmaxf(acc[j] + bj, 0.0f));
+            }
+        }
+    }
+}
+
+// Output layer (64 -> 3) for the row owned by this lane.
+__device__ __forceinline__ void out_layer(const uint16_t* h, const float* w5, const float* b5, float* pred) {
+    float s0 = b5[0], s1 = b5[1], s2 = b5[2];
+    for (int k = 0; k < kMlpHidden; ++k) {
+        const float x = bf16_float(h[k]);
+        s0 = fmaf(x, w5[3 * k + 0], s0);
+        s1 = fmaf(x, w5[3 * k + 1], s1);
+        s2 = fmaf(x, w5[3 * k + 2], s2);
+    }
+    pred[0] = s0;
+    pred[1] = s1;
+    pred[2] = s2;
+}
+
+struct MlpDev {
+    const float* w;         // FP32 master parameters (kMlpParams)
+    const uint16_t* wt_bf;  // BF16 W^T per hidden layer (forward)
+    const uint16_t* w_bf;   // BF16 W per layer 1..4 (backward)
+    float* grad;            // kGradCopies x kMlpParams partial gradients
+    float* m;               // Adam first moment
+    float* v;               // Adam second moment
+    float lo[3], inv_ext[3];
+};
+
+// Features of the path vertex recorded by the initial-sample pass for pixel
+// pix: x_s, outgoing direction towards x_v, n_s, albedo of x_s's material.
+__device__ __forceinline__ void pixel_features(const MlpDev& m, const float4* __restrict__ samples, uint32_t n,
+                                               uint32_t pix, const float4* __restrict__ materials, float* f) {
+    const float4 s0 = samples[pix], s2 = samples[2 * n + pix], s3 = samples[3 * n + pix];
+    const v3 xs = mk3(s2.x, s2.y, s2.z);
+    const v3 w = normalize(sub(mk3(s0.x, s0.y, s0.z), xs));
+    const uint32_t mat = __float_as_uint(s3.w);
+    const float4 mc = materials[2 * mat];
+    mlp_encode(xs, w, mk3(s3.x, s3.y, s3.z), mk3(mc.x, mc.y, mc.z), mk3(m.lo[0], m.lo[1], m.lo[2]),
+               mk3(m.inv_ext[0], m.inv_ext[1], m.inv_ext[2]), f);
+}
+
+__device__ __forceinline__ void store_row(uint16_t* row, const float* f, int nf) {
+    for (int i = 0; i < nf; i += 2) {
+        const uint32_t pk = (uint32_t)bf16_bits(f[i]) | ((uint32_t)bf16_bits(f[i + 1]) << 16);
+        *(uint32_t*)(row + i) = pk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inference (cache queries) and evaluation
+// ---------------------------------------------------------------------------
+// LDS: padded W^T of the five hidden layers, their biases and the output
+// layer, plus two activation buffers per wave.
+// Every array starts on a 16-byte boundary (the fragment loads are 16 bytes).
+struct InferLds {
+    alignas(16) uint16_t wt[64 * kStrideIn + 4 * 64 * kStrideHid];
+    alignas(16) float bias[5 * 64];
+    alignas(16) float w5[64 * 3 + 3];
+    alignas(16) uint16_t act[4][64 * kStrideIn + 64 * kStrideHid];
+};
+static_assert(offsetof(InferLds, act) % 16 == 0 && sizeof(InferLds) <= 160 * 1024, "InferLds layout");
+
+__device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
+    for (int i = threadIdx.x; i < 64 * kMlpIn; i += blockDim.x) {
+        const int r = i / kMlpIn, c = i % kMlpIn;
+        s.wt[r * kStrideIn + c] = m.wt_bf[i];
+    }
+    for (int i = threadIdx.x; i < 4 * 64 * 64; i += blockDim.x) {
+        const int l = i / 4096, r = (i % 4096) / 64, c = i % 64;
+        s.wt[64 * kStrideIn + l * 64 * kStrideHid + r * kStrideHid + c] = m.wt_bf[64 * 96 + i];
+    }
+    for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) s.bias[i] = m.w[mlp_boff(i / 64) + i % 64];
+    for (int i = threadIdx.x; i < 64 * 3 + 3; i += blockDim.x) s.w5[i] = m.w[mlp_woff(5) + i];
+}
+
+// Runs the five hidden layers and the output layer on the tile in act[w]
+// (inputs already stored in its first buffer with stride kStrideIn).
+__device__ __forceinline__ void forward_tile(InferLds& s, int w, int lane, float* pred) {
+    uint16_t* A = s.act[w];
+    uint16_t* B = s.act[w] + 64 * kStrideIn;
+    wave_sync();
+    fwd_layer<kMlpIn>(A, kStrideIn, s.wt, kStrideIn, s.bias, B, kStrideHid, lane);
+    wave_sync();
+    const uint16_t* wt1 = s.wt + 64 * kStrideIn;
+    fwd_layer<64>(B, kStrideHid, wt1 + 0 * 64 * kStrideHid, kStrideHid, s.bias + 64, A, kStrideHid, lane);
+    wave_sync();
+    fwd_layer<64>(A, kStrideHid, wt1 + 1 * 64 * kStrideHid, kStrideHid, s.bias + 128, B, kStrideHid, lane);
+    wave_sync();
+    fwd_layer<64>(B, kStrideHid, wt1 + 2 * 64 * kStrideHid, kStrideHid, s.bias + 192, A, kStrideHid, lane);
+    wave_sync();
+    fwd_layer<64>(A, kStrideHid, wt1 + 3 * 64 * kStrideHid, kStrideHid, s.bias + 256, B, kStrideHid, lane);
+    wave_sync();
+    out_layer(B + lane * kStrideHid, s.w5, s.w5 + 192, pred);
+    wave_sync();
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const float4* __restrict__ materials) {
+    __shared__ InferLds s;
+    load_weights_lds(m, s);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t n = fr.n;
+    const uint32_t ntiles = (n + 63) / 64;
+    for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+        const uint32_t pix = tile * 64 + lane;
+        bool query = false;
+        float f[kMlpIn];
+        if (pix < n) query = __float_as_uint(fr.samples[4 * n + pix].w) == 1u;
+        if (query) pixel_features(m, fr.samples, n, pix, materials, f);
+        else
+            for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
+        store_row(s.act[w] + lane * kStrideIn, f, kMlpIn);
+        float pred[3];
+        forward_tile(s, w, lane, pred);
+        if (query) {
+            const float4 old = fr.samples[4 * n + pix];
+            fr.samples[4 * n + pix] =
+                make_float4(fmaxf(pred[0], 0.0f), fmaxf(pred[1], 0.0f), fmaxf(pred[2], 0.0f), old.w);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_eval(MlpDev m, const float* __restrict__ feats, uint32_t rows,
+                                                  float* __restrict__ out) {
+    __shared__ InferLds s;
+    load_weights_lds(m, s);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t ntiles = (rows + 63) / 64;
+    for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+        const uint32_t r = tile * 64 + lane;
+        float f[kMlpIn];
+        for (int i = 0; i < kMlpIn; ++i) f[i] = r < rows ? feats[(size_t)r * kMlpIn + i] : 0.0f;
+        store_row(s.act[w] + lane * kStrideIn, f, kMlpIn);
+        float pred[3];
+        forward_tile(s, w, lane, pred);
+        if (r < rows)
+            for (int c = 0; c < 3; ++c) out[(size_t)r * 3 + c] = pred[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: forward + backward of 64-sample tiles, one wave per workgroup
+// ---------------------------------------------------------------------------
+struct TrainLds {
+    alignas(16) uint16_t x[64 * kStrideIn];
+    alignas(16) uint16_t h[5][64 * kStrideHid];  // outputs of hidden layers 0..4
+    alignas(16) uint16_t dz[64 * kStrideHid];
+    alignas(16) float g[64 * 4];
+};
+
+// Loss: relative L2, sum over samples and channels of (p - y)^2 / (sg(p)^2 + 0.01),
+// times inv_count.  g = loss_scale * dLoss/dp.
+template <bool kFromFrame>
+__global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __restrict__ samples, uint32_t n,
+                                                  uint32_t stride, uint32_t offset, uint32_t nslots,
+                                                  const float4* __restrict__ materials,
+                                                  const float* __restrict__ feats, const float* __restrict__ targets,
+                                                  float loss_scale, float inv_count) {
+    __shared__ TrainLds s;
+    const int lane = threadIdx.x;
+    const int lr = lane & 31, lk = 8 * (lane >> 5);
+    float* grad = m.grad + (size_t)(blockIdx.x % kGradCopies) * kMlpParams;
+    const uint32_t ntiles = (nslots + 63) / 64;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t slot = tile * 64 + lane;
+        float f[kMlpIn];
+        float y[3] = {0.f, 0.f, 0.f};
+        bool valid = false;
+        if (kFromFrame) {
+            const uint32_t pix = slot * stride + offset;
+            if (slot < nslots && pix < n) {
+                const float4 s4 = samples[4 * n + pix];
+                if (__float_as_uint(s4.w) == 2u) {
+                    valid = true;
+                    y[0] = s4.x; y[1] = s4.y; y[2] = s4.z;
+                    pixel_features(m, samples, n, pix, materials, f);
+                }
+            }
+        } else if (slot < nslots) {
+            valid = true;
+            for (int i = 0; i < kMlpIn; ++i) f[i] = feats[(size_t)slot * kMlpIn + i];
+            for (int c = 0; c < 3; ++c) y[c] = targets[(size_t)slot * 3 + c];
+        }
+        if (!valid)
+            for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
+        store_row(s.x + lane * kStrideIn, f, kMlpIn);
+        wave_sync();
+        // forward, keeping every hidden activation
+        fwd_layer<kMlpIn>(s.x, kStrideIn, m.wt_bf + wt_off(0), kMlpIn, m.w + mlp_boff(0), s.h[0], kStrideHid, lane);
+        for (int l = 1; l < 5; ++l) {
+            wave_sync();
+            fwd_layer<64>(s.h[l - 1], kStrideHid, m.wt_bf + wt_off(l), 64, m.w + mlp_boff(l), s.h[l], kStrideHid,
+                          lane);
+        }
+        wave_sync();
+        float pred[3];
+        out_layer(s.h[4] + lane * kStrideHid, m.w + mlp_woff(5), m.w + mlp_boff(5), pred);
+        for (int c = 0; c < 3; ++c) {
+            const float d = pred[c] - y[c];
+            const float gc = valid ? loss_scale * inv_count * 2.0f * d / (pred[c] * pred[c] + 0.01f) : 0.0f;
+            s.g[lane * 4 + c] = gc;
+        }
+        wave_sync();
+        // output layer: dW5[k][c] = sum_r h4[r][k] g[r][c]; db5[c] = sum_r g[r][c]
+        {
+            const float* w5 = m.w + mlp_woff(5);
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+            for (int r = 0; r < 64; ++r) {
+                const float x = bf16_float(s.h[4][r * kStrideHid + lane]);
+                a0 = fmaf(x, s.g[r * 4 + 0], a0);
+                a1 = fmaf(x, s.g[r * 4 + 1], a1);
+                a2 = fmaf(x, s.g[r * 4 + 2], a2);
+            }
+            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 0], a0);
+            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 1], a1);
+            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 2], a2);
+            if (lane < 3) {
+                float b = 0.f;
+                for (int r = 0; r < 64; ++r) b += s.g[r * 4 + lane];
+                atomicAdd(&grad[mlp_boff(5) + lane], b);
+            }
+            // dZ4 = (g . W5^T) masked by h4 > 0, for this lane's row
+            const float g0 = s.g[lane * 4 + 0], g1 = s.g[lane * 4 + 1], g2 = s.g[lane * 4 + 2];
+            for (int k = 0; k < 64; ++k) {
+                const float dh = g0 * w5[3 * k] + g1 * w5[3 * k + 1] + g2 * w5[3 * k + 2];
+                const bool on = bf16_float(s.h[4][lane * kStrideHid + k]) > 0.0f;
+                s.dz[lane * kStrideHid + k] = bf16_bits(on ? dh : 0.0f);
+            }
+        }
+        wave_sync();
+        // hidden layers 4..0: dW_l = in^T . dz, db_l = colsum(dz), dz_prev = (dz . W_l^T) * (in > 0)
+        for (int l = 4; l >= 0; --l) {
+            const uint16_t* in = l == 0 ? s.x : s.h[l - 1];
+            const int sin = l == 0 ? kStrideIn : kStrideHid;
+            const int K = l == 0 ? kMlpIn : 64;
+            const int wo = mlp_woff(l);
+            for (int tm = 0; tm < K / 32; ++tm) {
+                for (int tn = 0; tn < 2; ++tn) {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        const bf16x8 a = frag_gather(in + (ks * 16 + lk) * sin + tm * 32 + lr, sin);
+                        const bf16x8 b = frag_gather(s.dz + (ks * 16 + lk) * kStrideHid + tn * 32 + lr, kStrideHid);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                    }
+                    const int col = tn * 32 + lr;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int row = tm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                        atomicAdd(&grad[wo + row * 64 + col], acc[j]);
+                    }
+                }
+            }
+            {
+                float b = 0.f;
+                for (int r = 0; r < 64; ++r) b += bf16_float(s.dz[r * kStrideHid + lane]);
+                atomicAdd(&grad[mlp_boff(l) + lane], b);
+            }
+            if (l == 0) break;
+            // dH_in = dz . W_l^T  (W_l row-major [64][64] from the BF16 copy)
+            const uint16_t* wl = m.w_bf + (l - 1) * 4096;
+            f32x16 dh[2][2];
+#pragma unroll
+            for (int tr = 0; tr < 2; ++tr) {
+#pragma unroll
+                for (int tc = 0; tc < 2; ++tc) {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        const bf16x8 a = frag16(s.dz + (tr * 32 + lr) * kStrideHid + ks * 16 + lk);
+                        const bf16x8 b = frag16(wl + (tc * 32 + lr) * 64 + ks * 16 + lk);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                    }
+                    dh[tr][tc] = acc;
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int tr = 0; tr < 2; ++tr) {
+#pragma unroll
+                for (int tc = 0; tc < 2; ++tc) {
+                    const int col = tc * 32 + lr;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                        const bool on = bf16_float(in[row * sin + col]) > 0.0f;
+                        s.dz[row * kStrideHid + col] = bf16_bits(on ? dh[tr][tc][j] : 0.0f);
+                    }
+                }
+            }
+            wave_sync();
+        }
+        wave_sync();
+    }
+}
+
+// Adam on the FP32 master weights (one thread per parameter), then the BF16
+// copies.  The partial gradients are summed, unscaled and cleared.
+__global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, float* __restrict__ w, uint16_t* __restrict__ wt_bf,
+                                                  uint16_t* __restrict__ w_bf, float lr, float beta1, float beta2,
+                                                  float eps, float inv_scale, float bc1, float bc2, int do_step) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kMlpParams) return;
+    float wi = w[i];
+    if (do_step) {
+        float g = 0.0f;
+        for (int p = 0; p < kGradCopies; ++p) {
+            g += m.grad[(size_t)p * kMlpParams + i];
+            m.grad[(size_t)p * kMlpParams + i] = 0.0f;
+        }
+        g *= inv_scale;
+        if (isfinite(g)) {
+            const float mi = beta1 * m.m[i] + (1.0f - beta1) * g;
+            const float vi = beta2 * m.v[i] + (1.0f - beta2) * g * g;
+            m.m[i] = mi;
+            m.v[i] = vi;
+            wi -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+            w[i] = wi;
+        }
+    }
+    // repack: which layer / weight or bias
+    for (int l = 0; l < 5; ++l) {
+        const int wo = mlp_woff(l), in = mlp_in(l);
+        if (i >= wo && i < wo + in * 64) {
+            const int k = (i - wo) / 64, j = (i - wo) % 64;
+            wt_bf[wt_off(l) + j * in + k] = bf16_bits(wi);
+            if (l >= 1) w_bf[(l - 1) * 4096 + k * 64 + j] = bf16_bits(wi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static MlpDev mlp_dev(const MlpState& st) {
+    MlpDev m;
+    m.w = st.w;
+    m.wt_bf = st.wt_bf;
+    m.w_bf = st.w_bf;
+    m.grad = st.grad;
+    m.m = st.m;
+    m.v = st.v;
+    for (int a = 0; a < 3; ++a) {
+        m.lo[a] = st.lo[a];
+        m.inv_ext[a] = st.inv_ext[a];
+    }
+    return m;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+static const size_t kParamBytes = align256(sizeof(float) * kMlpParams);
+
+size_t mlp_bytes() {
+    return 3 * kParamBytes + align256(sizeof(float) * (size_t)kMlpParams * kGradCopies) +
+           align256(sizeof(uint16_t) * kWtSize) + align256(sizeof(uint16_t) * kWSize);
+}
+
+size_t mlp_grad_offset_bytes() { return 3 * kParamBytes; }
+
+// Regions (each 256-byte aligned, so every 16-byte BF16 fragment load is
+// aligned): w | m | v | grad[32] | wt_bf | w_bf.
+hipError_t mlp_bind(MlpState* st, void* base) {
+    char* p = (char*)base;
+    st->w = (float*)p;
+    p += kParamBytes;
+    st->m = (float*)p;
+    p += kParamBytes;
+    st->v = (float*)p;
+    p += kParamBytes;
+    st->grad = (float*)p;
+    p += align256(sizeof(float) * (size_t)kMlpParams * kGradCopies);
+    st->wt_bf = (uint16_t*)p;
+    p += align256(sizeof(uint16_t) * kWtSize);
+    st->w_bf = (uint16_t*)p;
+    return hipSuccess;
+}
+
+hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool step, hipStream_t s) {
+    MlpDev m = mlp_dev(st);
+    const float b1 = 0.9f, b2 = 0.99f;
+    const float bc1 = 1.0f - powf(b1, (float)st.steps);
+    const float bc2 = 1.0f - powf(b2, (float)st.steps);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, m, st.w, st.wt_bf, st.w_bf, lr,
+                       b1, b2, 1e-8f, inv_scale, step ? bc1 : 1.0f, step ? bc2 : 1.0f, step ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const MlpState& st, const FrameDev& fr, const float4* materials, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(256), 0, s, mlp_dev(st), fr, materials);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows, float* out, hipStream_t s) {
+    const uint32_t blocks = (rows + 255) / 256 < 256 ? (rows + 255) / 256 : 256;
+    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks ? blocks : 1), dim3(256), 0, s, mlp_dev(st), feats, rows, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, uint32_t stride,
+                                  uint32_t offset, uint32_t nslots, float loss_scale, hipStream_t s) {
+    const uint32_t tiles = (nslots + 63) / 64;
+    const uint32_t blocks = tiles < 512 ? tiles : 512;
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(blocks ? blocks : 1), dim3(64), 0, s, mlp_dev(st), fr.samples, fr.n,
+                       stride, offset, nslots, materials, (const float*)nullptr, (const float*)nullptr, loss_scale,
+                       1.0f / (3.0f * (float)(nslots ? nslots : 1)));
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const float* targets, uint32_t rows,
+                                 float loss_scale, hipStream_t s) {
+    const uint32_t tiles = (rows + 63) / 64;
+    const uint32_t blocks = tiles < 512 ? tiles : 512;
+    hipLaunchKernelGGL(k_mlp_train<false>, dim3(blocks ? blocks : 1), dim3(64), 0, s, mlp_dev(st),
+                       (const float4*)nullptr, 0u, 1u, 0u, rows, (const float4*)nullptr, feats, targets, loss_scale,
+                       1.0f / (3.0f * (float)(rows ? rows : 1)));
+    return hipGetLastError();
+}
+
+}  // namespace xr

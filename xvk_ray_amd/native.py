@@ -41,6 +41,9 @@ XR_BUF_MOTION = 15
 XR_BUF_RADIANCE = 16
 XR_BUF_CONVERGED = 17
 XR_FLAG_RADIANCE = 1
+XR_FLAG_MLP_CACHE = 2
+XR_MLP_INPUTS = 96
+XR_MLP_PARAMS = 23043
 XR_LAYOUT_REFERENCE = 0
 XR_LAYOUT_NATIVE = 1
 
@@ -75,7 +78,8 @@ class XrSceneDesc(C.Structure):
 class XrRestirParams(C.Structure):
     _fields_ = [("temporal_m_cap", C.c_uint32), ("spatial_m_cap", C.c_uint32), ("spatial_samples", C.c_uint32),
                 ("spatial_radius", C.c_float), ("visibility", C.c_uint32), ("plain_max_depth", C.c_uint32),
-                ("passes", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+                ("passes", C.c_uint32), ("flags", C.c_uint32), ("mlp_spread", C.c_float), ("mlp_lr", C.c_float),
+                ("mlp_train_pixels", C.c_uint32), ("mlp_loss_scale", C.c_float), ("reserved", C.c_uint32 * 4)]
 
 
 class XrAccelInfo(C.Structure):
@@ -113,6 +117,7 @@ XR_API_SYMBOLS = [
     "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
     "xr_reset_counters", "xr_get_accel_info",
     "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
+    "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_params",
     "xr_last_error", "xr_global_last_error",
 ]
 XR_HOST_SYMBOLS = [
@@ -168,6 +173,11 @@ def lib() -> C.CDLL:
         L.xr_read_accel.argtypes = [P, P, C.c_size_t, P, C.c_size_t]
         L.xr_stream.argtypes = [P]
         L.xr_stream.restype = P
+        L.xr_mlp_reset.argtypes = [P, C.c_uint64]
+        L.xr_mlp_eval.argtypes = [P, P, U32, P]
+        L.xr_mlp_gradient.argtypes = [P, P, P, U32, P]
+        L.xr_mlp_train_rows.argtypes = [P, P, P, U32, C.c_float]
+        L.xr_mlp_params.argtypes = [P, P, I]
         L.xr_last_error.argtypes = [P]
         L.xr_last_error.restype = C.c_char_p
         L.xr_global_last_error.argtypes = []
@@ -226,7 +236,10 @@ def _fptr(a: np.ndarray):
 def default_params(**overrides) -> XrRestirParams:
     p = XrRestirParams()
     lib().xr_default_params(C.byref(p))
+    names = {f[0] for f in XrRestirParams._fields_}
     for k, v in overrides.items():
+        if k not in names:
+            raise AttributeError(f"xr_restir_params has no field {k!r}")
         setattr(p, k, v)
     return p
 
@@ -445,11 +458,44 @@ class Context:
         self.check(lib().xr_rmse(self._h, C.byref(r)))
         return float(r.value)
 
+    # ---- MLP radiance cache ----
+    def mlp_reset(self, seed: int = 1) -> None:
+        self.check(lib().xr_mlp_reset(self._h, seed))
+
+    def mlp_eval(self, inputs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, XR_MLP_INPUTS)
+        out = np.zeros((x.shape[0], 3), np.float32)
+        self.check(lib().xr_mlp_eval(self._h, x.ctypes.data, x.shape[0], out.ctypes.data))
+        return out
+
+    def mlp_gradient(self, inputs: np.ndarray, targets: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, XR_MLP_INPUTS)
+        y = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        assert x.shape[0] == y.shape[0]
+        g = np.zeros(XR_MLP_PARAMS, np.float32)
+        self.check(lib().xr_mlp_gradient(self._h, x.ctypes.data, y.ctypes.data, x.shape[0], g.ctypes.data))
+        return g
+
+    def mlp_train_rows(self, inputs: np.ndarray, targets: np.ndarray, lr: float = 1e-3) -> None:
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, XR_MLP_INPUTS)
+        y = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        assert x.shape[0] == y.shape[0]
+        self.check(lib().xr_mlp_train_rows(self._h, x.ctypes.data, y.ctypes.data, x.shape[0], lr))
+
+    def mlp_params(self, params: np.ndarray | None = None) -> np.ndarray:
+        """FP32 master parameters; writes `params` into the network first when given."""
+        buf = np.zeros(XR_MLP_PARAMS, np.float32)
+        if params is not None:
+            buf[:] = np.asarray(params, np.float32).reshape(-1)
+        self.check(lib().xr_mlp_params(self._h, buf.ctypes.data, 0 if params is None else 1))
+        return buf
+
     def pass_time_totals(self) -> tuple[list[float], int]:
-        ms = (C.c_double * 3)()
+        """Summed per-pass milliseconds (initial or plain, temporal, spatial, cache) and frames."""
+        ms = (C.c_double * 4)()
         frames = C.c_uint32()
-        self.check(lib().xr_pass_time_totals(self._h, ms, 3, C.byref(frames)))
-        return [ms[i] for i in range(3)], int(frames.value)
+        self.check(lib().xr_pass_time_totals(self._h, ms, 4, C.byref(frames)))
+        return [ms[i] for i in range(4)], int(frames.value)
 
     def reset_pass_times(self) -> None:
         self.check(lib().xr_reset_pass_times(self._h))
