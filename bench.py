@@ -40,6 +40,21 @@ FOV = 45.0
 #   spatial : centre x_v, n_v, n_s, matID 40 + centre reservoir 84 + reservoir write 84 + output 4
 ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 
+# HBM bytes per launch of each pass's kernel on this workload, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1
+# --no-cpu-baseline --no-mlp-study` (KiB per dispatch, averaged over the
+# dispatches), each scaled by what the counter reads for a plain 1 GiB
+# device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
+# 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
+# computed here; the committed pmc summary under profiles/ holds the runs.
+PMC_FETCH_KIB = {"initial": 25641.8, "temporal": 236240.4, "spatial": 2686174.1}
+PMC_WRITE_KIB = {"initial": 301253.7, "temporal": 215801.4, "spatial": 190428.1}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def pmc_traffic(name: str) -> int:
+    return int((PMC_FETCH_KIB[name] * FETCH_SCALE + PMC_WRITE_KIB[name] * WRITE_SCALE) * 1024)
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -48,7 +63,99 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--no-mlp-study", action="store_true")
+    ap.add_argument("--study-frames", type=int, default=256)
+    ap.add_argument("--conv-spp", type=int, default=1024)
     return ap.parse_args()
+
+
+# MLP FLOPs: 2 * (96*64 + 4*64*64 + 64*3) = 45,440 per query; a training
+# sample adds the backward pass (about 2x the forward, plus the output layer):
+# 136,320.
+MLP_FLOP_QUERY = 45440
+MLP_FLOP_TRAIN = 136320
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+SPREADS = (0.01, 1.0, 10.0, 100.0)
+
+
+def mlp_study(scene, camera, args) -> dict:
+    """Equal-RMSE comparison of ReSTIR GI with and without the MLP radiance
+    cache on the same 1080p scene, camera held at the path's first view.
+
+    Each run starts from empty history (and, with the cache, a freshly
+    initialised network trained online one Adam step per frame on about
+    65,536 path suffixes).  RMSE of the pre-tone-mapping radiance against the
+    FP64 converged image is taken at checkpoint frames; ms/frame is the sum
+    of the GPU pass times (initial, cache, temporal, spatial) over all frames.
+    The cache is swept over the spread threshold c (paths stop at x_s when
+    its spread exceeds c times the primary footprint): larger c stops fewer
+    paths, costing time and removing cache error."""
+    import numpy as np
+
+    from xvk_ray_amd import native as N
+
+    view = camera.view(0)
+    ctx = N.Context(scene, W, H)
+    frames = args.study_frames
+    checkpoints = sorted({f for f in (1, 4, 16, 64, 128, 256, frames) if f <= frames})
+    ubo0 = N.make_ubo(view, view, FOV, W, H, bounces=BOUNCES, frame=1)
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.conv_spp:
+        k = min(64, args.conv_spp - done)
+        ctx.converge(ubo0, k, (1 << 20) + done)
+        ctx.sync()
+        done += k
+        if done % 256 == 0 or done == args.conv_spp:
+            log(f"converged image {done}/{args.conv_spp} spp ({time.perf_counter() - t0:.1f} s)")
+    conv = ctx.converged_mean()
+    conv_s = time.perf_counter() - t0
+
+    def run(cache: bool, spread: float) -> dict:
+        ctx.resize(W, H)  # empty history
+        if cache:
+            ctx.mlp_reset(1)
+        flags = N.XR_FLAG_RADIANCE | (N.XR_FLAG_MLP_CACHE if cache else 0)
+        p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, mlp_spread=spread, **PARAMS)
+        ctx.reset_counters()
+        ctx.reset_pass_times()
+        rmse = {}
+        for f in range(1, frames + 1):
+            ctx.render(N.make_ubo(view, view, FOV, W, H, bounces=BOUNCES, frame=f), N.XR_MODE_RESTIR, p)
+            if f in checkpoints:
+                rad = ctx.read(N.XR_BUF_RADIANCE)[:, :3].astype(np.float64)
+                rmse[f] = round(float(np.sqrt(np.mean((rad - conv) ** 2))), 6)
+        ctx.sync()
+        pass_ms, nfr = ctx.pass_time_totals()
+        ext, shadow, valid, queries, train = ctx.counters_all()
+        ms = sum(pass_ms) / nfr
+        out = {"ms_per_frame": round(ms, 4),
+               "pass_ms": {k: round(v / nfr, 4) for k, v in zip(("initial", "temporal", "spatial", "cache"), pass_ms)},
+               "rays_per_frame": (ext + shadow) // nfr, "rmse": rmse}
+        if cache:
+            flop = (MLP_FLOP_QUERY * queries + MLP_FLOP_TRAIN * train) / nfr
+            tflops = flop / (pass_ms[3] / nfr * 1e-3) / 1e12 if pass_ms[3] > 0 else 0.0
+            out.update({"spread_c": spread, "queries_per_frame": queries // nfr,
+                        "train_samples_per_frame": train // nfr,
+                        "mlp": {"gflop_per_frame": round(flop / 1e9, 3), "achieved_TFLOPs": round(tflops, 2),
+                                "frac_mfma_bf16": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)}})
+        log(f"{'cache c=' + str(spread) if cache else 'plain'}: {ms:.3f} ms/frame, rmse {rmse}")
+        return out
+
+    plain = run(False, 0.0)
+    sweep = [run(True, c) for c in SPREADS]
+    last = checkpoints[-1]
+    ok = [r for r in sweep if r["rmse"][last] <= plain["rmse"][last]]
+    best = min(ok, key=lambda r: r["ms_per_frame"]) if ok else None
+    ctx.close()
+    return {
+        "camera": "orbit path frame 0, held still", "frames": frames, "converged_spp": args.conv_spp,
+        "converged_seconds": round(conv_s, 2), "rmse_at": last,
+        "plain": plain, "cache": sweep,
+        "time_ratio_at_equal_rmse": (round(best["ms_per_frame"] / plain["ms_per_frame"], 4) if best else None),
+        "equal_rmse_setting": best["spread_c"] if best else None,
+        "target_ratio": 0.6,
+    }
 
 
 def cpu_baseline(scene, app, ctx, target_s: float) -> dict:
@@ -176,7 +283,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": per_pass[dom]["frac_hbm"],
-        "traffic": None,
+        "traffic": pmc_traffic(dom),
         "algo_bytes_per_launch": ALGO_BYTES[dom] * W * H,
     }
     result = {
@@ -211,6 +318,9 @@ def main():
         "roofline": roofline,
         "setup_s": round(setup_s, 2),
     }
+    if not args.no_mlp_study:
+        log("MLP radiance cache equal-RMSE study")
+        result["mlp_cache_study"] = mlp_study(scene, camera, args)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on a pixel sample)")
         result["cpu_baseline"] = cpu_baseline(scene, app, ctx, args.cpu_seconds)

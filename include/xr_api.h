@@ -263,8 +263,9 @@ int xr_reset_pass_times(xr_ctx* ctx);
 
 /* Device counters since xr_create / the last reset: [0] extend rays
  * (camera + bounce), [1] shadow rays, [2] pixels whose primary ray found a
- * non-emissive surface (primitiveHitValid of the initial sample pass).
- * Waits for queued work. */
+ * non-emissive surface (primitiveHitValid of the initial sample pass),
+ * [3] radiance-cache queries, [4] radiance-cache training samples.  Writes
+ * min(n, 5) values.  Waits for queued work. */
 int xr_counters(xr_ctx* ctx, uint64_t* out, uint32_t n);
 int xr_reset_counters(xr_ctx* ctx);
 

@@ -214,7 +214,9 @@ def test_cache_frame_matches_oracle():
     w_before = ctx.mlp_params()
     frame = 4
     say(f"[cache-frame] {W}x{H} frame {frame}")
+    ctx.reset_counters()
     ubo = _cache_frame(ctx, W, H, frame, params)
+    counts = ctx.counters_all()
     w_after = ctx.mlp_params()
     assert not np.array_equal(w_before, w_after)  # the frame trained the network
     gpu = ctx.read(N.XR_BUF_SAMPLES)
@@ -224,6 +226,7 @@ def test_cache_frame_matches_oracle():
     assert np.array_equal(gpu["pad2"], kind) and np.array_equal(gpu["pad1"], ref["pad1"])
     query = kind == 1
     assert query.sum() > 0.1 * W * H and (kind == 2).sum() > 0
+    assert counts[3] == query.sum() and counts[4] == (kind == 2).sum()
     # everything but the cached radiance is bit-exact
     g2, r2 = gpu.copy(), ref.copy()
     g2["l_out"][query] = 0

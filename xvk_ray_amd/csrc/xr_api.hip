@@ -41,6 +41,10 @@ static_assert(sizeof(xr_restir_sample) == sizeof(ReSTIRGISample), "xr_restir_sam
 static_assert(sizeof(xr_restir_reservoir) == sizeof(ReSTIRGIReservoir), "xr_restir_reservoir layout");
 static_assert(sizeof(xr_ray) == 32 && sizeof(xr_hit) == 32, "ray / hit records are two float4");
 
+// Device counters: extend rays, shadow rays, valid primary hits, cache
+// queries, cache training samples.
+constexpr uint32_t kCounters = 5;
+
 struct xr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -72,6 +76,7 @@ struct xr_ctx {
     uint32_t* d_motion = nullptr;
     uint32_t* d_output = nullptr;
     float4* d_radiance = nullptr;
+    uint32_t* d_cache_list = nullptr;  // radiance-cache work lists (2 + 2n words)
     unsigned long long* d_counters = nullptr;
 
     // converged image: FP64 RGB sums per pixel, samples accumulated, error scratch
@@ -165,6 +170,7 @@ void free_frame(xr_ctx* c) {
     dev_free(&c->d_motion);
     dev_free(&c->d_output);
     dev_free(&c->d_radiance);
+    dev_free(&c->d_cache_list);
     dev_free(&c->d_conv);
     c->conv_spp = 0;
     c->width = c->height = 0;
@@ -184,6 +190,8 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_motion, n * 4));
     XR_HIP(c, dev_alloc(&c->d_output, n * 4));
     XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
+    XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
+    XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_temporal, 0, 2 * kPlanes * n * f4, c->stream));
@@ -228,6 +236,7 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.accum = c->d_accum;
     f.radiance = nullptr;
     f.counters = c->d_counters;
+    f.cache_list = c->d_cache_list;
     return f;
 }
 
@@ -456,7 +465,7 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     XR_CREATE_HIP(dev_alloc(&c->d_materials, sizeof(xr_material) * (size_t)s->material_count));
     XR_CREATE_HIP(dev_alloc(&c->d_offsets, sizeof(uint32_t) * 2 * (size_t)s->instance_count));
     XR_CREATE_HIP(dev_alloc(&c->d_inst_first, sizeof(uint32_t) * first.size()));
-    XR_CREATE_HIP(dev_alloc(&c->d_counters, sizeof(unsigned long long) * 4));
+    XR_CREATE_HIP(dev_alloc(&c->d_counters, sizeof(unsigned long long) * kCounters));
     XR_CREATE_HIP(hipMemcpyAsync(c->d_vertices, s->vertices, sizeof(xr_vertex) * (size_t)s->vertex_count,
                                  hipMemcpyHostToDevice, c->stream));
     XR_CREATE_HIP(hipMemcpyAsync(c->d_indices, s->indices, sizeof(uint32_t) * (size_t)s->index_count,
@@ -467,7 +476,7 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
                                  hipMemcpyHostToDevice, c->stream));
     XR_CREATE_HIP(hipMemcpyAsync(c->d_inst_first, first.data(), sizeof(uint32_t) * first.size(),
                                  hipMemcpyHostToDevice, c->stream));
-    XR_CREATE_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    XR_CREATE_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
     XR_CREATE_HIP(hipStreamSynchronize(c->stream));
 #undef XR_CREATE_HIP
     rc = alloc_frame(c, width, height);
@@ -581,15 +590,14 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
             cache.spread_c = p.mlp_spread;
         }
         XR_HIP(c, hipEventRecord(ev[0], c->stream));
+        if (cache_on && (passes & XR_PASS_INITIAL)) XR_HIP(c, launch_clear_u32(fr.cache_list, 2, c->stream));
         if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, cache, c->stream));
         XR_HIP(c, hipEventRecord(ev[1], c->stream));
         if (cache_on && (passes & XR_PASS_INITIAL)) {
             // infer with the current weights, then one training step on this
             // frame's path suffixes
             XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
-            const uint32_t nslots = (fr.n + cache.stride - 1) / cache.stride;
-            XR_HIP(c, launch_mlp_train_frame(c->mlp, fr, c->d_materials, cache.stride, cache.offset, nslots,
-                                             p.mlp_loss_scale, c->stream));
+            XR_HIP(c, launch_mlp_train_frame(c->mlp, fr, c->d_materials, p.mlp_loss_scale, c->stream));
             c->mlp.steps++;
             XR_HIP(c, launch_mlp_adam(c->mlp, p.mlp_lr, 1.0f / p.mlp_loss_scale, true, c->stream));
         }
@@ -648,16 +656,16 @@ int xr_reset_pass_times(xr_ctx* c) {
 
 int xr_counters(xr_ctx* c, uint64_t* out, uint32_t n) {
     if (!c || !out) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
-    unsigned long long h[4];
+    unsigned long long h[kCounters];
     XR_HIP(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
-    for (uint32_t i = 0; i < n && i < 3; ++i) out[i] = h[i];
+    for (uint32_t i = 0; i < n && i < kCounters; ++i) out[i] = h[i];
     return XR_OK;
 }
 
 int xr_reset_counters(xr_ctx* c) {
     if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    XR_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * 4, c->stream));
+    XR_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
     return XR_OK;
 }
 

@@ -33,7 +33,10 @@ struct FrameDev {
     uint32_t* output;           // binding 2, B8G8R8A8_UNORM
     float4* accum;              // binding 1, RGBA32F
     float4* radiance;           // linear colour behind the output (no reference binding), or null
-    unsigned long long* counters;  // [0] extend rays, [1] shadow rays
+    unsigned long long* counters;  // [0] extend rays, [1] shadow rays, [2] valid hits, [3] / [4] cache
+    // Radiance-cache work lists of the frame: [0] query count, [1] training
+    // count, [2, 2+n) query pixels, [2+n, 2+2n) training pixels.
+    uint32_t* cache_list;
 };
 
 // Parameters of one ReSTIR GI / plain frame as the kernels see them.

@@ -39,10 +39,11 @@ size_t mlp_bytes();
 size_t mlp_grad_offset_bytes();
 hipError_t mlp_bind(MlpState* st, void* base);
 hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool step, hipStream_t s);
+hipError_t launch_clear_u32(uint32_t* p, uint32_t n, hipStream_t s);
 hipError_t launch_mlp_infer(const MlpState& st, const FrameDev& fr, const float4* materials, hipStream_t s);
 hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows, float* out, hipStream_t s);
-hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, uint32_t stride,
-                                  uint32_t offset, uint32_t nslots, float loss_scale, hipStream_t s);
+hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, float loss_scale,
+                                  hipStream_t s);
 hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const float* targets, uint32_t rows,
                                  float loss_scale, hipStream_t s);
 

@@ -47,6 +47,19 @@ uint32_t image_grid(uint32_t width, uint32_t height) {
     return ((tiles + kXcds - 1) / kXcds) * kXcds;
 }
 
+// Appends `value` of every lane whose pred is set to list (one global atomic
+// per wave, slots in lane order).  Called by every lane of the wave.
+__device__ __forceinline__ void wave_append(uint32_t* count, uint32_t* list, bool pred, uint32_t value) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 // One LDS atomic per lane, one global atomic per workgroup.
 __device__ __forceinline__ void count_rays(unsigned long long* counter, uint32_t n, uint32_t* lds_count) {
     if (threadIdx.x == 0) *lds_count = 0;
@@ -195,13 +208,14 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
-    uint32_t rays = 0, valid = 0;
+    uint32_t rays = 0, valid = 0, kind = 0;
     if (tc.in_image) {
         const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * fr.width + x;
         PathOut o;
         rays = trace_path(sc, ubo, x, y, fr.width, fr.height, ubo.currentFrame, ubo.numberOfBounces,
                           stack + threadIdx.x, &o, cache);
+        kind = o.cache;
         const uint32_t n = fr.n;
         float4* S = fr.samples;
         // Spare words (zero without the cache): plane 3 .w = material at x_s,
@@ -235,6 +249,23 @@ __global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev
     }
     count_rays(&fr.counters[0], rays, &lds_count);
     count_rays(&fr.counters[2], valid, &lds_count);
+    if (cache.enabled) {  // uniform over the launch
+        const uint32_t pix = tc.y * fr.width + tc.x;
+        wave_append(&fr.cache_list[0], fr.cache_list + 2, kind == 1u, pix);
+        wave_append(&fr.cache_list[1], fr.cache_list + 2 + fr.n, kind == 2u, pix);
+        count_rays(&fr.counters[3], kind == 1u ? 1u : 0u, &lds_count);
+        count_rays(&fr.counters[4], kind == 2u ? 1u : 0u, &lds_count);
+    }
+}
+
+__global__ void k_clear_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_clear_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

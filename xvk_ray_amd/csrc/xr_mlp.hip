@@ -182,12 +182,16 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const 
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = fr.n;
-    const uint32_t ntiles = (n + 63) / 64;
+    // the initial pass listed this frame's query pixels
+    const uint32_t count = fr.cache_list[0];
+    const uint32_t* list = fr.cache_list + 2;
+    const uint32_t ntiles = (count + 63) / 64;
     for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
-        const uint32_t pix = tile * 64 + lane;
+        const uint32_t idx = tile * 64 + lane;
+        const uint32_t pix = idx < count ? list[idx] : 0u;
         bool query = false;
         float f[kMlpIn];
-        if (pix < n) query = __float_as_uint(fr.samples[4 * n + pix].w) == 1u;
+        if (idx < count) query = __float_as_uint(fr.samples[4 * n + pix].w) == 1u;
         if (query) pixel_features(m, fr.samples, n, pix, materials, f);
         else
             for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
@@ -233,9 +237,13 @@ struct TrainLds {
 
 // Loss: relative L2, sum over samples and channels of (p - y)^2 / (sg(p)^2 + 0.01),
 // times inv_count.  g = loss_scale * dLoss/dp.
+// kFromFrame: the samples are the frame's training pixels listed by the
+// initial pass (*count of them in list); otherwise nslots rows of
+// caller-given features and targets.  inv_count = 1 / (3 * sample count).
 template <bool kFromFrame>
 __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __restrict__ samples, uint32_t n,
-                                                  uint32_t stride, uint32_t offset, uint32_t nslots,
+                                                  const uint32_t* __restrict__ count,
+                                                  const uint32_t* __restrict__ list, uint32_t nslots,
                                                   const float4* __restrict__ materials,
                                                   const float* __restrict__ feats, const float* __restrict__ targets,
                                                   float loss_scale, float inv_count) {
@@ -243,6 +251,10 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
     const int lane = threadIdx.x;
     const int lr = lane & 31, lk = 8 * (lane >> 5);
     float* grad = m.grad + (size_t)(blockIdx.x % kGradCopies) * kMlpParams;
+    if (kFromFrame) {
+        nslots = *count;
+        inv_count = 1.0f / (3.0f * (float)(nslots ? nslots : 1u));
+    }
     const uint32_t ntiles = (nslots + 63) / 64;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint32_t slot = tile * 64 + lane;
@@ -250,8 +262,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
         float y[3] = {0.f, 0.f, 0.f};
         bool valid = false;
         if (kFromFrame) {
-            const uint32_t pix = slot * stride + offset;
-            if (slot < nslots && pix < n) {
+            const uint32_t pix = slot < nslots ? list[slot] : n;
+            if (pix < n) {
                 const float4 s4 = samples[4 * n + pix];
                 if (__float_as_uint(s4.w) == 2u) {
                     valid = true;
@@ -479,13 +491,13 @@ hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, uint32_t stride,
-                                  uint32_t offset, uint32_t nslots, float loss_scale, hipStream_t s) {
-    const uint32_t tiles = (nslots + 63) / 64;
-    const uint32_t blocks = tiles < 512 ? tiles : 512;
-    hipLaunchKernelGGL(k_mlp_train<true>, dim3(blocks ? blocks : 1), dim3(64), 0, s, mlp_dev(st), fr.samples, fr.n,
-                       stride, offset, nslots, materials, (const float*)nullptr, (const float*)nullptr, loss_scale,
-                       1.0f / (3.0f * (float)(nslots ? nslots : 1)));
+hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, float loss_scale,
+                                  hipStream_t s) {
+    // The sample count is on the device (cache_list[1]); 256 workgroups
+    // stride over its 64-sample tiles.
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(256), dim3(64), 0, s, mlp_dev(st), fr.samples, fr.n,
+                       (const uint32_t*)(fr.cache_list + 1), (const uint32_t*)(fr.cache_list + 2 + fr.n), 0u,
+                       materials, (const float*)nullptr, (const float*)nullptr, loss_scale, 0.0f);
     return hipGetLastError();
 }
 
@@ -494,8 +506,9 @@ hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const f
     const uint32_t tiles = (rows + 63) / 64;
     const uint32_t blocks = tiles < 512 ? tiles : 512;
     hipLaunchKernelGGL(k_mlp_train<false>, dim3(blocks ? blocks : 1), dim3(64), 0, s, mlp_dev(st),
-                       (const float4*)nullptr, 0u, 1u, 0u, rows, (const float4*)nullptr, feats, targets, loss_scale,
-                       1.0f / (3.0f * (float)(rows ? rows : 1)));
+                       (const float4*)nullptr, 0u, (const uint32_t*)nullptr, (const uint32_t*)nullptr, rows,
+                       (const float4*)nullptr, feats,
+                       targets, loss_scale, 1.0f / (3.0f * (float)(rows ? rows : 1)));
     return hipGetLastError();
 }
 

@@ -506,9 +506,13 @@ class Context:
 
     def counters3(self) -> tuple[int, int, int]:
         """(extend rays, shadow rays, valid primary hits) since the last reset."""
-        c = (C.c_uint64 * 3)()
-        self.check(lib().xr_counters(self._h, c, 3))
-        return int(c[0]), int(c[1]), int(c[2])
+        return self.counters_all()[:3]
+
+    def counters_all(self) -> tuple[int, int, int, int, int]:
+        """(extend rays, shadow rays, valid primary hits, cache queries, cache training samples)."""
+        c = (C.c_uint64 * 5)()
+        self.check(lib().xr_counters(self._h, c, 5))
+        return tuple(int(c[i]) for i in range(5))
 
     def reset_counters(self) -> None:
         self.check(lib().xr_reset_counters(self._h))
