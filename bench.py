@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--no-mlp-study", action="store_true")
     ap.add_argument("--study-frames", type=int, default=256)
     ap.add_argument("--conv-spp", type=int, default=1024)
+    ap.add_argument("--workload", choices=("c2", "c1", "c5"), default="c2",
+                    help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
+                         "Cornell box; c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain")
     return ap.parse_args()
 
 
@@ -227,6 +230,158 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def base_result(args, metric, value, unit, ms, dtype, config) -> dict:
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config}
+
+
+def run_c1(args) -> dict:
+    """Plain 1-spp path tracing, 640x360, Cornell box (24 quad tris + 1,280-tri
+    icosphere + ceiling light), pinhole 60 degrees at (0.5, 0.5, -1.2), max
+    depth 3, progressive accumulation into the RGBA32F image.  The sample cap
+    (maxSpp) is set above warm-up + timed frames so every frame traces (the
+    driver stops tracing once the cap is reached, pathTracer.cpp:71-78)."""
+    from xvk_ray_amd import native as N
+
+    Wc, Hc = 640, 360
+    frames = args.warmup + args.steps
+    scene = N.Scene.cornell(1)
+    camera = N.Camera.static((0.5, 0.5, -1.2), (0.5, 0.5, 0.5))
+    app = N.App(scene, camera, Wc, Hc, mode=N.XR_MODE_PLAIN, spp=1, bounces=3, max_spp=frames + 1, fov=60.0,
+                accumulate=True, sky=False)
+    ctx = app.context()
+    params = N.default_params(plain_max_depth=3)
+    for _ in range(args.warmup):
+        app.draw_frame(params)
+    ctx.sync()
+    steps = args.steps
+    ctx.reset_counters()
+    assert args.steps >= 1
+    ctx.reset_pass_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        app.draw_frame(params)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ext, _, _ = ctx.counters3()
+    pass_ms, nfr = ctx.pass_time_totals()
+    ms = wall * 1e3 / steps
+    res = base_result(args, "ms/frame & Mrays/s @640x360 plain path tracing", round(ext / wall / 1e6, 2), "Mrays/s",
+                      round(ms, 4), "f32",
+                      {"workload": "plain_pt_640x360_cornell", "width": Wc, "height": Hc,
+                       "triangles": scene.triangle_count, "max_depth": 3, "spp": 1, "accumulated_frames": frames,
+                       "parallelism": "single_gpu"})
+    res.update({"ms_per_frame": round(ms, 4), "kernel_ms_per_frame": round(pass_ms[0] / nfr, 4),
+                "rays_per_frame": ext // steps})
+    log(f"c1: {ms:.4f} ms/frame (kernel {pass_ms[0] / nfr:.4f} ms), {ext // steps} rays/frame")
+    if not args.no_cpu_baseline:
+        import numpy as np
+
+        from oracle.binding import Oracle, OracleScene
+        L = Oracle.lib()
+        threads = min(L.orc_num_threads(), 16)
+        L.orc_set_threads(threads)
+        orc = OracleScene(*scene.arrays())
+        ub = N.ubo_to_bytes(app.last_ubo())
+        accum = np.zeros((Wc * Hc, 4), np.float32)
+        out = np.zeros(Wc * Hc, np.uint32)
+        total, rays, reps = 0.0, 0, 0
+        while total < args.cpu_seconds and reps < 1000:
+            t = time.perf_counter()
+            rays += orc.plain(ub, Wc, Hc, 3, accum, out)
+            total += time.perf_counter() - t
+            reps += 1
+        res["cpu_baseline"] = {"value": round(rays / total / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "ms_per_frame": round(total * 1e3 / reps, 2),
+                               "sample": f"{reps} full 640x360 frames, {total:.1f} s; C++ restatement of the "
+                                         f"shaders, not lavapipe"}
+    return res
+
+
+def run_c5(args) -> dict:
+    """3840x2160 ReSTIR GI, max depth 6, on the large terrain (2048x2048-vertex
+    fBm heightfield + 4,096 icospheres of 1,280 tris + 256 quad lights, seed
+    1), camera on the seeded 128-frame path; plain ReSTIR GI vs ReSTIR GI with
+    the MLP radiance cache in FP8 E4M3 weights.  RMSE of each method's
+    radiance against an FP64 converged image of the same camera at frames 1,
+    16 and 128."""
+    import numpy as np
+
+    from xvk_ray_amd import native as N
+
+    W5, H5, depth = 3840, 2160, 6
+    frames = 128
+    checkpoints = (1, 16, 128)
+    t = time.perf_counter()
+    scene = N.Scene.terrain(1, 2048, 4096, 256)
+    camera = N.Camera.orbit(2, 8, frames)
+    log(f"c5 scene {scene.triangle_count} tris ({time.perf_counter() - t:.1f} s)")
+    saved = {}
+
+    def run(cache: bool) -> dict:
+        app = N.App(scene, camera, W5, H5, mode=N.XR_MODE_RESTIR, spp=1, bounces=depth, fov=FOV, sky=False,
+                    reuse=True)
+        ctx = app.context()
+        info = ctx.accel_info()
+        flags = N.XR_FLAG_RADIANCE | ((N.XR_FLAG_MLP_CACHE | N.XR_FLAG_MLP_FP8) if cache else 0)
+        p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, **PARAMS)
+        ctx.reset_counters()
+        ctx.reset_pass_times()
+        t0 = time.perf_counter()
+        for f in range(1, frames + 1):
+            app.draw_frame(p)
+            if f in checkpoints:
+                saved[(cache, f)] = (ctx.read(N.XR_BUF_RADIANCE)[:, :3].astype(np.float64), app.last_ubo())
+                log(f"c5 {'cache' if cache else 'plain'} frame {f}")
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        pass_ms, nfr = ctx.pass_time_totals()
+        ext, shadow, valid, queries, train = ctx.counters_all()
+        out = {"ms_per_frame_gpu": round(sum(pass_ms) / nfr, 4),
+               "pass_ms": {k: round(v / nfr, 4) for k, v in zip(("initial", "temporal", "spatial", "cache"), pass_ms)},
+               "rays_per_frame": (ext + shadow) // nfr, "mrays_per_s": round((ext + shadow) / wall / 1e6, 1),
+               "bvh": {"nodes": info.node_count, "depth": info.max_depth, "build_ms": round(info.build_ms, 1)}}
+        if cache:
+            out.update({"queries_per_frame": queries // nfr, "train_samples_per_frame": train // nfr,
+                        "mlp_gflop_per_frame": round((MLP_FLOP_QUERY * queries + MLP_FLOP_TRAIN * train)
+                                                     / nfr / 1e9, 3)})
+        app.close()
+        return out
+
+    res_plain = run(False)
+    res_cache = run(True)
+    # converged images of the checkpoint cameras
+    ctx = N.Context(scene, W5, H5)
+    rmse = {False: {}, True: {}}
+    for f in checkpoints:
+        ubo = saved[(False, f)][1]
+        ctx.converge_reset()
+        done = 0
+        while done < args.conv_spp:
+            k = min(32, args.conv_spp - done)
+            ctx.converge(ubo, k, (1 << 20) + done)
+            ctx.sync()
+            done += k
+        log(f"c5 converged image for frame {f}: {done} spp")
+        conv = ctx.converged_mean()
+        for cache in (False, True):
+            rad = saved[(cache, f)][0]
+            rmse[cache][f] = round(float(np.sqrt(np.mean((rad - conv) ** 2))), 6)
+    ctx.close()
+    res_plain["rmse"], res_cache["rmse"] = rmse[False], rmse[True]
+    ms = res_cache["ms_per_frame_gpu"]
+    res = base_result(args, "ms/frame & Mrays/s @3840x2160 ReSTIR GI + FP8 MLP radiance cache",
+                      res_cache["mrays_per_s"], "Mrays/s", ms, "f32+fp8",
+                      {"workload": "restir_gi_mlp_fp8_4k_large_terrain", "width": W5, "height": H5,
+                       "triangles": scene.triangle_count, "max_depth": depth, "frames": frames,
+                       "camera": "seeded 128-frame orbit", "parallelism": "single_gpu"})
+    res["steps"], res["warmup"] = frames, 0
+    res.update({"plain_restir": res_plain, "mlp_cache_fp8": res_cache, "converged_spp": args.conv_spp,
+                "time_ratio_cache_vs_plain": round(ms / res_plain["ms_per_frame_gpu"], 4)})
+    return res
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -234,6 +389,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "requested_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "c1":
+        print(json.dumps(run_c1(args)))
+        return
+    if args.workload == "c5":
+        print(json.dumps(run_c5(args)))
+        return
 
     from xvk_ray_amd import native as N
 

@@ -162,6 +162,12 @@ typedef struct xr_restir_params {
  * FP32 accumulation).  Paths whose x_s is too close to their primary footprint
  * fall back to plain tracing. */
 #define XR_FLAG_MLP_CACHE 0x2u
+/* flags: with XR_FLAG_MLP_CACHE, the cache's hidden-layer weights are FP8
+ * E4M3 with one power-of-two scale per layer (dequantised exactly to BF16 for
+ * the MFMA; activations stay BF16, accumulation FP32).  Training keeps the
+ * FP32 master weights and backpropagates through the quantised values.
+ * Without it the weights are BF16. */
+#define XR_FLAG_MLP_FP8 0x4u
 
 /* Fills p with the reference's compile-time constants (30 / 300 / 6 / 5.0,
  * always-visible, unbounded plain depth, all passes). */
@@ -325,9 +331,21 @@ int xr_mlp_eval(xr_ctx* ctx, const float* inputs, uint32_t rows, float* outputs)
 int xr_mlp_gradient(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float* grad_out);
 /* One Adam step (lr; loss scale as in the frame) on the given data. */
 int xr_mlp_train_rows(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float lr);
-/* Copies the FP32 master parameters (XR_MLP_PARAMS floats; per layer W[in][out]
- * row-major then bias) to params; if not NULL, writes them from params first
- * when set != 0. */
+#define XR_MLP_WEIGHTS_BF16 0
+#define XR_MLP_WEIGHTS_FP8 1
+#define XR_MLP_FP8_BYTES 22528 /* E4M3 W^T of the hidden layers: [64][96] then 4 x [64][64] */
+/* Weight format of the network's GEMM operands for xr_mlp_eval,
+ * xr_mlp_gradient and xr_mlp_train_rows (a frame with XR_FLAG_MLP_CACHE
+ * selects FP8 when XR_FLAG_MLP_FP8 is set, BF16 otherwise). */
+int xr_mlp_set_weight_format(xr_ctx* ctx, int format);
+/* In the FP8 format: the E4M3 codes (XR_MLP_FP8_BYTES; layer l's W^T is
+ * row j = output unit, column k = input) and the 5 per-layer scales.
+ * XR_ERR_STATE in the BF16 format. */
+int xr_mlp_fp8_weights(xr_ctx* ctx, uint8_t* codes, float* scales);
+/* FP32 master parameters (XR_MLP_PARAMS floats; per layer W[in][out]
+ * row-major, then the bias).  set != 0: the network takes params first (and
+ * its GEMM copies are rebuilt); then the parameters are copied back into
+ * params.  Synchronous. */
 int xr_mlp_params(xr_ctx* ctx, float* params, int set);
 
 /* ---- errors ---- */

@@ -81,6 +81,10 @@ class Oracle:
             L.orc_mlp_loss_grad.restype = C.c_double
             L.orc_adam_step.argtypes = [P, P, P, P, U32, C.c_double, C.c_double, C.c_double, C.c_double, U32]
             L.orc_adam_step.restype = None
+            L.orc_e4m3_round.argtypes = [P, U32, P, P]
+            L.orc_e4m3_round.restype = None
+            L.orc_mlp_quantize.argtypes = [P, P, P]
+            L.orc_mlp_quantize.restype = None
             L.orc_temporal.argtypes = [U32, U32, U32, U32, P, U32, P, P, P, P, P, P]
             L.orc_temporal.restype = None
             L.orc_spatial.argtypes = [P, U32, U32, U32, U32, U32, F, U32, I, P, U32, P, P, P, P]
@@ -191,6 +195,24 @@ def bf16_round(x) -> np.ndarray:
     return out
 
 
+def e4m3_round(x):
+    """(codes uint8, values float32) of FP8 E4M3 round-to-nearest-even with saturation."""
+    a = np.ascontiguousarray(x, np.float32)
+    codes = np.zeros(a.size, np.uint8)
+    vals = np.zeros(a.size, np.float32)
+    Oracle.lib().orc_e4m3_round(a.ctypes.data, a.size, codes.ctypes.data, vals.ctypes.data)
+    return codes, vals
+
+
+def mlp_quantize(params):
+    """(E4M3 codes of the hidden layers' W^T, 22,528 bytes; 5 per-layer scales)."""
+    w = np.ascontiguousarray(params, np.float32)
+    codes = np.zeros(22528, np.uint8)
+    scales = np.zeros(5, np.float32)
+    Oracle.lib().orc_mlp_quantize(w.ctypes.data, codes.ctypes.data, scales.ctypes.data)
+    return codes, scales
+
+
 def mlp_layer_offsets(layer: int) -> tuple[int, int]:
     """(weight offset, bias offset) of a layer in the parameter vector."""
     L = Oracle.lib()
@@ -205,7 +227,8 @@ def mlp_encode(p, w, n, a, lo, inv_ext) -> np.ndarray:
 
 
 def mlp_forward(params, x, mode: int = 1) -> np.ndarray:
-    """mode 0: exact FP64 network; mode 1: the kernels' BF16 rounding points."""
+    """mode 0: exact FP64 network; mode 1: the kernels' BF16 rounding points;
+    mode 2: mode 1 with FP8 E4M3 hidden-layer weights."""
     w = np.ascontiguousarray(params, np.float32)
     xx = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
     out = np.zeros((len(xx), 3), np.float64)

@@ -14,16 +14,23 @@
 //     (Mueller et al. 2021, "Real-time Neural Radiance Caching", section 5);
 //   * Adam (Kingma & Ba 2015, algorithm 1) with bias correction;
 //   * BF16 round to nearest even, defined by choosing the nearer of the two
-//     BF16 neighbours of x (ties to the even significand), not by bit tricks.
+//     BF16 neighbours of x (ties to the even significand), not by bit tricks;
+//   * FP8 E4M3 (OCP E4M3FN: bias 7, no infinities, largest 448) round to
+//     nearest even with saturation, defined as the nearest entry of the
+//     table of all finite E4M3 values (ties to the even code), and the
+//     per-layer scale: the smallest power of two 2^k with max|W| / 2^k <= 448.
 //
 // Precision modes of the forward pass:
 //   mode 0  exact network in FP64 (the finite-difference reference);
 //   mode 1  the rounding points of the HIP kernels: inputs, hidden weights and
 //           hidden activations rounded to BF16 (FP64 dot products, bias added
-//           in FP32), output layer in FP32 fmaf from the bias.
+//           in FP32), output layer in FP32 fmaf from the bias;
+//   mode 2  as mode 1 with the hidden weights quantised to FP8 E4M3 times the
+//           layer scale (exact in BF16).
 // The backward pass runs in FP64 on the activations of the forward pass; in
 // mode 1 the backpropagated pre-activation gradients are rounded to BF16 where
 // the kernels round them.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -58,6 +65,66 @@ float bf16(float x) {
     if (dl < dh) return lo;
     if (dh < dl) return hi;
     return ((lo_bits >> 16) & 1u) ? hi : lo;  // tie: even significand
+}
+
+// The 127 finite non-negative E4M3 values (codes 0x00-0x7E) in increasing order.
+struct E4m3Table {
+    double value[127];
+    uint8_t code[127];
+    int n = 0;
+    E4m3Table() {
+        for (int c = 0; c < 0x7F; ++c) {  // 0x7F is NaN
+            const int E = c >> 3, m = c & 7;
+            value[n] = E == 0 ? m * std::ldexp(1.0, -9) : (1.0 + m / 8.0) * std::ldexp(1.0, E - 7);
+            code[n] = (uint8_t)c;
+            ++n;
+        }
+    }
+};
+const E4m3Table& e4m3_table() {
+    static const E4m3Table t;
+    return t;
+}
+
+uint8_t e4m3_code(float x) {
+    const E4m3Table& t = e4m3_table();
+    const double a = std::fabs((double)x);
+    const uint8_t sign = std::signbit(x) ? 0x80 : 0x00;
+    if (a >= t.value[t.n - 1]) return sign | t.code[t.n - 1];  // saturate at 448
+    int lo = 0;
+    while (lo + 1 < t.n && t.value[lo + 1] <= a) ++lo;
+    if (t.value[lo] == a) return sign | t.code[lo];
+    const double dl = a - t.value[lo], dh = t.value[lo + 1] - a;
+    const int pick = dl < dh ? lo : dh < dl ? lo + 1 : ((t.code[lo] & 1) ? lo + 1 : lo);
+    return sign | t.code[pick];
+}
+
+double e4m3_value(uint8_t c) {
+    const double v = e4m3_table().value[c & 0x7F];
+    return (c & 0x80) ? -v : v;
+}
+
+float layer_scale(const float* w, int l) {
+    float mx = 0.0f;
+    for (int i = 0; i < lin(l) * lout(l); ++i) mx = std::max(mx, std::fabs(w[woff(l) + i]));
+    if (!(mx > 0.0f)) return 1.0f;
+    double s = 1.0;  // smallest power of two with mx / s <= 448
+    while ((double)mx / s > 448.0) s *= 2.0;
+    while ((double)mx / (s / 2.0) <= 448.0) s /= 2.0;
+    return (float)s;
+}
+
+// Parameters with the hidden-layer weights replaced by their FP8 values.
+std::vector<float> fp8_params(const float* w) {
+    std::vector<float> q(w, w + kParams);
+    for (int l = 0; l < 5; ++l) {
+        const float s = layer_scale(w, l);
+        for (int i = 0; i < lin(l) * kHid; ++i) {
+            const int k = woff(l) + i;
+            q[k] = (float)(e4m3_value(e4m3_code((float)((double)w[k] / s))) * s);
+        }
+    }
+    return q;
 }
 
 struct Trace {
@@ -139,8 +206,34 @@ void orc_mlp_encode(const float* p, const float* w, const float* n, const float*
     for (int i = 81; i < kIn; ++i) out[i] = 0.0f;
 }
 
+void orc_e4m3_round(const float* x, uint32_t n, uint8_t* codes, float* values) {
+    for (uint32_t i = 0; i < n; ++i) {
+        codes[i] = e4m3_code(x[i]);
+        values[i] = (float)e4m3_value(codes[i]);
+    }
+}
+
+// FP8 form of the hidden layers: codes of W^T ([64][in] per layer, layers
+// back to back, 22,528 bytes) and the 5 scales.
+void orc_mlp_quantize(const float* w, uint8_t* codes, float* scales) {
+    size_t o = 0;
+    for (int l = 0; l < 5; ++l) {
+        const float s = layer_scale(w, l);
+        scales[l] = s;
+        for (int j = 0; j < kHid; ++j)
+            for (int k = 0; k < lin(l); ++k) codes[o++] = e4m3_code((float)((double)w[woff(l) + k * kHid + j] / s));
+    }
+}
+
 // rows x 96 inputs -> rows x 3 outputs.
-void orc_mlp_forward(const float* w, const float* x, uint32_t rows, int mode, double* out) {
+void orc_mlp_forward(const float* w_in, const float* x, uint32_t rows, int mode, double* out) {
+    std::vector<float> wq;
+    const float* w = w_in;
+    if (mode == 2) {
+        wq = fp8_params(w_in);
+        w = wq.data();
+        mode = 1;
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < (int64_t)rows; ++r) {
         Trace t;
@@ -151,7 +244,14 @@ void orc_mlp_forward(const float* w, const float* x, uint32_t rows, int mode, do
 
 // Loss and its gradient over the parameters (FP64).  grad may be NULL.
 // Returns the loss mean((p - y)^2 / (p^2 + 0.01)) over rows x 3.
-double orc_mlp_loss_grad(const float* w, const float* x, const float* y, uint32_t rows, int mode, double* grad) {
+double orc_mlp_loss_grad(const float* w_in, const float* x, const float* y, uint32_t rows, int mode, double* grad) {
+    std::vector<float> wq;
+    const float* w = w_in;
+    if (mode == 2) {  // straight-through: gradients w.r.t. the master weights through the quantised ones
+        wq = fp8_params(w_in);
+        w = wq.data();
+        mode = 1;
+    }
     const double inv_count = 1.0 / (3.0 * (double)rows);
     double loss = 0.0;
     std::vector<double> g_total(grad ? kParams : 0, 0.0);

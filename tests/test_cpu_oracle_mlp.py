@@ -81,6 +81,54 @@ def test_bf16_matches_torch_rne():
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_e4m3_matches_torch_rne():
+    rng = np.random.default_rng(40)
+    finite = torch.arange(0, 127, dtype=torch.uint8).view(torch.float8_e4m3fn).to(torch.float32).numpy()
+    mids = ((finite[1:] + finite[:-1]) / 2).astype(np.float32)  # exact ties
+    x = np.concatenate([rng.uniform(-448, 448, 20000), rng.normal(size=20000) * 10.0 ** rng.integers(-4, 2, 20000),
+                        finite, -finite, mids, -mids, np.nextafter(mids, 0), np.nextafter(mids, 1000)])
+    x = np.clip(x, -448, 448).astype(np.float32)
+    codes, vals = B.e4m3_round(x)
+    ref = torch.from_numpy(x).to(torch.float8_e4m3fn)
+    assert np.array_equal(codes, ref.view(torch.uint8).numpy())
+    assert np.array_equal(vals, ref.to(torch.float32).numpy())
+    # saturation (torch makes NaN beyond 448; the kernels clamp)
+    c, v = B.e4m3_round(np.array([449.0, 1e6, -1e30], np.float32))
+    assert list(v) == [448.0, 448.0, -448.0] and list(c) == [0x7E, 0x7E, 0xFE]
+
+
+def test_fp8_layer_quantisation():
+    w = he_params(41)
+    w[B.mlp_layer_offsets(2)[0]:B.mlp_layer_offsets(2)[1]] *= 3000.0  # a layer that needs a scale > 1
+    codes, scales = B.mlp_quantize(w)
+    off = 0
+    for l in range(5):
+        wo, bo = B.mlp_layer_offsets(l)
+        fi = 96 if l == 0 else 64
+        W = w[wo:bo].reshape(fi, 64)
+        s = float(scales[l])
+        assert s == 2.0 ** round(np.log2(s))  # power of two
+        assert np.abs(W).max() / s <= 448.0 < np.abs(W).max() / (s / 2)
+        c = codes[off:off + 64 * fi].reshape(64, fi)  # W^T
+        ref_codes, ref_vals = B.e4m3_round((W / np.float32(s)).astype(np.float32).T)
+        assert np.array_equal(c.reshape(-1), ref_codes)
+        # quantisation error: at most half an E4M3 ulp (2^-4 relative) above the subnormal range
+        deq = ref_vals.reshape(64, fi).T * s
+        big = np.abs(W) > s * 2.0 ** -6
+        assert np.all(np.abs(deq - W)[big] <= np.abs(W)[big] * 2.0 ** -4 + 1e-12)
+        off += 64 * fi
+    assert off == 22528
+
+
+def test_fp8_forward_close_to_bf16():
+    w = he_params(42)
+    x = random_inputs(500, 43)
+    b = B.mlp_forward(w, x, mode=1)
+    f = B.mlp_forward(w, x, mode=2)
+    err = np.abs(f - b).max() / np.abs(b).max()
+    assert 1e-3 < err < 0.2
+
+
 def test_encoding_layout():
     p = np.array([0.3, -1.7, 2.2], np.float32)
     lo = np.array([-2.0, -3.0, 0.5], np.float32)

@@ -174,6 +174,74 @@ def test_training_reduces_loss():
     assert loss1 < 0.01 * loss0 and loss1 < 0.1
 
 
+# ---------------------------------------------------------------------------
+# FP8 E4M3 weights
+# ---------------------------------------------------------------------------
+def test_fp8_codes_and_scales_bit_exact():
+    ctx = _ctx()
+    w = he_params(50)
+    w[B.mlp_layer_offsets(2)[0]:B.mlp_layer_offsets(2)[1]] *= 3000.0
+    ctx.mlp_params(w)
+    with pytest.raises(N.XrError):
+        ctx.mlp_fp8_weights()  # BF16 format: no FP8 copy
+    ctx.mlp_set_weight_format(N.XR_MLP_WEIGHTS_FP8)
+    codes, scales = ctx.mlp_fp8_weights()
+    ref_codes, ref_scales = B.mlp_quantize(w)
+    assert np.array_equal(scales, ref_scales)
+    assert np.array_equal(codes, ref_codes), int((codes != ref_codes).sum())
+    # an Adam step re-quantises from the updated master weights
+    x = random_inputs(512, 51)
+    y = np.ones((512, 3), np.float32)
+    ctx.mlp_train_rows(x, y, 1e-3)
+    codes2, scales2 = ctx.mlp_fp8_weights()
+    rc2, rs2 = B.mlp_quantize(ctx.mlp_params())
+    assert np.array_equal(codes2, rc2) and np.array_equal(scales2, rs2)
+    assert not np.array_equal(codes2, codes)
+
+
+@pytest.mark.parametrize("rows", [64, 5000])
+def test_fp8_forward_and_gradient_match_oracle(rows):
+    ctx = _ctx()
+    w = _positive_params(52)
+    ctx.mlp_params(w)
+    ctx.mlp_set_weight_format(N.XR_MLP_WEIGHTS_FP8)
+    x = random_inputs(rows, 53)
+    say(f"[mlp-fp8] rows={rows}")
+    gpu = ctx.mlp_eval(x).astype(np.float64)
+    ref = B.mlp_forward(w, x, mode=2)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(gpu - ref).max() <= 1e-2 * scale, float(np.abs(gpu - ref).max())
+    bf = B.mlp_forward(w, x, mode=1)
+    assert np.abs(gpu - bf).max() > np.abs(gpu - ref).max()  # the FP8 weights are in use
+    y = np.random.default_rng(54).uniform(0.0, 8.0, (rows, 3)).astype(np.float32)
+    g = ctx.mlp_gradient(x, y).astype(np.float64)
+    _, gr = B.mlp_loss_grad(w, x, y, mode=2)
+    for l, wo, end in _layers():
+        a, b = g[wo:end], gr[wo:end]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert rel < 5e-2 and cos > 0.998, (l, rel, cos)
+
+
+def test_fp8_training_reduces_loss():
+    ctx = _ctx()
+    ctx.mlp_reset(3)
+    ctx.mlp_set_weight_format(N.XR_MLP_WEIGHTS_FP8)
+    rng = np.random.default_rng(60)
+    rows = 8192
+    lo, inv = np.zeros(3, np.float32), np.ones(3, np.float32)
+    pos = rng.uniform(0, 1, (rows, 3)).astype(np.float32)
+    nrm = np.array([0, 1, 0], np.float32)
+    x = np.stack([B.mlp_encode(p, nrm, nrm, np.full(3, 0.5, np.float32), lo, inv) for p in pos])
+    y = np.stack([1.0 + np.sin(3 * pos[:, 0]), 0.5 + pos[:, 1], 2.0 * pos[:, 2] ** 2], 1).astype(np.float32)
+    loss0, _ = B.mlp_loss_grad(ctx.mlp_params(), x, y, mode=2, grad=False)
+    for _ in range(300):
+        ctx.mlp_train_rows(x, y, 1e-3)
+    loss1, _ = B.mlp_loss_grad(ctx.mlp_params(), x, y, mode=2, grad=False)
+    say(f"[mlp-fp8-train] loss {loss0:.4f} -> {loss1:.4f}")
+    assert loss1 < 0.01 * loss0 and loss1 < 0.2
+
+
 def _cache_frame(ctx, W, H, frame, params):
     view = look_view(CORNELL_EYE, CORNELL_TARGET)
     ubo = N.make_ubo(view, view, 45.0, W, H, bounces=4, frame=frame)
@@ -195,10 +263,11 @@ def _features(samples, mats, lo, inv, idx):
     return x
 
 
-def test_cache_frame_matches_oracle():
+@pytest.mark.parametrize("fp8", [False, True])
+def test_cache_frame_matches_oracle(fp8):
     """One cache frame: termination flags and every traced field bit-exact vs
     the oracle; query pixels carry max(0, network(features)) computed with the
-    weights before the frame's training step."""
+    weights before the frame's training step (BF16 or FP8 weights)."""
     scene = N.Scene.cornell(1)
     W, H = 64, 48
     ctx = N.Context(scene, W, H)
@@ -207,7 +276,8 @@ def test_cache_frame_matches_oracle():
     lo = verts["pos"].min(0).astype(np.float32)
     inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
     stride = 8
-    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, mlp_train_pixels=W * H // stride, mlp_spread=0.01)
+    flags = N.XR_FLAG_MLP_CACHE | (N.XR_FLAG_MLP_FP8 if fp8 else 0)
+    params = N.default_params(flags=flags, mlp_train_pixels=W * H // stride, mlp_spread=0.01)
     # train a few frames first so the network is not at its initial state
     for frame in range(1, 4):
         _cache_frame(ctx, W, H, frame, params)
@@ -233,7 +303,7 @@ def test_cache_frame_matches_oracle():
     eq = bytes_equal_rows(g2, r2)
     assert eq.all(), describe_mismatch("cache-frame samples", g2, r2, eq)
     idx = np.nonzero(query)[0]
-    pred = np.maximum(B.mlp_forward(w_before, _features(ref, mats, lo, inv, idx), mode=1), 0.0)
+    pred = np.maximum(B.mlp_forward(w_before, _features(ref, mats, lo, inv, idx), mode=2 if fp8 else 1), 0.0)
     got = gpu["l_out"][idx].astype(np.float64)
     scale = max(1.0, np.abs(pred).max())
     assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())

@@ -318,7 +318,7 @@ int ensure_mlp_io(xr_ctx* c, size_t floats) {
 int check_params(xr_ctx* c, const xr_restir_params* p) {
     for (int i = 0; i < 4; ++i)
         if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
-    if (p->flags & ~(XR_FLAG_RADIANCE | XR_FLAG_MLP_CACHE))
+    if (p->flags & ~(XR_FLAG_RADIANCE | XR_FLAG_MLP_CACHE | XR_FLAG_MLP_FP8))
         return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.flags has unknown bits");
     if (p->flags & XR_FLAG_MLP_CACHE) {
         if (!(p->mlp_spread >= 0.0f) || !(p->mlp_lr >= 0.0f && p->mlp_lr < 1.0f) || p->mlp_train_pixels == 0 ||
@@ -583,6 +583,11 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
         if (cache_on) {
             rc = ensure_mlp(c);
             if (rc) return rc;
+            const bool fp8 = (p.flags & XR_FLAG_MLP_FP8) != 0;
+            if (c->mlp.fp8 != fp8) {  // rebuild the GEMM copies in the requested format
+                c->mlp.fp8 = fp8;
+                XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
+            }
             const uint32_t n = fr.n;
             cache.enabled = 1u;
             cache.stride = p.mlp_train_pixels >= n ? 1u : n / p.mlp_train_pixels;
@@ -1002,6 +1007,27 @@ int xr_mlp_params(xr_ctx* c, float* params, int set) {
         XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
     }
     XR_HIP(c, hipMemcpyAsync(params, c->mlp.w, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_mlp_set_weight_format(xr_ctx* c, int format) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (format != XR_MLP_WEIGHTS_BF16 && format != XR_MLP_WEIGHTS_FP8)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown weight format %d", format);
+    int rc = ensure_mlp(c);
+    if (rc) return rc;
+    c->mlp.fp8 = format == XR_MLP_WEIGHTS_FP8;
+    XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+int xr_mlp_fp8_weights(xr_ctx* c, uint8_t* codes, float* scales) {
+    if (!c || !codes || !scales) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->d_mlp || !c->mlp.fp8) return set_err(c, XR_ERR_STATE, "the network is not in the FP8 weight format");
+    XR_HIP(c, hipMemcpyAsync(codes, c->mlp.wt_f8, XR_MLP_FP8_BYTES, hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipMemcpyAsync(scales, c->mlp.f8_scale, 5 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
     return XR_OK;
 }

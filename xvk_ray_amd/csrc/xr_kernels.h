@@ -32,6 +32,9 @@ struct MlpState {
     float* grad = nullptr;      // 32 partial gradient copies
     uint16_t* wt_bf = nullptr;  // BF16 W^T of the hidden layers (forward)
     uint16_t* w_bf = nullptr;   // BF16 W of hidden layers 1..4 (backward)
+    uint8_t* wt_f8 = nullptr;   // FP8 E4M3 W^T of the hidden layers (FP8 weight format)
+    float* f8_scale = nullptr;  // per hidden layer power-of-two scale of wt_f8
+    bool fp8 = false;           // weight format: the BF16 copies hold dequantised FP8 values
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};  // position normalisation
     uint32_t steps = 0;         // Adam steps taken
 };

@@ -41,6 +41,47 @@ XR_HD uint16_t bf16_bits(float f) {
 XR_HD float bf16_float(uint16_t h) { return as_float((uint32_t)h << 16); }
 XR_HD float bf16_round(float f) { return bf16_float(bf16_bits(f)); }
 
+// FP32 -> FP8 E4M3 (the OCP "E4M3FN" format: 1 sign, 4 exponent bits with
+// bias 7, 3 mantissa bits, no infinities, largest finite 448, NaN
+// S.1111.111), round to nearest even, saturating at +-448.
+XR_HD uint8_t e4m3_bits(float f) {
+    const uint32_t u = as_uint(f);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const float a = fabsf(f);
+    if (!(a == a)) return (uint8_t)(sign | 0x7Fu);
+    if (a < 0.015625f) {  // below 2^-6: multiples of 2^-9 (8 * 2^-9 is the smallest normal, code 0x08)
+        const uint32_t q = (uint32_t)rintf(a * 512.0f);
+        return (uint8_t)(sign | q);
+    }
+    const uint32_t ua = as_uint(a);
+    int e = (int)((ua >> 23) & 0xFFu) - 127;
+    uint32_t keep = (ua & 0x7FFFFFu) >> 20;
+    const uint32_t rest = ua & 0xFFFFFu;
+    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) ++keep;
+    if (keep == 8u) {
+        keep = 0u;
+        ++e;
+    }
+    if (e > 8 || (e == 8 && keep > 6u)) return (uint8_t)(sign | 0x7Eu);  // 448
+    return (uint8_t)(sign | ((uint32_t)(e + 7) << 3) | keep);
+}
+XR_HD float e4m3_float(uint8_t b) {
+    const uint32_t E = (b >> 3) & 0xFu, m = b & 7u;
+    float v;
+    if (E == 0u) v = (float)m * 0.001953125f;  // m * 2^-9
+    else if (E == 15u && m == 7u) v = as_float(0x7FC00000u);
+    else v = as_float(((E + 120u) << 23) | (m << 20));
+    return (b & 0x80u) ? -v : v;
+}
+// Per-layer power-of-two scale of the FP8 weights: the smallest 2^k with
+// max|W| / 2^k <= 448 (1 for an all-zero layer).
+XR_HD float e4m3_layer_scale(float maxabs) {
+    if (!(maxabs > 0.0f)) return 1.0f;
+    int k;
+    const float m = frexpf(maxabs / 448.0f, &k);  // maxabs / 448 = m * 2^k, m in [0.5, 1)
+    return ldexpf(1.0f, m == 0.5f ? k - 1 : k);
+}
+
 // The 96 network inputs of a vertex at position p (in world units, scaled to
 // [0,1]^3 by the scene box lo / inv_ext), outgoing direction w (unit,
 // pointing away from the vertex), shading normal n, albedo a:

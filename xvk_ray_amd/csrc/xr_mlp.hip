@@ -105,6 +105,8 @@ struct MlpDev {
     float* grad;            // kGradCopies x kMlpParams partial gradients
     float* m;               // Adam first moment
     float* v;               // Adam second moment
+    const uint8_t* wt_f8;   // FP8 E4M3 W^T per hidden layer, or null (BF16 weight format)
+    const float* f8_scale;  // its per-layer scales
     float lo[3], inv_ext[3];
 };
 
@@ -142,14 +144,22 @@ struct InferLds {
 };
 static_assert(offsetof(InferLds, act) % 16 == 0 && sizeof(InferLds) <= 160 * 1024, "InferLds layout");
 
+// FP8 weight format: the E4M3 codes are read (1 byte per weight) and
+// dequantised into the BF16 LDS copy; E4M3 times a power of two is exact in
+// BF16, so the MFMA sees exactly the quantised weights.
+__device__ __forceinline__ uint16_t weight_bf16(const MlpDev& m, int l, int i) {
+    if (m.wt_f8) return bf16_bits(e4m3_float(m.wt_f8[i]) * m.f8_scale[l]);
+    return m.wt_bf[i];
+}
+
 __device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
     for (int i = threadIdx.x; i < 64 * kMlpIn; i += blockDim.x) {
         const int r = i / kMlpIn, c = i % kMlpIn;
-        s.wt[r * kStrideIn + c] = m.wt_bf[i];
+        s.wt[r * kStrideIn + c] = weight_bf16(m, 0, i);
     }
     for (int i = threadIdx.x; i < 4 * 64 * 64; i += blockDim.x) {
         const int l = i / 4096, r = (i % 4096) / 64, c = i % 64;
-        s.wt[64 * kStrideIn + l * 64 * kStrideHid + r * kStrideHid + c] = m.wt_bf[64 * 96 + i];
+        s.wt[64 * kStrideIn + l * 64 * kStrideHid + r * kStrideHid + c] = weight_bf16(m, 1 + l, 64 * 96 + i);
     }
     for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) s.bias[i] = m.w[mlp_boff(i / 64) + i % 64];
     for (int i = threadIdx.x; i < 64 * 3 + 3; i += blockDim.x) s.w5[i] = m.w[mlp_woff(5) + i];
@@ -424,6 +434,37 @@ __global__ __launch_bounds__(256) void k_mlp_adam(MlpDev m, float* __restrict__ 
     }
 }
 
+// FP8 weight format: one workgroup per hidden layer finds max|W|, picks the
+// power-of-two scale, writes the E4M3 codes of W^T and overwrites the BF16
+// forward / backward copies with the dequantised values (straight-through:
+// training keeps the FP32 master weights and backpropagates through the
+// quantised ones).
+__global__ __launch_bounds__(256) void k_mlp_quant(const float* __restrict__ w, uint8_t* __restrict__ wt_f8,
+                                                   float* __restrict__ scales, uint16_t* __restrict__ wt_bf,
+                                                   uint16_t* __restrict__ w_bf) {
+    __shared__ float red[256];
+    const int l = blockIdx.x;
+    const int in = mlp_in(l), cnt = in * 64, wo = mlp_woff(l);
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < cnt; i += 256) mx = fmaxf(mx, fabsf(w[wo + i]));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    const float scale = e4m3_layer_scale(red[0]);
+    if (threadIdx.x == 0) scales[l] = scale;
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        const int k = i / 64, j = i % 64;
+        const uint8_t q = e4m3_bits(w[wo + i] / scale);
+        const uint16_t b = bf16_bits(e4m3_float(q) * scale);
+        wt_f8[wt_off(l) + j * in + k] = q;
+        wt_bf[wt_off(l) + j * in + k] = b;
+        if (l >= 1) w_bf[(l - 1) * 4096 + k * 64 + j] = b;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -432,6 +473,8 @@ static MlpDev mlp_dev(const MlpState& st) {
     m.w = st.w;
     m.wt_bf = st.wt_bf;
     m.w_bf = st.w_bf;
+    m.wt_f8 = st.fp8 ? st.wt_f8 : nullptr;
+    m.f8_scale = st.f8_scale;
     m.grad = st.grad;
     m.m = st.m;
     m.v = st.v;
@@ -447,13 +490,14 @@ static const size_t kParamBytes = align256(sizeof(float) * kMlpParams);
 
 size_t mlp_bytes() {
     return 3 * kParamBytes + align256(sizeof(float) * (size_t)kMlpParams * kGradCopies) +
-           align256(sizeof(uint16_t) * kWtSize) + align256(sizeof(uint16_t) * kWSize);
+           align256(sizeof(uint16_t) * kWtSize) + align256(sizeof(uint16_t) * kWSize) + align256(kWtSize) +
+           align256(sizeof(float) * 5);
 }
 
 size_t mlp_grad_offset_bytes() { return 3 * kParamBytes; }
 
 // Regions (each 256-byte aligned, so every 16-byte BF16 fragment load is
-// aligned): w | m | v | grad[32] | wt_bf | w_bf.
+// aligned): w | m | v | grad[32] | wt_bf | w_bf | wt_f8 | f8_scale.
 hipError_t mlp_bind(MlpState* st, void* base) {
     char* p = (char*)base;
     st->w = (float*)p;
@@ -467,6 +511,10 @@ hipError_t mlp_bind(MlpState* st, void* base) {
     st->wt_bf = (uint16_t*)p;
     p += align256(sizeof(uint16_t) * kWtSize);
     st->w_bf = (uint16_t*)p;
+    p += align256(sizeof(uint16_t) * kWSize);
+    st->wt_f8 = (uint8_t*)p;
+    p += align256(kWtSize);
+    st->f8_scale = (float*)p;
     return hipSuccess;
 }
 
@@ -477,6 +525,7 @@ hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool s
     const float bc2 = 1.0f - powf(b2, (float)st.steps);
     hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, m, st.w, st.wt_bf, st.w_bf, lr,
                        b1, b2, 1e-8f, inv_scale, step ? bc1 : 1.0f, step ? bc2 : 1.0f, step ? 1 : 0);
+    if (st.fp8) hipLaunchKernelGGL(k_mlp_quant, dim3(5), dim3(256), 0, s, st.w, st.wt_f8, st.f8_scale, st.wt_bf, st.w_bf);
     return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ XR_FLAG_RADIANCE = 1
 XR_FLAG_MLP_CACHE = 2
 XR_MLP_INPUTS = 96
 XR_MLP_PARAMS = 23043
+XR_FLAG_MLP_FP8 = 4
+XR_MLP_WEIGHTS_BF16 = 0
+XR_MLP_WEIGHTS_FP8 = 1
+XR_MLP_FP8_BYTES = 22528
 XR_LAYOUT_REFERENCE = 0
 XR_LAYOUT_NATIVE = 1
 
@@ -117,7 +121,8 @@ XR_API_SYMBOLS = [
     "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
     "xr_reset_counters", "xr_get_accel_info",
     "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
-    "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_params",
+    "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
+    "xr_mlp_fp8_weights", "xr_mlp_params",
     "xr_last_error", "xr_global_last_error",
 ]
 XR_HOST_SYMBOLS = [
@@ -178,6 +183,8 @@ def lib() -> C.CDLL:
         L.xr_mlp_gradient.argtypes = [P, P, P, U32, P]
         L.xr_mlp_train_rows.argtypes = [P, P, P, U32, C.c_float]
         L.xr_mlp_params.argtypes = [P, P, I]
+        L.xr_mlp_set_weight_format.argtypes = [P, I]
+        L.xr_mlp_fp8_weights.argtypes = [P, P, P]
         L.xr_last_error.argtypes = [P]
         L.xr_last_error.restype = C.c_char_p
         L.xr_global_last_error.argtypes = []
@@ -481,6 +488,16 @@ class Context:
         y = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
         assert x.shape[0] == y.shape[0]
         self.check(lib().xr_mlp_train_rows(self._h, x.ctypes.data, y.ctypes.data, x.shape[0], lr))
+
+    def mlp_set_weight_format(self, fmt: int) -> None:
+        self.check(lib().xr_mlp_set_weight_format(self._h, fmt))
+
+    def mlp_fp8_weights(self) -> tuple[np.ndarray, np.ndarray]:
+        """(E4M3 codes of the hidden layers' W^T, per-layer scales)."""
+        codes = np.zeros(XR_MLP_FP8_BYTES, np.uint8)
+        scales = np.zeros(5, np.float32)
+        self.check(lib().xr_mlp_fp8_weights(self._h, codes.ctypes.data, scales.ctypes.data))
+        return codes, scales
 
     def mlp_params(self, params: np.ndarray | None = None) -> np.ndarray:
         """FP32 master parameters; writes `params` into the network first when given."""
