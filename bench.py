@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-mlp-study", action="store_true")
     ap.add_argument("--study-frames", type=int, default=256)
     ap.add_argument("--conv-spp", type=int, default=1024)
+    ap.add_argument("--builder", choices=("lbvh", "ploc"), default="lbvh",
+                    help="BVH builder (the build itself is outside the timed region)")
     ap.add_argument("--workload", choices=("c2", "c1", "c5"), default="c2",
                     help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
                          "Cornell box; c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain")
@@ -98,7 +100,7 @@ def mlp_study(scene, camera, args) -> dict:
     from xvk_ray_amd import native as N
 
     view = camera.view(0)
-    ctx = N.Context(scene, W, H)
+    ctx = N.Context(scene, W, H, builder=builder_id(args))
     frames = args.study_frames
     checkpoints = sorted({f for f in (1, 4, 16, 64, 128, 256, frames) if f <= frames})
     ubo0 = N.make_ubo(view, view, FOV, W, H, bounces=BOUNCES, frame=1)
@@ -230,6 +232,11 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def builder_id(args) -> int:
+    from xvk_ray_amd import native as N
+    return {"lbvh": N.XR_BUILDER_LBVH, "ploc": N.XR_BUILDER_PLOC}[args.builder]
+
+
 def base_result(args, metric, value, unit, ms, dtype, config) -> dict:
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
@@ -251,6 +258,7 @@ def run_c1(args) -> dict:
     app = N.App(scene, camera, Wc, Hc, mode=N.XR_MODE_PLAIN, spp=1, bounces=3, max_spp=frames + 1, fov=60.0,
                 accumulate=True, sky=False)
     ctx = app.context()
+    ctx.build_accel(builder_id(args))
     params = N.default_params(plain_max_depth=3)
     for _ in range(args.warmup):
         app.draw_frame(params)
@@ -323,6 +331,7 @@ def run_c5(args) -> dict:
         app = N.App(scene, camera, W5, H5, mode=N.XR_MODE_RESTIR, spp=1, bounces=depth, fov=FOV, sky=False,
                     reuse=True)
         ctx = app.context()
+        ctx.build_accel(builder_id(args))
         info = ctx.accel_info()
         flags = N.XR_FLAG_RADIANCE | ((N.XR_FLAG_MLP_CACHE | N.XR_FLAG_MLP_FP8) if cache else 0)
         p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, **PARAMS)
@@ -352,7 +361,7 @@ def run_c5(args) -> dict:
     res_plain = run(False)
     res_cache = run(True)
     # converged images of the checkpoint cameras
-    ctx = N.Context(scene, W5, H5)
+    ctx = N.Context(scene, W5, H5, builder=builder_id(args))
     rmse = {False: {}, True: {}}
     for f in checkpoints:
         ubo = saved[(False, f)][1]
@@ -403,6 +412,7 @@ def main():
     camera = N.Camera.orbit(**CAMERA)
     app = N.App(scene, camera, W, H, mode=N.XR_MODE_RESTIR, spp=1, bounces=BOUNCES, fov=FOV, sky=False, reuse=True)
     ctx = app.context()
+    ctx.build_accel(builder_id(args))
     params = N.default_params(visibility=N.XR_VIS_TRACE, **PARAMS)
     info = ctx.accel_info()
     setup_s = time.perf_counter() - t_setup
@@ -465,7 +475,7 @@ def main():
             "width": W, "height": H, "triangles": info.triangle_count, "max_depth": BOUNCES,
             "temporal_m_cap": PARAMS["temporal_m_cap"], "spatial_samples": PARAMS["spatial_samples"],
             "spatial_radius_px": PARAMS["spatial_radius"], "visibility": "traced",
-            "bvh": {"builder": "lbvh", "nodes": info.node_count, "depth": info.max_depth,
+            "bvh": {"builder": args.builder, "nodes": info.node_count, "depth": info.max_depth,
                     "build_ms": round(info.build_ms, 2)},
             "parallelism": "single_gpu",
         },

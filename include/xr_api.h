@@ -181,8 +181,11 @@ void xr_default_params(xr_restir_params* p);
 int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ctx** out);
 
 /* Builds the acceleration structure (CreateAccelerationStructures,
- * ApplicationRT.cpp:55-66).  builder: XR_BUILDER_LBVH. */
+ * ApplicationRT.cpp:55-66).  builder: XR_BUILDER_LBVH (Morton-order radix
+ * tree, fastest build) or XR_BUILDER_PLOC (locally-ordered clustering,
+ * higher tree quality, faster traversal). */
 #define XR_BUILDER_LBVH 0
+#define XR_BUILDER_PLOC 1
 int xr_build_accel(xr_ctx* ctx, int builder);
 
 /* Reallocates the frame buffers (swapchain recreation,

@@ -65,12 +65,14 @@ def check_bvh(nodes: np.ndarray, tris: np.ndarray, tri_count: int) -> int:
     return max_depth
 
 
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC])
 @pytest.mark.parametrize("which", ["cornell", "terrain"])
-def test_bvh_structure(which):
+def test_bvh_structure(which, builder):
     scene = cornell_scene() if which == "cornell" else small_terrain()
-    say(f"[bvh] building {which} ({scene.triangle_count} tris)")
-    ctx = N.Context(scene, 8, 8)
+    say(f"[bvh] building {which} ({scene.triangle_count} tris) with builder {builder}")
+    ctx = N.Context(scene, 8, 8, builder=builder)
     info = ctx.accel_info()
+    assert info.builder == builder
     nodes, tris = ctx.read_accel()
     depth = check_bvh(nodes, tris, scene.triangle_count)
     # the builder reports the uncollapsed depth, an upper bound on the linked tree's
@@ -84,6 +86,26 @@ def test_bvh_structure(which):
     prim = tris[order, 11].copy().view(np.uint32)
     vi = off[2 * inst + 1][:, None] + idx[(off[2 * inst] + 3 * prim)[:, None] + np.arange(3)]
     assert np.array_equal(canon, v["pos"][vi])
+
+
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC])
+def test_bvh_two_triangles_and_rebuild(builder):
+    """Smallest clustered tree, and a rebuild of one context with the other builder."""
+    verts = np.zeros(6, N.VERTEX_DTYPE)
+    verts["pos"] = [[0, 0, 1], [1, 0, 1], [0, 1, 1], [0, 0, 2], [1, 0, 2], [0, 1, 2]]
+    mats = np.zeros(1, N.MATERIAL_DTYPE)
+    mats["base_color_texture_index"] = -1
+    scene = N.Scene.from_arrays(verts, np.arange(6, dtype=np.uint32), mats, np.array([0, 0], np.uint32))
+    ctx = N.Context(scene, 4, 4, builder=builder)
+    check_bvh(*ctx.read_accel(), 2)
+    ctx.build_accel(N.XR_BUILDER_PLOC if builder == N.XR_BUILDER_LBVH else N.XR_BUILDER_LBVH)
+    check_bvh(*ctx.read_accel(), 2)
+    r = np.zeros(1, N.RAY_DTYPE)
+    r["origin"] = [[0.2, 0.2, 0.0]]
+    r["direction"] = [0, 0, 1]
+    r["tmin"], r["tmax"] = 1e-4, 1e4
+    h = ctx.trace(r)
+    assert h["t"][0] == np.float32(1.0) and h["triangle"][0] == 0
 
 
 def test_bvh_single_triangle():

@@ -28,11 +28,15 @@ def _max_lsb_diff(a: np.ndarray, b: np.ndarray) -> int:
 # ---------------------------------------------------------------------------
 # TraceRay
 # ---------------------------------------------------------------------------
+BUILDERS = [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC]
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("which", ["cornell", "terrain"])
-def test_trace_closest_bit_exact(which):
+def test_trace_closest_bit_exact(which, builder):
     scene = cornell_scene() if which == "cornell" else small_terrain()
-    say(f"[trace] {which}: {scene.triangle_count} tris")
-    ctx = N.Context(scene, 8, 8)
+    say(f"[trace] {which}: {scene.triangle_count} tris, builder {builder}")
+    ctx = N.Context(scene, 8, 8, builder=builder)
     orc = oracle_scene(scene)
     lo, hi = ((0.02, 0.98) if which == "cornell" else (0.0, 0.5))
     rays = random_rays(20000, lo, hi, seed=7)
@@ -43,11 +47,12 @@ def test_trace_closest_bit_exact(which):
     assert (gpu["t"] > 0).mean() > 0.3  # the batch actually hits geometry
 
 
-def test_trace_matches_fp64_judge():
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_trace_matches_fp64_judge(builder):
     """North-star criterion: >= 99.99% identical closest-hit ids vs an FP64
     search; every mismatch is a distance tie within 1e-6 * t."""
     scene = small_terrain(grid=256, spheres=64, lights=16)
-    ctx = N.Context(scene, 8, 8)
+    ctx = N.Context(scene, 8, 8, builder=builder)
     orc = oracle_scene(scene)
     rng = np.random.default_rng(3)
     n = 50000
@@ -70,9 +75,10 @@ def test_trace_matches_fp64_judge():
             list(zip(t_gpu[:5], t_ref[:5]))
 
 
-def test_trace_any_hit_matches():
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_trace_any_hit_matches(builder):
     scene = small_terrain()
-    ctx = N.Context(scene, 8, 8)
+    ctx = N.Context(scene, 8, 8, builder=builder)
     orc = oracle_scene(scene)
     rays = random_rays(20000, 0.0, 0.6, seed=11, tmin=0.0, tmax=0.2)
     gpu = ctx.trace(rays, any_hit=True)

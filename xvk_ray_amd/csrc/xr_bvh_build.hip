@@ -1,5 +1,7 @@
-// GPU LBVH builder (Karras, "Maximizing Parallelism in the Construction of
-// BVHs, Octrees, and k-d Trees", HPG 2012) for gfx950.
+// GPU BVH builders for gfx950: LBVH (Karras, "Maximizing Parallelism in the
+// Construction of BVHs, Octrees, and k-d Trees", HPG 2012) and PLOC (Meister
+// & Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume
+// Hierarchy Construction", IEEE TVCG 2018).
 //
 // Replaces vkCmdBuildAccelerationStructuresKHR of the reference
 // (XVKBLAccelerationStructure.cpp:44-58, XVKTLAccelerationStructure.cpp:49-66),
@@ -14,6 +16,13 @@
 //   4. karras   internal-node topology from the sorted codes
 //   5. refit    bottom-up boxes (padded, see box_pad), one atomic counter per internal node
 //   6. emit     64-byte two-box nodes and the BVH-ordered triangle array
+// PLOC shares 1-3 and replaces 4-5 by bottom-up clustering of the sorted
+// triangles (each cluster merges with its nearest neighbour by merged-box
+// area within 16 positions when the choice is mutual), computing boxes, SAH
+// costs and leaf collapses as nodes are created; a depth-first numbering of
+// the leaves then makes every collapsed subtree a contiguous triangle range,
+// and the linked nodes are stored in depth-first preorder (a node's left
+// child follows it), with no storage for nodes inside collapsed subtrees.
 #include <hipcub/hipcub.hpp>
 #include <stdio.h>
 #include "xr_kernels.h"
@@ -292,6 +301,202 @@ __global__ void k_emit(int n, const int2* __restrict__ child, const Box* __restr
     nodes[i] = nd;
 }
 
+// ---------------------------------------------------------------------------
+// PLOC
+// ---------------------------------------------------------------------------
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+    Box m;
+    for (int k = 0; k < 3; ++k) {
+        m.lo[k] = fminf(a.lo[k], b.lo[k]);
+        m.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    return m;
+}
+
+// Node ids: internal 0..n-2 in creation order (the root is n-2), leaf of
+// sorted position j -> (n-1) + j.
+__global__ void k_ploc_leaves(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
+                              const int* __restrict__ bounds, Box* __restrict__ boxes, float* __restrict__ cost,
+                              int* __restrict__ count, int* __restrict__ linked, int* __restrict__ clusters) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const BvhTri t = ctris[sorted_ids[j]];
+    const float pad = box_pad(bounds);
+    Box b;
+    b.lo[0] = fminf(fminf(t.v0.x, t.v1.x), t.v2.x) - pad;
+    b.lo[1] = fminf(fminf(t.v0.y, t.v1.y), t.v2.y) - pad;
+    b.lo[2] = fminf(fminf(t.v0.z, t.v1.z), t.v2.z) - pad;
+    b.hi[0] = fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x) + pad;
+    b.hi[1] = fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y) + pad;
+    b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z) + pad;
+    const int id = (n - 1) + j;
+    boxes[id] = b;
+    cost[id] = kSahTri * box_area(b);
+    count[id] = 1;
+    linked[id] = 0;
+    clusters[j] = id;
+}
+
+// Nearest neighbour of each cluster among the 2 * kPlocRadius around it, by
+// the area of the merged box; ties go to the lower position, which orders
+// candidates exactly as the pairs (area, lower, higher) are ordered, so the
+// globally best pair is always mutual and every iteration merges.
+__global__ void k_ploc_nn(const int* __restrict__ clusters, int C, const Box* __restrict__ boxes,
+                          int* __restrict__ nn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const Box bi = boxes[clusters[i]];
+    float best_d = __builtin_huge_valf();
+    int best = -1;
+    const int lo = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+    const int hi = i + kPlocRadius > C - 1 ? C - 1 : i + kPlocRadius;
+    for (int j = lo; j <= hi; ++j) {
+        if (j == i) continue;
+        const float d = box_area(box_union(bi, boxes[clusters[j]]));
+        if (d < best_d) {
+            best_d = d;
+            best = j;
+        }
+    }
+    nn[i] = best;
+}
+
+// flags[i]: high word 1 when cluster i creates a node (mutual pair, i the
+// lower), low word 1 when position i survives (it is not the higher of a
+// mutual pair).
+__global__ void k_ploc_flags(const int* __restrict__ nn, int C, unsigned long long* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    const unsigned long long make = (mutual && i < j) ? 1ull : 0ull;
+    const unsigned long long keep = (mutual && i > j) ? 0ull : 1ull;
+    flags[i] = (make << 32) | keep;
+}
+
+__global__ void k_ploc_merge(int n, const int* __restrict__ clusters, const int* __restrict__ nn, int C,
+                             const unsigned long long* __restrict__ flags, const unsigned long long* __restrict__ offs,
+                             int node_base, Box* __restrict__ boxes, float* __restrict__ cost,
+                             int* __restrict__ count, int* __restrict__ linked, int2* __restrict__ child,
+                             int* __restrict__ parent, int* __restrict__ collapse, int* __restrict__ next) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const unsigned long long f = flags[i];
+    if (!(f & 1ull)) return;
+    const unsigned long long o = offs[i];
+    const int pos = (int)(o & 0xFFFFFFFFull);
+    if (!(f >> 32)) {
+        next[pos] = clusters[i];
+        return;
+    }
+    const int a = clusters[i], b = clusters[nn[i]];
+    const int id = node_base + (int)(o >> 32);
+    const Box m = box_union(boxes[a], boxes[b]);
+    boxes[id] = m;
+    const float area = box_area(m);
+    const int cnt = count[a] + count[b];
+    const float split = kSahNode * area + cost[a] + cost[b];
+    const float as_leaf = kSahTri * (float)cnt * area;
+    const bool leaf = cnt <= kMaxLeafTris && as_leaf <= split;
+    collapse[id] = leaf ? 1 : 0;
+    cost[id] = leaf ? as_leaf : split;
+    count[id] = cnt;
+    // linked[x]: nodes stored for the subtree of x when x itself is stored;
+    // leaves and collapsed nodes are encoded in their parent and store none
+    auto stored = [&](int x) { return (x >= n - 1 || collapse[x]) ? 0 : linked[x]; };
+    linked[id] = 1 + stored(a) + stored(b);
+    child[id] = make_int2(a, b);
+    parent[a] = id;
+    parent[b] = id;
+    next[pos] = id;
+}
+
+// Depth-first position of each leaf (left subtree first) and its depth.
+__global__ void k_ploc_leaf_order(int n, const int* __restrict__ parent, const int2* __restrict__ child,
+                                  const int* __restrict__ count, int* __restrict__ leaf_off,
+                                  uint32_t* __restrict__ max_depth) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int root = n - 2;
+    int node = (n - 1) + j, off = 0;
+    uint32_t depth = 0;
+    while (node != root && depth < (uint32_t)n) {
+        const int p = parent[node];
+        const int2 c = child[p];
+        if (c.y == node) off += count[c.x];
+        node = p;
+        ++depth;
+    }
+    leaf_off[j] = off;
+    atomicMax(max_depth, depth);
+}
+
+// First depth-first triangle position below each internal node (its leftmost leaf).
+__global__ void k_ploc_range(int n, const int2* __restrict__ child, const int* __restrict__ leaf_off,
+                             int* __restrict__ first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int node = i;
+    for (int steps = 0; node < n - 1 && steps < n; ++steps) node = child[node].x;
+    first[i] = leaf_off[node - (n - 1)];
+}
+
+// Depth-first preorder position of each stored internal node (-1 for nodes
+// inside a collapsed subtree): the left child of a stored node follows it,
+// the right child follows the left child's stored subtree.  The root is 0.
+__global__ void k_ploc_preorder(int n, const int* __restrict__ parent, const int2* __restrict__ child,
+                                const int* __restrict__ collapse, const int* __restrict__ linked,
+                                int* __restrict__ pre) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int root = n - 2;
+    if (i != root && collapse[i]) {
+        pre[i] = -1;
+        return;
+    }
+    int node = i, pos = 0;
+    for (int steps = 0; node != root && steps < n; ++steps) {
+        const int p = parent[node];
+        if (p != root && collapse[p]) {
+            pre[i] = -1;
+            return;
+        }
+        const int2 c = child[p];
+        if (c.x == node) pos += 1;
+        else pos += 1 + ((c.x >= n - 1 || collapse[c.x]) ? 0 : linked[c.x]);
+        node = p;
+    }
+    pre[i] = pos;
+}
+
+// Stored nodes at their preorder positions; triangles at their depth-first
+// positions.
+__global__ void k_ploc_emit(int n, const int2* __restrict__ child, const Box* __restrict__ boxes,
+                            const int* __restrict__ count, const int* __restrict__ collapse,
+                            const int* __restrict__ leaf_off, const int* __restrict__ first,
+                            const int* __restrict__ pre, const BvhTri* __restrict__ ctris,
+                            const uint32_t* __restrict__ sorted_ids, BvhNode* __restrict__ nodes,
+                            BvhTri* __restrict__ tris) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tris[leaf_off[i]] = ctris[sorted_ids[i]];
+    if (i >= n - 1 || pre[i] < 0) return;
+    const int2 c = child[i];
+    const Box b0 = boxes[c.x], b1 = boxes[c.y];
+    auto enc = [&](int cid) -> int {
+        if (cid >= n - 1) return ~(leaf_off[cid - (n - 1)] * 16);
+        if (collapse[cid]) return ~(first[cid] * 16 + (count[cid] - 1));
+        return pre[cid];
+    };
+    BvhNode nd;
+    nd.a = make_float4(b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]);
+    nd.b = make_float4(b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]);
+    nd.c = make_float4(b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]);
+    nd.d = make_int4(enc(c.x), enc(c.y), 0, 0);
+    nodes[pre[i]] = nd;
+}
+
 }  // namespace
 
 #define XR_TRY(expr)                                                                         \
@@ -413,6 +618,195 @@ done:
     hipFree(range);
     hipFree(cost);
     hipFree(collapse);
+    hipFree(dmax);
+    hipFree(temp);
+    hipFree(nodes);
+    hipFree(tris);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return rc;
+}
+
+hipError_t build_ploc(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len) {
+    if (tri_count < 2)  // one triangle: the single-node tree of the LBVH path
+        return build_lbvh(vertices, indices, offsets, inst_first_tri, instance_count, tri_count, st, out, err,
+                          err_len);
+    hipError_t rc = hipSuccess;
+    const int n = (int)tri_count;
+    const size_t nall = 2 * (size_t)n - 1;
+    BvhTri* ctris = nullptr;
+    float4* cent = nullptr;
+    int* bounds = nullptr;
+    unsigned long long *keys = nullptr, *keys2 = nullptr;
+    uint32_t *vals = nullptr, *vals2 = nullptr;
+    Box* boxes = nullptr;
+    float* cost = nullptr;
+    int* count = nullptr;
+    int* linked = nullptr;
+    int* pre = nullptr;
+    int2* child = nullptr;
+    int* parent = nullptr;
+    int* collapse = nullptr;
+    int *clusters = nullptr, *next = nullptr, *nn = nullptr;
+    int stored = 0;
+    unsigned long long *flags = nullptr, *offs = nullptr;
+    int *leaf_off = nullptr, *first = nullptr;
+    uint32_t* dmax = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0, scan_bytes = 0;
+    BvhNode* nodes = nullptr;
+    BvhTri* tris = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int init_bounds[12] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
+                                 (int)0x80000000, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000,
+                                 (int)0x80000000, (int)0x80000000};
+    const uint32_t blocks = (tri_count + 255) / 256;
+    uint32_t host_depth = 0;
+    float ms = 0.f;
+    int C = n, node_base = 0, iterations = 0;
+    XR_TRY(hipEventCreate(&e0));
+    XR_TRY(hipEventCreate(&e1));
+    XR_TRY(hipEventRecord(e0, st));
+    XR_TRY(hipMalloc(&ctris, sizeof(BvhTri) * tri_count));
+    XR_TRY(hipMalloc(&cent, sizeof(float4) * tri_count));
+    XR_TRY(hipMalloc(&bounds, sizeof(int) * 12));
+    XR_TRY(hipMalloc(&keys, sizeof(unsigned long long) * tri_count));
+    XR_TRY(hipMalloc(&keys2, sizeof(unsigned long long) * tri_count));
+    XR_TRY(hipMalloc(&vals, sizeof(uint32_t) * tri_count));
+    XR_TRY(hipMalloc(&vals2, sizeof(uint32_t) * tri_count));
+    XR_TRY(hipMalloc(&boxes, sizeof(Box) * nall));
+    XR_TRY(hipMalloc(&cost, sizeof(float) * nall));
+    XR_TRY(hipMalloc(&count, sizeof(int) * nall));
+    XR_TRY(hipMalloc(&linked, sizeof(int) * nall));
+    XR_TRY(hipMalloc(&pre, sizeof(int) * (n - 1)));
+    XR_TRY(hipMalloc(&child, sizeof(int2) * (n - 1)));
+    XR_TRY(hipMalloc(&parent, sizeof(int) * nall));
+    XR_TRY(hipMalloc(&collapse, sizeof(int) * (n - 1)));
+    XR_TRY(hipMalloc(&clusters, sizeof(int) * n));
+    XR_TRY(hipMalloc(&next, sizeof(int) * n));
+    XR_TRY(hipMalloc(&nn, sizeof(int) * n));
+    XR_TRY(hipMalloc(&flags, sizeof(unsigned long long) * n));
+    XR_TRY(hipMalloc(&offs, sizeof(unsigned long long) * n));
+    XR_TRY(hipMalloc(&leaf_off, sizeof(int) * n));
+    XR_TRY(hipMalloc(&first, sizeof(int) * (n - 1)));
+    XR_TRY(hipMalloc(&dmax, sizeof(uint32_t)));
+    XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * (n - 1)));
+    XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * tri_count));
+    XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
+    XR_TRY(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
+    XR_TRY(hipMemsetAsync(parent, 0, sizeof(int) * nall, st));
+
+    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
+                       instance_count, tri_count, ctris, cent, bounds);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, st, cent, tri_count, bounds, keys, vals);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
+    XR_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags, offs, n, st));
+    if (scan_bytes > temp_bytes) temp_bytes = scan_bytes;
+    XR_TRY(hipMalloc(&temp, temp_bytes));
+    {
+        size_t tb = temp_bytes;
+        XR_TRY(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys2, vals, vals2, n, 0, 63, st));
+    }
+    hipLaunchKernelGGL(k_ploc_leaves, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, bounds, boxes, cost, count,
+                       linked, clusters);
+    XR_TRY(hipGetLastError());
+    // Clustering: every iteration merges at least the globally closest pair,
+    // so at most n - 1 iterations run (in practice a few dozen).
+    while (C > 1) {
+        if (++iterations > n) {
+            snprintf(err, err_len, "PLOC made no progress at %d clusters", C);
+            goto fail;
+        }
+        const uint32_t cb = (uint32_t)(C + 255) / 256;
+        hipLaunchKernelGGL(k_ploc_nn, dim3(cb), dim3(256), 0, st, clusters, C, boxes, nn);
+        XR_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_ploc_flags, dim3(cb), dim3(256), 0, st, nn, C, flags);
+        XR_TRY(hipGetLastError());
+        {
+            size_t tb = temp_bytes;
+            XR_TRY(hipcub::DeviceScan::ExclusiveSum(temp, tb, flags, offs, C, st));
+        }
+        unsigned long long last[2];
+        XR_TRY(hipMemcpyAsync(&last[0], flags + (C - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        XR_TRY(hipMemcpyAsync(&last[1], offs + (C - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_ploc_merge, dim3(cb), dim3(256), 0, st, n, clusters, nn, C, flags, offs, node_base,
+                           boxes, cost, count, linked, child, parent, collapse, next);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipStreamSynchronize(st));
+        const unsigned long long tot = last[0] + last[1];
+        const int made = (int)(tot >> 32), kept = (int)(tot & 0xFFFFFFFFull);
+        if (made <= 0 || node_base + made > n - 1) {
+            snprintf(err, err_len, "PLOC merge count %d invalid at %d clusters", made, C);
+            goto fail;
+        }
+        node_base += made;
+        C = kept;
+        int* t = clusters;
+        clusters = next;
+        next = t;
+    }
+    if (node_base != n - 1) {
+        snprintf(err, err_len, "PLOC built %d internal nodes for %d triangles", node_base, n);
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_ploc_leaf_order, dim3(blocks), dim3(256), 0, st, n, parent, child, count, leaf_off, dmax);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_ploc_range, dim3(blocks), dim3(256), 0, st, n, child, leaf_off, first);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_ploc_preorder, dim3(blocks), dim3(256), 0, st, n, parent, child, collapse, linked, pre);
+    XR_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_ploc_emit, dim3(blocks), dim3(256), 0, st, n, child, boxes, count, collapse, leaf_off, first,
+                       pre, ctris, vals2, nodes, tris);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipEventRecord(e1, st));
+    XR_TRY(hipMemcpyAsync(&host_depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipMemcpyAsync(&stored, linked + (n - 2), sizeof(int), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    XR_TRY(hipEventElapsedTime(&ms, e0, e1));
+
+    out->nodes = nodes;
+    out->tris = tris;
+    if (stored < 1 || stored > n - 1) {
+        snprintf(err, err_len, "PLOC stored-node count %d out of range", stored);
+        goto fail;
+    }
+    out->node_count = (uint32_t)stored;
+    out->tri_count = tri_count;
+    out->max_depth = host_depth;
+    out->build_ms = ms;
+    nodes = nullptr;
+    tris = nullptr;
+    rc = hipSuccess;
+    goto done;
+fail:
+    rc = hipErrorUnknown;
+done:
+    hipFree(ctris);
+    hipFree(cent);
+    hipFree(bounds);
+    hipFree(keys);
+    hipFree(keys2);
+    hipFree(vals);
+    hipFree(vals2);
+    hipFree(boxes);
+    hipFree(cost);
+    hipFree(count);
+    hipFree(linked);
+    hipFree(pre);
+    hipFree(child);
+    hipFree(parent);
+    hipFree(collapse);
+    hipFree(clusters);
+    hipFree(next);
+    hipFree(nn);
+    hipFree(flags);
+    hipFree(offs);
+    hipFree(leaf_off);
+    hipFree(first);
     hipFree(dmax);
     hipFree(temp);
     hipFree(nodes);

@@ -76,5 +76,8 @@ struct BvhBuildResult {
 hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint2* offsets,
                       const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
                       hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
+hipError_t build_ploc(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
 
 }  // namespace xr

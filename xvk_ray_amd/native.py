@@ -28,6 +28,7 @@ XR_VIS_REFERENCE = 0
 XR_VIS_TRACE = 1
 XR_PASS_INITIAL, XR_PASS_TEMPORAL, XR_PASS_SPATIAL = 1, 2, 4
 XR_BUILDER_LBVH = 0
+XR_BUILDER_PLOC = 1
 
 XR_BUF_ACCUMULATION = 1
 XR_BUF_OUTPUT = 2
@@ -379,7 +380,7 @@ def make_ubo(view, view_prev, fov, width, height, spp=1, total=1, bounces=4, sky
 class Context:
     """One xr_ctx: scene on the device, frame buffers, acceleration structure."""
 
-    def __init__(self, scene: Scene, width: int, height: int, build: bool = True):
+    def __init__(self, scene: Scene, width: int, height: int, build: bool = True, builder: int = XR_BUILDER_LBVH):
         self.scene = scene
         d = scene.desc()
         h = C.c_void_p()
@@ -389,7 +390,10 @@ class Context:
         self._h = h
         self.width, self.height = width, height
         if build:
-            self.check(lib().xr_build_accel(self._h, XR_BUILDER_LBVH))
+            self.build_accel(builder)
+
+    def build_accel(self, builder: int = XR_BUILDER_LBVH) -> None:
+        self.check(lib().xr_build_accel(self._h, builder))
 
     def check(self, rc: int) -> None:
         if rc:
