@@ -203,7 +203,7 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
     return rays;
 }
 
-__global__ __launch_bounds__(kBlock) void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
                                                            CacheCfg cache) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
@@ -277,7 +277,7 @@ hipError_t launch_clear_u32(uint32_t* p, uint32_t n, hipStream_t s) {
 // valid primary hit and 0 elsewhere (the black / white early-out pixels of
 // the spatial pass).  Sample s of pixel p uses frame number frame_base + s.
 // Each pixel adds its spp samples into three FP64 sums.
-__global__ __launch_bounds__(kBlock) void k_converge(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_converge(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
                                                      uint32_t frame_base, uint32_t spp, double* __restrict__ acc) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
@@ -487,7 +487,7 @@ __device__ __forceinline__ bool similar(float4 cx, float4 cn, float4 hx, float4 
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame, uint32_t reuse,
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame, uint32_t reuse,
                                                     PassParams pp) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev sc, FrameDev fr, ui
 // The plain closest-hit program (rhit.hlsl:29-59) returns the same position,
 // normal, UV and material as the ReSTIR one; it does not compute the
 // geometric normal, which the plain loop does not use.
-__global__ __launch_bounds__(kBlock) void k_plain(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
                                                   uint32_t max_depth) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
