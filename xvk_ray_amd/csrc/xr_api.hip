@@ -77,6 +77,8 @@ struct xr_ctx {
     uint32_t* d_output = nullptr;
     float4* d_radiance = nullptr;
     uint32_t* d_cache_list = nullptr;  // radiance-cache work lists (2 + 2n words)
+    float4* d_tail = nullptr;          // initial-pass paths past the head bounces (4 float4 each)
+    uint32_t* d_tail_count = nullptr;
     unsigned long long* d_counters = nullptr;
 
     // converged image: FP64 RGB sums per pixel, samples accumulated, error scratch
@@ -171,6 +173,8 @@ void free_frame(xr_ctx* c) {
     dev_free(&c->d_output);
     dev_free(&c->d_radiance);
     dev_free(&c->d_cache_list);
+    dev_free(&c->d_tail);
+    dev_free(&c->d_tail_count);
     dev_free(&c->d_conv);
     c->conv_spp = 0;
     c->width = c->height = 0;
@@ -191,6 +195,8 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_output, n * 4));
     XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_tail, 4 * n * f4));
+    XR_HIP(c, dev_alloc(&c->d_tail_count, sizeof(uint32_t)));
     XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));
@@ -237,6 +243,8 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.radiance = nullptr;
     f.counters = c->d_counters;
     f.cache_list = c->d_cache_list;
+    f.tail = c->d_tail;
+    f.tail_count = c->d_tail_count;
     return f;
 }
 

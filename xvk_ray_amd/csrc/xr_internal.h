@@ -37,6 +37,10 @@ struct FrameDev {
     // Radiance-cache work lists of the frame: [0] query count, [1] training
     // count, [2, 2+n) query pixels, [2+n, 2+2n) training pixels.
     uint32_t* cache_list;
+    // Paths alive after the head bounces of the initial-sample pass (4 float4
+    // each, n capacity) and their count.
+    float4* tail;
+    uint32_t* tail_count;
 };
 
 // Parameters of one ReSTIR GI / plain frame as the kernels see them.

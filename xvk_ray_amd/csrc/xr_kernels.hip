@@ -103,34 +103,47 @@ struct PathOut {
     bool valid;       // primitiveHitValid
 };
 
-// Camera ray + bounce loop of the initial sample pass (InitialSample.rgen:41-181)
-// for pixel (x, y) of frame `frame`; returns the number of rays traced.
+// Camera ray + bounce loop of the initial sample pass (InitialSample.rgen:41-181),
+// split so that a kernel can run a range of bounces and hand the path on.
 // With the radiance cache enabled (no reference counterpart) the path stops
 // at its first diffuse vertex x_s when that vertex's spread
 // t1^2 / (p_q |cos theta_s|) exceeds spread_c times the primary footprint
 // t0^2 / (4 pi |cos theta_v|) (Mueller et al. 2021, "Real-time Neural
 // Radiance Caching"), leaving L_out to the cache; every stride-th pixel keeps
 // tracing to full depth and becomes a training sample.
-__device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const UniformBufferObject& ubo, uint32_t x,
-                                               uint32_t y, uint32_t w, uint32_t h, uint32_t frame,
-                                               uint32_t bounces, int* col, PathOut* o, const CacheCfg& cache) {
-    uint32_t rays = 0;
-    uint32_t rng = init_seed(x, y, w, frame);
-    v3 origin, dir;
-    camera_ray(ubo, x, y, w, h, &rng, &origin, &dir);
-    v3 thr = mk3(1.0f, 1.0f, 1.0f);
+struct PathState {
+    v3 origin, dir, thr;
+    uint32_t rng;
+    float a0;  // primary footprint (cache mode)
+};
+
+// Seed, camera ray and the empty PathOut of pixel (x, y) in frame `frame`.
+__device__ __forceinline__ void path_begin(const UniformBufferObject& ubo, uint32_t x, uint32_t y, uint32_t w,
+                                           uint32_t h, uint32_t frame, PathState* ps, PathOut* o) {
+    ps->rng = init_seed(x, y, w, frame);
+    camera_ray(ubo, x, y, w, h, &ps->rng, &ps->origin, &ps->dir);
+    ps->thr = mk3(1.0f, 1.0f, 1.0f);
+    ps->a0 = 0.0f;
     const v3 zero = mk3(0.f, 0.f, 0.f);
     o->x_view = zero; o->n_view = zero; o->x_s = zero; o->n_s = zero; o->L = zero; o->f0 = zero;
     o->p_q = 0.0f; o->uv_u = 0.0f; o->uv_v = 0.0f;
     o->mat_idx = 0xFFFFFFFFu; o->rand_seed = 0;
     o->cache = 0u; o->xs_mat = 0u;
     o->valid = false;
-    float a0 = 0.0f;  // primary footprint (cache mode)
-    for (uint32_t bounce = 0;; ++bounce) {
-        if (bounce == bounces) break;
-        const RayPre rp = make_ray_pre(origin, dir);
+}
+
+// Bounces b0 .. b1-1 of the loop.  Returns true when the path is still alive
+// at bounce b1 (< bounces), false when it ended (miss, emitter, cache query,
+// Russian roulette or the bounce limit).  *rays counts the rays traced.
+__device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uint32_t y, uint32_t w,
+                                             uint32_t bounces, uint32_t b0, uint32_t b1, bool sky, PathState* ps,
+                                             int* col, PathOut* o, const CacheCfg& cache, uint32_t* rays) {
+    for (uint32_t bounce = b0;; ++bounce) {
+        if (bounce == bounces) return false;
+        if (bounce == b1) return true;
+        const RayPre rp = make_ray_pre(ps->origin, ps->dir);
         Hit hit;
-        ++rays;
+        ++*rays;
         // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
         Payload pl;
         bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
@@ -139,42 +152,42 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
             found = pl.material != -1;
         }
         if (!found) {
-            if (ubo.hasSkyBox) o->L = add(o->L, thr);
+            if (sky) o->L = add(o->L, ps->thr);
             if (bounce == 1) {
-                o->x_s = add(origin, muls(dir, 1000.0f));
-                o->n_s = neg(dir);
+                o->x_s = add(ps->origin, muls(ps->dir, 1000.0f));
+                o->n_s = neg(ps->dir);
             }
-            break;
+            return false;
         }
         const float4 m0 = sc.materials[2 * pl.material];
         const float4 m1 = sc.materials[2 * pl.material + 1];
         if (__float_as_uint(m1.w) == kDiffuseLight) {
-            o->L = add(o->L, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+            o->L = add(o->L, muls(mul(ps->thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
             if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
             if (bounce == 0) { o->x_view = pl.position; o->n_view = pl.normal; }
-            break;
+            return false;
         }
         if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
         if (cache.enabled) {
             if (bounce == 0) {
-                a0 = pl.t * pl.t / (4.0f * XR_PI * fabsf(dot(pl.normal, dir)));
+                ps->a0 = pl.t * pl.t / (4.0f * XR_PI * fabsf(dot(pl.normal, ps->dir)));
             } else if (bounce == 1) {
                 o->xs_mat = (uint32_t)pl.material;
                 if ((y * w + x) % cache.stride == cache.offset) {
                     o->cache = 2u;
                 } else {
-                    const float a1 = pl.t * pl.t / (o->p_q * fabsf(dot(pl.normal, dir)));
-                    if (a1 > cache.spread_c * a0) {
+                    const float a1 = pl.t * pl.t / (o->p_q * fabsf(dot(pl.normal, ps->dir)));
+                    if (a1 > cache.spread_c * ps->a0) {
                         o->cache = 1u;
-                        break;
+                        return false;
                     }
                 }
             }
         }
-        const uint32_t seed_here = rng;
+        const uint32_t seed_here = ps->rng;
         v3 wi;
         float pdf, cos_t;
-        const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
+        const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &ps->rng);
         if (bounce == 0) {
             o->valid = true;
             o->x_view = pl.position;
@@ -186,35 +199,79 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
             o->p_q = pdf;
             o->f0 = f;
         } else {
-            thr = mul(thr, divs(muls(f, cos_t), pdf));
+            ps->thr = mul(ps->thr, divs(muls(f, cos_t), pdf));
         }
         if (bounce > 3) {
-            const float p = maxf(thr.x, maxf(thr.y, thr.z));
-            if (rand01(&rng) > p) break;
-            thr = muls(thr, 1.0f / p);
+            const float p = maxf(ps->thr.x, maxf(ps->thr.y, ps->thr.z));
+            if (rand01(&ps->rng) > p) return false;
+            ps->thr = muls(ps->thr, 1.0f / p);
         }
-        const v3 hp = add(origin, muls(dir, pl.t));
+        const v3 hp = add(ps->origin, muls(ps->dir, pl.t));
         const v3 g = pl.geo_normal;
         const v3 od = dot(wi, g) > 0.0f ? g : neg(g);
-        origin = add(hp, muls(od, 0.01f));
-        dir = wi;
+        ps->origin = add(hp, muls(od, 0.01f));
+        ps->dir = wi;
     }
-    if (!o->valid) { o->L = zero; o->n_s = zero; }
+}
+
+// What the loop leaves for a path that never found a diffuse primary hit.
+__device__ __forceinline__ void path_end(PathOut* o) {
+    if (!o->valid) {
+        const v3 zero = mk3(0.f, 0.f, 0.f);
+        o->L = zero;
+        o->n_s = zero;
+    }
+}
+
+// The whole loop for one pixel; returns the number of rays traced.
+__device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const UniformBufferObject& ubo, uint32_t x,
+                                               uint32_t y, uint32_t w, uint32_t h, uint32_t frame,
+                                               uint32_t bounces, int* col, PathOut* o, const CacheCfg& cache) {
+    PathState ps;
+    path_begin(ubo, x, y, w, h, frame, &ps, o);
+    uint32_t rays = 0;
+    path_bounces(sc, x, y, w, bounces, 0u, bounces, ubo.hasSkyBox != 0u, &ps, col, o, cache, &rays);
+    path_end(o);
     return rays;
 }
 
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
-                                                           CacheCfg cache) {
+// Initial-sample pass in two kernels.  k_initial_sample runs bounces 0 and 1
+// of every pixel (all of them have a camera ray, most a first bounce) and
+// writes everything the pass produces; a path still alive after bounce 1 is
+// handed on as a PathRecord (compacted: one global atomic per workgroup) and
+// k_initial_tail runs its remaining bounces and writes its L_out.  Beyond
+// bounce 1 few paths survive (most bounce rays leave an outdoor scene), so
+// compacting them keeps whole waves from tracing for one or two live lanes.
+constexpr uint32_t kHeadBounces = 2;
+
+// Path handed from the head to the tail kernel: 4 float4.
+//   [0] origin, pixel   [1] direction, rng   [2] throughput, cache kind   [3] L_out so far
+__device__ __forceinline__ void store_path(float4* rec, uint32_t pix, const PathState& ps, const PathOut& o) {
+    rec[0] = f4(ps.origin, __uint_as_float(pix));
+    rec[1] = f4(ps.dir, __uint_as_float(ps.rng));
+    rec[2] = f4(ps.thr, __uint_as_float(o.cache));
+    rec[3] = f4(o.L, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr,
+                                                                                   UniformBufferObject ubo,
+                                                                                   CacheCfg cache) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
+    __shared__ uint32_t lds_base;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
     uint32_t rays = 0, valid = 0, kind = 0;
+    bool alive = false;
+    PathState ps;
+    PathOut o;
     if (tc.in_image) {
         const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * fr.width + x;
-        PathOut o;
-        rays = trace_path(sc, ubo, x, y, fr.width, fr.height, ubo.currentFrame, ubo.numberOfBounces,
-                          stack + threadIdx.x, &o, cache);
+        path_begin(ubo, x, y, fr.width, fr.height, ubo.currentFrame, &ps, &o);
+        const uint32_t nb = ubo.numberOfBounces;
+        alive = path_bounces(sc, x, y, fr.width, nb, 0u, nb < kHeadBounces ? nb : kHeadBounces,
+                             ubo.hasSkyBox != 0u, &ps, stack + threadIdx.x, &o, cache, &rays);
+        if (!alive) path_end(&o);
         kind = o.cache;
         const uint32_t n = fr.n;
         float4* S = fr.samples;
@@ -224,7 +281,7 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
         S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
         S[2 * n + pix] = f4(o.x_s, o.p_q);
         S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
-        S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
+        if (!alive) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
         S[5 * n + pix] = f4(o.f0, 0.0f);
         if (o.valid) {
             fr.gpos[pix] = f4(o.x_view, 1.0f);
@@ -247,6 +304,14 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
         }
         valid = o.valid ? 1u : 0u;
     }
+    // hand the live paths on: one global atomic per workgroup
+    if (threadIdx.x == 0) lds_count = 0;
+    __syncthreads();
+    const uint32_t local = alive ? atomicAdd(&lds_count, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) lds_base = lds_count ? atomicAdd(fr.tail_count, lds_count) : 0u;
+    __syncthreads();
+    if (alive) store_path(fr.tail + 4 * (size_t)(lds_base + local), tc.y * fr.width + tc.x, ps, o);
     count_rays(&fr.counters[0], rays, &lds_count);
     count_rays(&fr.counters[2], valid, &lds_count);
     if (cache.enabled) {  // uniform over the launch
@@ -256,6 +321,36 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
         count_rays(&fr.counters[3], kind == 1u ? 1u : 0u, &lds_count);
         count_rays(&fr.counters[4], kind == 2u ? 1u : 0u, &lds_count);
     }
+}
+
+// Remaining bounces of the paths the head kernel handed on (persistent grid,
+// the path count is on the device).
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_tail(SceneDev sc, FrameDev fr,
+                                                                                 UniformBufferObject ubo,
+                                                                                 CacheCfg cache) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const uint32_t count = *fr.tail_count;
+    uint32_t rays = 0;
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < count; q += gridDim.x * kBlock) {
+        const float4* rec = fr.tail + 4 * (size_t)q;
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+        const uint32_t pix = __float_as_uint(r0.w);
+        PathState ps;
+        ps.origin = xyz(r0);
+        ps.dir = xyz(r1);
+        ps.rng = __float_as_uint(r1.w);
+        ps.thr = xyz(r2);
+        ps.a0 = 0.0f;
+        PathOut o;
+        o.L = xyz(r3);
+        o.cache = __float_as_uint(r2.w);
+        o.valid = true;  // only paths with a diffuse primary hit get past bounce 0
+        path_bounces(sc, pix % fr.width, pix / fr.width, fr.width, ubo.numberOfBounces, kHeadBounces,
+                     ubo.numberOfBounces, ubo.hasSkyBox != 0u, &ps, stack + threadIdx.x, &o, cache, &rays);
+        fr.samples[4 * (size_t)fr.n + pix] = f4(o.L, __uint_as_float(o.cache));
+    }
+    count_rays(&fr.counters[0], rays, &lds_count);
 }
 
 __global__ void k_clear_u32(uint32_t* p, uint32_t n) {
@@ -691,8 +786,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __r
 // ---------------------------------------------------------------------------
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
                                  const CacheCfg& cache, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, 1u);
     hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        cache);
+    hipLaunchKernelGGL(k_initial_tail, dim3(2048), dim3(kBlock), 0, st, sc, fr, ubo, cache);
     return hipGetLastError();
 }
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
