@@ -582,8 +582,14 @@ __device__ __forceinline__ bool similar(float4 cx, float4 cn, float4 hx, float4 
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame, uint32_t reuse,
-                                                    PassParams pp) {
+// The reservoir r of Spatial.rgen is kept as its w_sum and M plus the index of
+// the pixel whose temporal reservoir holds the selected sample (MergeReservoir
+// copies that sample; the copy is made once, after the loop) and that
+// sample's visible point and normal (the origin of the next visibility ray).
+// Per neighbour only the planes that p_hat and the merge weight read are
+// loaded (x_s, n_s, L_out + W, M).
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame,
+                                                                            uint32_t reuse, PassParams pp) {
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
@@ -592,7 +598,6 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
         const uint32_t n = fr.n, w = fr.width, h = fr.height;
         const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * w + x;
-        uint32_t rng = init_seed(x, y, w, frame);
         const uint32_t ping = frame % 2u;
         const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
         float4* wr = fr.spatial + (size_t)(1u - ping) * kPlanes * n;
@@ -606,11 +611,23 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
             fr.output[pix] = 0xFFFFFFFFu;
             if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
-            Res center;
-            load_planes(rd, n, pix, center.p);
-            Res r;
-            res_init(&r);
-            res_merge(&r, center, length(xyz(center.p[4])), &rng);
+            uint32_t rng = init_seed(x, y, w, frame);
+            uint32_t sel = 0xFFFFFFFFu;  // none yet: InitReservoir's sample
+            float4 sel0 = make_float4(0.f, 0.f, 0.f, 0.f), sel1 = sel0;
+            float ws = 0.0f;
+            uint32_t m = 0u;
+            {  // MergeReservoir(r, centerRes, |L|)
+                const float4 c4 = rd[4 * n + pix], c5 = rd[5 * n + pix];
+                const uint32_t mc = __float_as_uint(c5.w);
+                const float wc = length(xyz(c4)) * c4.w * (float)mc;
+                ws = ws + wc;
+                if (rand01(&rng) < wc / maxf(1e-10f, ws)) {
+                    sel = pix;
+                    sel0 = rd[pix];
+                    sel1 = rd[n + pix];
+                }
+                m = 0u + mc;
+            }
             int* col = stack + threadIdx.x;
             for (uint32_t i = 0; i < pp.spatial_samples; ++i) {
                 const float ra = rand01(&rng) * 2.0f * XR_PI;
@@ -625,10 +642,10 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
                 const float4 q0 = fr.samples[nidx];
                 const float4 q1 = fr.samples[n + nidx];
                 if (length(xyz(q1)) == 0.0f || !similar(s0, s1, q0, q1)) continue;
-                Res rn;
-                load_planes(rd, n, nidx, rn.p);
-                const v3 xs = xyz(rn.p[2]);
-                const v3 ns = xyz(rn.p[3]);
+                const float4 rn2 = rd[2 * n + nidx], rn3 = rd[3 * n + nidx], rn4 = rd[4 * n + nidx];
+                const uint32_t mn = __float_as_uint(rd[5 * n + nidx].w);
+                const v3 xs = xyz(rn2);
+                const v3 ns = xyz(rn3);
                 const v3 vo = sub(xyz(q0), xs);
                 const float d2o = dot(vo, vo);
                 const v3 vc = sub(xyz(s0), xs);
@@ -642,27 +659,45 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
                     if (co > 1e-6f) jac = (cc * d2o) / (co * d2c);
                 }
                 jac = clampf(jac, 0.1f, 10.0f);
-                float ph = length(xyz(rn.p[4])) * jac;
+                float ph = length(xyz(rn4)) * jac;
                 // CheckVisibility (Spatial.rgen:44-62) from the currently
                 // selected sample's visible point, any-hit.  Its only effect
                 // is to zero ph, so it is traced only when ph is not already
                 // zero (same result, fewer rays: samples that found no light
                 // carry L_out = 0).
                 if (pp.visibility && ph != 0.0f) {
-                    const v3 from = xyz(r.p[0]);
-                    const v3 nf = xyz(r.p[1]);
+                    const v3 from = xyz(sel0);
                     const v3 dv = sub(xs, from);
                     const float dist = length(dv);
-                    const v3 o = add(from, muls(nf, 0.01f));
+                    const v3 o = add(from, muls(xyz(sel1), 0.01f));
                     const v3 dn = normalize(dv);
                     const RayPre rp = make_ray_pre(o, dn);
                     Hit hh;
                     ++rays;
                     if (trace_ray<kBlock, true>(sc.nodes, sc.tris, rp, 0.0f, dist - 0.01f, col, &hh)) ph = 0.0f;
                 }
-                res_merge(&r, rn, ph, &rng);
-                if (ph == 0.0f) set_m(&r, r_m(r) - r_m(rn));
+                // MergeReservoir(r, r_neighbor, ph): w = ph * W * M, update
+                // (one draw), m = M0 + M; then m -= M if ph == 0.
+                const uint32_t m0 = m;
+                const float wn = ph * rn4.w * (float)mn;
+                ws = ws + wn;
+                if (rand01(&rng) < wn / maxf(1e-10f, ws)) {
+                    sel = nidx;
+                    sel0 = rd[nidx];
+                    sel1 = rd[n + nidx];
+                }
+                m = m0 + mn;
+                if (ph == 0.0f) m = m - mn;
             }
+            Res r;
+            res_init(&r);
+            if (sel != 0xFFFFFFFFu) {
+                float4 sp[kPlanes];
+                load_planes(rd, n, sel, sp);
+                take_sample(&r, sp);
+            }
+            set_wsum(&r, ws);
+            set_m(&r, m);
             if (r_m(r) > pp.spatial_m_cap) set_m(&r, pp.spatial_m_cap);
             const float phf = length(xyz(r.p[4]));
             if (phf > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * phf));
