@@ -25,26 +25,31 @@ struct TileCoord {
     bool in_image;
 };
 
-// Logical tile of this workgroup: XCD k (= blockIdx.x % 8) gets tiles
-// [k*per, (k+1)*per), walked in row-major order.
+// Logical tile of this workgroup.  Workgroups are dispatched round-robin over
+// the 8 XCDs (blockIdx.x % 8), each with its own L2: XCD k gets the band of
+// tile rows [k*rows, (k+1)*rows) and walks it column by column, so the tiles
+// resident on one XCD at a time form a compact patch (the spatial pass
+// gathers neighbours up to 30 px away from it).
 __device__ __forceinline__ TileCoord tile_pixel(uint32_t width, uint32_t height) {
     const uint32_t tiles_x = (width + kTile - 1) / kTile;
     const uint32_t tiles_y = (height + kTile - 1) / kTile;
-    const uint32_t ntiles = tiles_x * tiles_y;
-    const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+    const uint32_t rows = (tiles_y + kXcds - 1) / kXcds;
     const uint32_t b = blockIdx.x;
-    const uint32_t tile = (b % kXcds) * per + b / kXcds;
+    const uint32_t j = b / kXcds;
+    const uint32_t ty = (b % kXcds) * rows + j % rows;
+    const uint32_t tx = j / rows;
     TileCoord tc;
     const uint32_t lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
-    tc.x = (tile % tiles_x) * kTile + lx;
-    tc.y = (tile / tiles_x) * kTile + ly;
-    tc.in_image = tile < ntiles && tc.x < width && tc.y < height;
+    tc.x = tx * kTile + lx;
+    tc.y = ty * kTile + ly;
+    tc.in_image = tx < tiles_x && ty < tiles_y && tc.x < width && tc.y < height;
     return tc;
 }
 
 uint32_t image_grid(uint32_t width, uint32_t height) {
-    const uint32_t tiles = ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
-    return ((tiles + kXcds - 1) / kXcds) * kXcds;
+    const uint32_t tiles_x = (width + kTile - 1) / kTile;
+    const uint32_t tiles_y = (height + kTile - 1) / kTile;
+    return kXcds * ((tiles_y + kXcds - 1) / kXcds) * tiles_x;
 }
 
 // Appends `value` of every lane whose pred is set to list (one global atomic
