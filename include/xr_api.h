@@ -351,6 +351,45 @@ int xr_mlp_fp8_weights(xr_ctx* ctx, uint8_t* codes, float* scales);
  * params.  Synchronous. */
 int xr_mlp_params(xr_ctx* ctx, float* params, int set);
 
+/* ---- batched views (no reference counterpart) ----
+ * Several cameras on the scene of one context, each with its own frame
+ * buffers and reservoir history, rendered in ReSTIR GI mode with one kernel
+ * launch per pass for all views; the frame is captured once as a HIP graph
+ * and replayed.  Each view's results equal those of rendering it alone.
+ * Errors are reported through xr_last_error(ctx).  Destroy a batch before
+ * its context. */
+typedef struct xr_batch xr_batch;
+int xr_batch_create(xr_ctx* ctx, uint32_t view_count, const uint32_t* widths, const uint32_t* heights,
+                    xr_batch** out);
+/* One ReSTIR GI frame of every view; ubos[view_count].  params as for
+ * xr_render (no radiance cache, every pass).  Asynchronous. */
+int xr_batch_render(xr_batch* batch, const xr_ubo* ubos, const xr_restir_params* params);
+/* Milliseconds of the last xr_batch_render (events around it).  Waits. */
+int xr_batch_frame_time(xr_batch* batch, float* ms);
+/* enable != 0 (default): replay a captured HIP graph; 0: launch eagerly. */
+int xr_batch_set_graph(xr_batch* batch, int enable);
+/* Kernel configurations (tile shape and LDS traversal-stack depth). */
+uint32_t xr_batch_config_count(void);
+int xr_batch_config_info(uint32_t config, uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_depth);
+int xr_batch_set_config(xr_batch* batch, uint32_t config);
+uint32_t xr_batch_config(const xr_batch* batch);
+/* Autotuner: renders frames (1..64) of the given views with every
+ * configuration, keeps the fastest (ms_per_config, if not NULL, receives
+ * xr_batch_config_count() averages), then clears the views' buffers and the
+ * counters. */
+int xr_batch_autotune(xr_batch* batch, const xr_ubo* ubos, const xr_restir_params* params, uint32_t frames,
+                      float* ms_per_config);
+/* Clears every view's buffers (reservoir history included). */
+int xr_batch_reset(xr_batch* batch);
+/* Counters summed over the views, as xr_counters. */
+int xr_batch_counters(xr_batch* batch, uint64_t* out, uint32_t n);
+int xr_batch_reset_counters(xr_batch* batch);
+/* Per-view buffers: XR_BUF_OUTPUT, XR_BUF_MOTION, XR_BUF_GBUF_POS,
+ * XR_BUF_SAMPLES, XR_BUF_TEMPORAL, XR_BUF_SPATIAL (0 bytes: not kept). */
+size_t xr_batch_buffer_bytes(const xr_batch* batch, uint32_t view, int which, int layout);
+int xr_batch_read_buffer(xr_batch* batch, uint32_t view, int which, void* dst, size_t bytes, int layout);
+void xr_batch_destroy(xr_batch* batch);
+
 /* ---- errors ---- */
 const char* xr_last_error(const xr_ctx* ctx);
 /* Error of the last failed call that had no context (xr_create). Thread local. */

@@ -778,22 +778,19 @@ static inline const float* plane_at(const std::vector<float4>& v, size_t n, size
     return &v[(slot * kPlanes + p) * n + i].x;
 }
 
-int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
-    if (!c || !dst) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (layout != XR_LAYOUT_REFERENCE && layout != XR_LAYOUT_NATIVE)
-        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
-    const size_t want = xr_buffer_bytes(c, which, layout);
-    if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
-    if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
-    void* src = buffer_ptr(c, which);
-    if (!src) return set_err(c, XR_ERR_STATE, "buffer %d has not been created yet", which);
+}  // extern "C"
+
+// Copies buffer `which` of an image of n pixels at device address src to dst
+// (bytes already checked), converting the sample / reservoir planes to the
+// reference AoS layout when asked.
+static int read_device_buffer(xr_ctx* c, const void* src, size_t n, int which, void* dst, size_t bytes,
+                              int layout) {
     const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
     if (!planar || layout == XR_LAYOUT_NATIVE) {
         XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         XR_HIP(c, hipStreamSynchronize(c->stream));
         return XR_OK;
     }
-    const size_t n = (size_t)c->width * c->height;
     const size_t entries = which == XR_BUF_SAMPLES ? n : 2 * n;
     std::vector<float4> tmp;
     try {
@@ -835,6 +832,20 @@ int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
         }
     }
     return XR_OK;
+}
+
+extern "C" {
+
+int xr_read_buffer(xr_ctx* c, int which, void* dst, size_t bytes, int layout) {
+    if (!c || !dst) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (layout != XR_LAYOUT_REFERENCE && layout != XR_LAYOUT_NATIVE)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    const size_t want = xr_buffer_bytes(c, which, layout);
+    if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown buffer %d", which);
+    if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
+    const void* src = buffer_ptr(c, which);
+    if (!src) return set_err(c, XR_ERR_STATE, "buffer %d has not been created yet", which);
+    return read_device_buffer(c, src, (size_t)c->width * c->height, which, dst, bytes, layout);
 }
 
 int xr_write_buffer(xr_ctx* c, int which, const void* src, size_t bytes, int layout) {
@@ -1040,6 +1051,382 @@ int xr_mlp_fp8_weights(xr_ctx* c, uint8_t* codes, float* scales) {
     XR_HIP(c, hipMemcpyAsync(scales, c->mlp.f8_scale, 5 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
     return XR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched views
+// ---------------------------------------------------------------------------
+struct xr_batch {
+    xr_ctx* ctx = nullptr;  // scene, BVH, stream, errors
+    uint32_t nviews = 0;
+    struct View {
+        uint32_t w = 0, h = 0;
+        float4 *samples = nullptr, *temporal = nullptr, *spatial = nullptr, *gpos = nullptr, *gnrm = nullptr,
+               *gmatuv = nullptr;
+        uint32_t *motion = nullptr, *output = nullptr;
+    };
+    std::vector<View> views;
+    std::vector<FrameDev> frames;  // host copy of d_frames
+    FrameDev* d_frames = nullptr;
+    UniformBufferObject* d_ubos = nullptr;
+    float4* d_tail = nullptr;
+    uint32_t* d_tail_count = nullptr;
+    unsigned long long* d_counters = nullptr;
+    uint32_t cfg = 0;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PassParams graph_pp{};
+    uint32_t graph_cfg = 0;
+    const void* graph_nodes = nullptr;  // BVH the graph was captured with
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+void batch_free(xr_batch* b) {
+    for (auto& v : b->views) {
+        dev_free(&v.samples);
+        dev_free(&v.temporal);
+        dev_free(&v.spatial);
+        dev_free(&v.gpos);
+        dev_free(&v.gnrm);
+        dev_free(&v.gmatuv);
+        dev_free(&v.motion);
+        dev_free(&v.output);
+    }
+    dev_free(&b->d_frames);
+    dev_free(&b->d_ubos);
+    dev_free(&b->d_tail);
+    dev_free(&b->d_tail_count);
+    dev_free(&b->d_counters);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->e0) hipEventDestroy(b->e0);
+    if (b->e1) hipEventDestroy(b->e1);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Zeroes every view's buffers (history included).
+int batch_clear(xr_batch* b) {
+    xr_ctx* c = b->ctx;
+    const size_t f4 = sizeof(float4);
+    for (auto& v : b->views) {
+        const size_t n = (size_t)v.w * v.h;
+        XR_HIP(c, hipMemsetAsync(v.samples, 0, kPlanes * n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.temporal, 0, 2 * kPlanes * n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.spatial, 0, 2 * kPlanes * n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.gpos, 0, n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.gnrm, 0, n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.gmatuv, 0, n * f4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.motion, 0, n * 4, c->stream));
+        XR_HIP(c, hipMemsetAsync(v.output, 0, n * 4, c->stream));
+    }
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}
+
+BatchDev batch_dev(const xr_batch* b) {
+    BatchDev bd;
+    bd.views = b->d_frames;
+    bd.ubos = b->d_ubos;
+    bd.nviews = b->nviews;
+    bd.tail = b->d_tail;
+    bd.tail_count = b->d_tail_count;
+    bd.counters = b->d_counters;
+    return bd;
+}
+
+PassParams pass_params(const xr_restir_params& p) {
+    PassParams pp;
+    pp.temporal_m_cap = p.temporal_m_cap;
+    pp.spatial_m_cap = p.spatial_m_cap;
+    pp.spatial_samples = p.spatial_samples;
+    pp.spatial_radius = p.spatial_radius;
+    pp.visibility = p.visibility;
+    pp.plain_max_depth = p.plain_max_depth;
+    pp.history = 0;
+    return pp;
+}
+
+bool same_pp(const PassParams& a, const PassParams& b) { return memcmp(&a, &b, sizeof(a)) == 0; }
+
+// One frame of every view: eager launches, or the captured graph (captured
+// again when the parameters, the kernel configuration or the BVH changed).
+int batch_frame(xr_batch* b, const PassParams& pp) {
+    xr_ctx* c = b->ctx;
+    const SceneDev sc = scene_dev(c);
+    const BatchDev bd = batch_dev(b);
+    if (!b->use_graph) {
+        XR_HIP(c, launch_batch_frame(sc, bd, b->frames.data(), b->cfg, pp, c->stream));
+        return XR_OK;
+    }
+    if (!b->exec || !same_pp(pp, b->graph_pp) || b->graph_cfg != b->cfg || b->graph_nodes != (const void*)c->d_nodes) {
+        if (b->exec) hipGraphExecDestroy(b->exec);
+        if (b->graph) hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        XR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch_batch_frame(sc, bd, b->frames.data(), b->cfg, pp, c->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        XR_HIP(c, le);
+        XR_HIP(c, ce);
+        b->graph = g;
+        XR_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_pp = pp;
+        b->graph_cfg = b->cfg;
+        b->graph_nodes = c->d_nodes;
+    }
+    XR_HIP(c, hipGraphLaunch(b->exec, c->stream));
+    return XR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, const uint32_t* heights,
+                    xr_batch** out) {
+    if (!out) return set_err(c, XR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!c || !widths || !heights) return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (view_count == 0 || view_count > 64) return set_err(c, XR_ERR_INVALID_ARGUMENT, "view_count must be 1..64");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "build the acceleration structure first");
+    size_t total = 0;
+    for (uint32_t v = 0; v < view_count; ++v) {
+        if (widths[v] == 0 || heights[v] == 0 || widths[v] > 16384 || heights[v] > 16384)
+            return set_err(c, XR_ERR_INVALID_ARGUMENT, "view %u size %ux%u out of range", v, widths[v], heights[v]);
+        total += (size_t)widths[v] * heights[v];
+    }
+    if (total > 0xFFFFFFFFull / 4) return set_err(c, XR_ERR_INVALID_ARGUMENT, "batch has too many pixels");
+    xr_batch* b = new (std::nothrow) xr_batch;
+    if (!b) return set_err(c, XR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    b->ctx = c;
+    b->nviews = view_count;
+    b->views.resize(view_count);
+    b->frames.resize(view_count);
+    const size_t f4 = sizeof(float4);
+    int rc = XR_OK;
+#define XR_BATCH_HIP(expr)                                                                                \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            rc = set_err(c, hip_code(e_), "%s: %s", #expr, hipGetErrorString(e_));                       \
+            goto fail;                                                                                    \
+        }                                                                                                 \
+    } while (0)
+    for (uint32_t v = 0; v < view_count; ++v) {
+        xr_batch::View& vw = b->views[v];
+        vw.w = widths[v];
+        vw.h = heights[v];
+        const size_t n = (size_t)vw.w * vw.h;
+        XR_BATCH_HIP(dev_alloc(&vw.samples, kPlanes * n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.temporal, 2 * kPlanes * n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.spatial, 2 * kPlanes * n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.gpos, n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.gnrm, n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.gmatuv, n * f4));
+        XR_BATCH_HIP(dev_alloc(&vw.motion, n * 4));
+        XR_BATCH_HIP(dev_alloc(&vw.output, n * 4));
+    }
+    XR_BATCH_HIP(dev_alloc(&b->d_frames, sizeof(FrameDev) * view_count));
+    XR_BATCH_HIP(dev_alloc(&b->d_ubos, sizeof(UniformBufferObject) * view_count));
+    XR_BATCH_HIP(dev_alloc(&b->d_tail, 4 * total * f4));
+    XR_BATCH_HIP(dev_alloc(&b->d_tail_count, sizeof(uint32_t)));
+    XR_BATCH_HIP(dev_alloc(&b->d_counters, sizeof(unsigned long long) * kCounters));
+    XR_BATCH_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
+    XR_BATCH_HIP(hipEventCreate(&b->e0));
+    XR_BATCH_HIP(hipEventCreate(&b->e1));
+    for (uint32_t v = 0; v < view_count; ++v) {
+        const xr_batch::View& vw = b->views[v];
+        FrameDev f{};
+        f.width = vw.w;
+        f.height = vw.h;
+        f.n = vw.w * vw.h;
+        f.samples = vw.samples;
+        f.temporal = vw.temporal;
+        f.spatial = vw.spatial;
+        f.gpos = vw.gpos;
+        f.gnrm = vw.gnrm;
+        f.gmatuv = vw.gmatuv;
+        f.motion = vw.motion;
+        f.output = vw.output;
+        f.accum = nullptr;
+        f.radiance = nullptr;
+        f.counters = b->d_counters;
+        f.cache_list = nullptr;
+        f.tail = b->d_tail;
+        f.tail_count = b->d_tail_count;
+        b->frames[v] = f;
+    }
+    XR_BATCH_HIP(hipMemcpyAsync(b->d_frames, b->frames.data(), sizeof(FrameDev) * view_count, hipMemcpyHostToDevice,
+                                c->stream));
+#undef XR_BATCH_HIP
+    rc = batch_clear(b);
+    if (rc) goto fail;
+    *out = b;
+    return XR_OK;
+fail:
+    batch_free(b);
+    delete b;
+    return rc;
+}
+
+int xr_batch_render(xr_batch* b, const xr_ubo* ubos, const xr_restir_params* params) {
+    if (!b || !ubos) return set_err(b ? b->ctx : nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    xr_ctx* c = b->ctx;
+    xr_restir_params p;
+    if (params) p = *params;
+    else xr_default_params(&p);
+    int rc = check_params(c, &p);
+    if (rc) return rc;
+    if (p.flags & (XR_FLAG_MLP_CACHE | XR_FLAG_MLP_FP8))
+        return set_err(c, XR_ERR_UNSUPPORTED, "batched views do not use the radiance cache");
+    if (p.passes && p.passes != 7u) return set_err(c, XR_ERR_UNSUPPORTED, "batched views run every pass");
+    if (!c->built) return set_err(c, XR_ERR_STATE, "acceleration structure not built");
+    // pageable source: the copy is staged before the call returns
+    XR_HIP(c, hipMemcpyAsync(b->d_ubos, ubos, sizeof(UniformBufferObject) * b->nviews, hipMemcpyHostToDevice,
+                             c->stream));
+    XR_HIP(c, hipEventRecord(b->e0, c->stream));
+    rc = batch_frame(b, pass_params(p));
+    if (rc) return rc;
+    XR_HIP(c, hipEventRecord(b->e1, c->stream));
+    return XR_OK;
+}
+
+int xr_batch_frame_time(xr_batch* b, float* ms) {
+    if (!b || !ms) return set_err(b ? b->ctx : nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    XR_HIP(b->ctx, hipEventSynchronize(b->e1));
+    XR_HIP(b->ctx, hipEventElapsedTime(ms, b->e0, b->e1));
+    return XR_OK;
+}
+
+int xr_batch_set_graph(xr_batch* b, int enable) {
+    if (!b) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    b->use_graph = enable != 0;
+    return XR_OK;
+}
+
+uint32_t xr_batch_config_count(void) { return batch_config_count(); }
+
+int xr_batch_config_info(uint32_t config, uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_depth) {
+    if (config >= batch_config_count() || !tile_w || !tile_h || !lds_depth)
+        return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "config %u / NULL argument", config);
+    batch_config_info(config, tile_w, tile_h, lds_depth);
+    return XR_OK;
+}
+
+int xr_batch_set_config(xr_batch* b, uint32_t config) {
+    if (!b) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (config >= batch_config_count()) return set_err(b->ctx, XR_ERR_INVALID_ARGUMENT, "config %u", config);
+    b->cfg = config;
+    return XR_OK;
+}
+
+uint32_t xr_batch_config(const xr_batch* b) { return b ? b->cfg : 0u; }
+
+int xr_batch_autotune(xr_batch* b, const xr_ubo* ubos, const xr_restir_params* params, uint32_t frames,
+                      float* ms_per_config) {
+    if (!b || !ubos) return set_err(b ? b->ctx : nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (frames == 0 || frames > 64) return set_err(b->ctx, XR_ERR_INVALID_ARGUMENT, "frames must be 1..64");
+    xr_ctx* c = b->ctx;
+    const uint32_t ncfg = batch_config_count();
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    float best = 0.0f;
+    uint32_t best_cfg = 0;
+    std::vector<xr_ubo> u(ubos, ubos + b->nviews);
+    int rc = XR_OK;
+    for (uint32_t k = 0; k < ncfg && rc == XR_OK; ++k) {
+        b->cfg = k;
+        float sum = 0.0f;
+        for (uint32_t f = 0; f <= frames && rc == XR_OK; ++f) {  // frame 0 warms up
+            for (uint32_t v = 0; v < b->nviews; ++v) u[v].current_frame = ubos[v].current_frame + f;
+            rc = xr_batch_render(b, u.data(), params);
+            float ms = 0.0f;
+            if (rc == XR_OK) rc = xr_batch_frame_time(b, &ms);
+            if (f > 0) sum += ms;
+        }
+        if (rc) break;
+        const float avg = sum / (float)frames;
+        if (ms_per_config) ms_per_config[k] = avg;
+        if (k == 0 || avg < best) {
+            best = avg;
+            best_cfg = k;
+        }
+    }
+    b->use_graph = graph;
+    b->cfg = best_cfg;
+    if (rc) return rc;
+    rc = batch_clear(b);  // tuning frames leave no history behind
+    if (rc) return rc;
+    XR_HIP(c, hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
+    return XR_OK;
+}
+
+int xr_batch_reset(xr_batch* b) {
+    if (!b) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    return batch_clear(b);
+}
+
+int xr_batch_counters(xr_batch* b, uint64_t* out, uint32_t n) {
+    if (!b || !out) return set_err(b ? b->ctx : nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    unsigned long long h[kCounters];
+    XR_HIP(b->ctx, hipMemcpyAsync(h, b->d_counters, sizeof(h), hipMemcpyDeviceToHost, b->ctx->stream));
+    XR_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    for (uint32_t i = 0; i < n && i < kCounters; ++i) out[i] = h[i];
+    return XR_OK;
+}
+
+int xr_batch_reset_counters(xr_batch* b) {
+    if (!b) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    XR_HIP(b->ctx, hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * kCounters, b->ctx->stream));
+    return XR_OK;
+}
+
+size_t xr_batch_buffer_bytes(const xr_batch* b, uint32_t view, int which, int layout) {
+    if (!b || view >= b->nviews) return 0;
+    const size_t n = (size_t)b->views[view].w * b->views[view].h;
+    switch (which) {
+        case XR_BUF_OUTPUT:
+        case XR_BUF_MOTION: return n * 4;
+        case XR_BUF_GBUF_POS: return n * 16;
+        case XR_BUF_SAMPLES: return layout == XR_LAYOUT_REFERENCE ? n * sizeof(ReSTIRGISample) : n * kPlanes * 16;
+        case XR_BUF_TEMPORAL:
+        case XR_BUF_SPATIAL:
+            return layout == XR_LAYOUT_REFERENCE ? 2 * n * sizeof(ReSTIRGIReservoir) : 2 * n * kPlanes * 16;
+        default: return 0;
+    }
+}
+
+int xr_batch_read_buffer(xr_batch* b, uint32_t view, int which, void* dst, size_t bytes, int layout) {
+    if (!b || !dst) return set_err(b ? b->ctx : nullptr, XR_ERR_INVALID_ARGUMENT, "NULL argument");
+    xr_ctx* c = b->ctx;
+    if (view >= b->nviews) return set_err(c, XR_ERR_INVALID_ARGUMENT, "view %u of %u", view, b->nviews);
+    if (layout != XR_LAYOUT_REFERENCE && layout != XR_LAYOUT_NATIVE)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown layout %d", layout);
+    const size_t want = xr_batch_buffer_bytes(b, view, which, layout);
+    if (want == 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d is not kept per view", which);
+    if (bytes != want) return set_err(c, XR_ERR_INVALID_ARGUMENT, "buffer %d needs %zu bytes, got %zu", which, want, bytes);
+    const xr_batch::View& vw = b->views[view];
+    const void* src = which == XR_BUF_OUTPUT     ? (const void*)vw.output
+                      : which == XR_BUF_MOTION   ? (const void*)vw.motion
+                      : which == XR_BUF_GBUF_POS ? (const void*)vw.gpos
+                      : which == XR_BUF_SAMPLES  ? (const void*)vw.samples
+                      : which == XR_BUF_TEMPORAL ? (const void*)vw.temporal
+                                                 : (const void*)vw.spatial;
+    return read_device_buffer(c, src, (size_t)vw.w * vw.h, which, dst, bytes, layout);
+}
+
+void xr_batch_destroy(xr_batch* b) {
+    if (!b) return;
+    if (b->ctx && b->ctx->stream) hipStreamSynchronize(b->ctx->stream);
+    batch_free(b);
+    delete b;
 }
 
 }  // extern "C"

@@ -78,11 +78,13 @@ constexpr uint32_t kMaxNodeVisits = 1u << 24;
 // latency-bound and the extra waves win (5.00 -> 4.63 ms per 1080p frame).
 #define XR_TRAVERSAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
 
-template <int kBlock, bool kAnyHit>
+// kLds: stack levels kept in LDS (the rest, up to kMaxTraversalDepth, in a
+// private array); the batched kernels pick it by measurement.
+template <int kBlock, bool kAnyHit, int kLds = kLdsDepth>
 __device__ __forceinline__ bool trace_ray(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
                                           const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
                                           Hit* hit) {
-    int spill[kSpillDepth];
+    int spill[kMaxTraversalDepth - kLds];
     int sp = 0;
     int node = 0;
     float best_t = tmax;
@@ -133,8 +135,8 @@ __device__ __forceinline__ bool trace_ray(const BvhNode* __restrict__ nodes, con
         }
         if (h0 && h1) {
             if (e1 < e0) { int tmp = c0; c0 = c1; c1 = tmp; }
-            if (sp < kLdsDepth) lds_col[sp * kBlock] = c1;
-            else spill[sp - kLdsDepth] = c1;
+            if (sp < kLds) lds_col[sp * kBlock] = c1;
+            else spill[sp - kLds] = c1;
             ++sp;
             node = c0;
         } else if (h0) {
@@ -144,7 +146,7 @@ __device__ __forceinline__ bool trace_ray(const BvhNode* __restrict__ nodes, con
         } else {
             if (sp == 0) break;
             --sp;
-            node = sp < kLdsDepth ? lds_col[sp * kBlock] : spill[sp - kLdsDepth];
+            node = sp < kLds ? lds_col[sp * kBlock] : spill[sp - kLds];
         }
     }
     if (kAnyHit) return false;

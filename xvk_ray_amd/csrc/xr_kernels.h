@@ -12,6 +12,20 @@ hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const U
                                  const CacheCfg& cache, hipStream_t st);
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
                            hipStream_t st);
+// Views rendered together: device arrays of their frame buffers and UBOs, a
+// shared handed-on path list (records name their view) and shared counters.
+struct BatchDev {
+    const FrameDev* views;
+    const UniformBufferObject* ubos;
+    uint32_t nviews;
+    float4* tail;
+    uint32_t* tail_count;
+    unsigned long long* counters;
+};
+uint32_t batch_config_count();
+void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds);
+hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                              const PassParams& pp, hipStream_t st);
 hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
                           const PassParams& pp, hipStream_t st);
 hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,

@@ -30,27 +30,34 @@ struct TileCoord {
 // tile rows [k*rows, (k+1)*rows) and walks it column by column, so the tiles
 // resident on one XCD at a time form a compact patch (the spatial pass
 // gathers neighbours up to 30 px away from it).
-__device__ __forceinline__ TileCoord tile_pixel(uint32_t width, uint32_t height) {
-    const uint32_t tiles_x = (width + kTile - 1) / kTile;
-    const uint32_t tiles_y = (height + kTile - 1) / kTile;
+// Tiles of kTw x kTh pixels (kTw * kTh = kBlock); b is the workgroup's index
+// within the image's grid (a multiple of 8 workgroups, see image_grid).
+template <int kTw, int kTh>
+__device__ __forceinline__ TileCoord tile_coord(uint32_t b, uint32_t width, uint32_t height) {
+    static_assert(kTw * kTh == kBlock, "one thread per pixel of the tile");
+    const uint32_t tiles_x = (width + kTw - 1) / kTw;
+    const uint32_t tiles_y = (height + kTh - 1) / kTh;
     const uint32_t rows = (tiles_y + kXcds - 1) / kXcds;
-    const uint32_t b = blockIdx.x;
     const uint32_t j = b / kXcds;
     const uint32_t ty = (b % kXcds) * rows + j % rows;
     const uint32_t tx = j / rows;
     TileCoord tc;
-    const uint32_t lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
-    tc.x = tx * kTile + lx;
-    tc.y = ty * kTile + ly;
+    const uint32_t lx = threadIdx.x % kTw, ly = threadIdx.x / kTw;
+    tc.x = tx * kTw + lx;
+    tc.y = ty * kTh + ly;
     tc.in_image = tx < tiles_x && ty < tiles_y && tc.x < width && tc.y < height;
     return tc;
 }
+__device__ __forceinline__ TileCoord tile_pixel(uint32_t width, uint32_t height) {
+    return tile_coord<kTile, kTile>(blockIdx.x, width, height);
+}
 
-uint32_t image_grid(uint32_t width, uint32_t height) {
-    const uint32_t tiles_x = (width + kTile - 1) / kTile;
-    const uint32_t tiles_y = (height + kTile - 1) / kTile;
+uint32_t image_grid(uint32_t width, uint32_t height, uint32_t tw, uint32_t th) {
+    const uint32_t tiles_x = (width + tw - 1) / tw;
+    const uint32_t tiles_y = (height + th - 1) / th;
     return kXcds * ((tiles_y + kXcds - 1) / kXcds) * tiles_x;
 }
+uint32_t image_grid(uint32_t width, uint32_t height) { return image_grid(width, height, kTile, kTile); }
 
 // Appends `value` of every lane whose pred is set to list (one global atomic
 // per wave, slots in lane order).  Called by every lane of the wave.
@@ -140,6 +147,7 @@ __device__ __forceinline__ void path_begin(const UniformBufferObject& ubo, uint3
 // Bounces b0 .. b1-1 of the loop.  Returns true when the path is still alive
 // at bounce b1 (< bounces), false when it ended (miss, emitter, cache query,
 // Russian roulette or the bounce limit).  *rays counts the rays traced.
+template <int kLds = kLdsDepth>
 __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uint32_t y, uint32_t w,
                                              uint32_t bounces, uint32_t b0, uint32_t b1, bool sky, PathState* ps,
                                              int* col, PathOut* o, const CacheCfg& cache, uint32_t* rays) {
@@ -151,7 +159,7 @@ __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uin
         ++*rays;
         // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
         Payload pl;
-        bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+        bool found = trace_ray<kBlock, false, kLds>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
         if (found) {
             closest_hit(sc, hit, &pl);
             found = pl.material != -1;
@@ -250,12 +258,74 @@ __device__ __forceinline__ uint32_t trace_path(const SceneDev& sc, const Uniform
 constexpr uint32_t kHeadBounces = 2;
 
 // Path handed from the head to the tail kernel: 4 float4.
-//   [0] origin, pixel   [1] direction, rng   [2] throughput, cache kind   [3] L_out so far
-__device__ __forceinline__ void store_path(float4* rec, uint32_t pix, const PathState& ps, const PathOut& o) {
+//   [0] origin, pixel   [1] direction, rng   [2] throughput, cache kind   [3] L_out so far, view
+__device__ __forceinline__ void store_path(float4* rec, uint32_t pix, uint32_t view, const PathState& ps,
+                                           const PathOut& o) {
     rec[0] = f4(ps.origin, __uint_as_float(pix));
     rec[1] = f4(ps.dir, __uint_as_float(ps.rng));
     rec[2] = f4(ps.thr, __uint_as_float(o.cache));
-    rec[3] = f4(o.L, 0.0f);
+    rec[3] = f4(o.L, __uint_as_float(view));
+}
+
+// Head of the initial-sample pass for pixel (x, y): bounces 0 and 1 and every
+// write of the pass except, for a path still alive, L_out (written by the
+// tail).  Returns whether the path is alive.
+template <int kLds>
+__device__ __forceinline__ bool initial_head_pixel(const SceneDev& sc, const FrameDev& fr,
+                                                   const UniformBufferObject& ubo, uint32_t x, uint32_t y,
+                                                   const CacheCfg& cache, int* col, PathState* ps, PathOut* po,
+                                                   uint32_t* rays) {
+    PathOut& o = *po;
+    const uint32_t pix = y * fr.width + x;
+    path_begin(ubo, x, y, fr.width, fr.height, ubo.currentFrame, ps, &o);
+    const uint32_t nb = ubo.numberOfBounces;
+    const bool alive = path_bounces<kLds>(sc, x, y, fr.width, nb, 0u, nb < kHeadBounces ? nb : kHeadBounces,
+                                          ubo.hasSkyBox != 0u, ps, col, &o, cache, rays);
+    if (!alive) path_end(&o);
+    const uint32_t n = fr.n;
+    float4* S = fr.samples;
+    // Spare words (zero without the cache): plane 3 .w = material at x_s,
+    // plane 4 .w = 1 cache query / 2 training sample.
+    S[0 * n + pix] = f4(o.x_view, __uint_as_float(o.mat_idx));
+    S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
+    S[2 * n + pix] = f4(o.x_s, o.p_q);
+    S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
+    if (!alive) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
+    S[5 * n + pix] = f4(o.f0, 0.0f);
+    if (o.valid) {
+        fr.gpos[pix] = f4(o.x_view, 1.0f);
+        fr.gnrm[pix] = f4(o.n_view, 0.0f);
+        fr.gmatuv[pix] = make_float4((float)o.mat_idx, o.uv_u, o.uv_v, 0.0f);
+        const float cu = ((float)x + 0.5f) / (float)fr.width;
+        const float cv = ((float)y + 0.5f) / (float)fr.height;
+        float vp[4], cp[4];
+        mat_vec(ubo.viewPrev, o.x_view.x, o.x_view.y, o.x_view.z, 1.0f, vp);
+        mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
+        const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
+        const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
+        const uint32_t mx = f32_to_f16_bits(cu - pu), my = f32_to_f16_bits(cv - pv);
+        fr.motion[pix] = mx | (my << 16);
+    } else {
+        fr.gpos[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        fr.gnrm[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        fr.gmatuv[pix] = make_float4(-1.f, 0.f, 0.f, 0.f);
+        fr.motion[pix] = 0u;
+    }
+    return alive;
+}
+
+// Hands the workgroup's live paths to the tail list: one global atomic per
+// workgroup.  Called by every thread of the workgroup.
+__device__ __forceinline__ void hand_on(bool alive, float4* tail, uint32_t* tail_count, uint32_t pix, uint32_t view,
+                                        const PathState& ps, const PathOut& o, uint32_t* lds_count,
+                                        uint32_t* lds_base) {
+    if (threadIdx.x == 0) *lds_count = 0;
+    __syncthreads();
+    const uint32_t local = alive ? atomicAdd(lds_count, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_base = *lds_count ? atomicAdd(tail_count, *lds_count) : 0u;
+    __syncthreads();
+    if (alive) store_path(tail + 4 * (size_t)(*lds_base + local), pix, view, ps, o);
 }
 
 __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr,
@@ -270,53 +340,11 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
     PathState ps;
     PathOut o;
     if (tc.in_image) {
-        const uint32_t x = tc.x, y = tc.y;
-        const uint32_t pix = y * fr.width + x;
-        path_begin(ubo, x, y, fr.width, fr.height, ubo.currentFrame, &ps, &o);
-        const uint32_t nb = ubo.numberOfBounces;
-        alive = path_bounces(sc, x, y, fr.width, nb, 0u, nb < kHeadBounces ? nb : kHeadBounces,
-                             ubo.hasSkyBox != 0u, &ps, stack + threadIdx.x, &o, cache, &rays);
-        if (!alive) path_end(&o);
+        alive = initial_head_pixel<kLdsDepth>(sc, fr, ubo, tc.x, tc.y, cache, stack + threadIdx.x, &ps, &o, &rays);
         kind = o.cache;
-        const uint32_t n = fr.n;
-        float4* S = fr.samples;
-        // Spare words (zero without the cache): plane 3 .w = material at x_s,
-        // plane 4 .w = 1 cache query / 2 training sample.
-        S[0 * n + pix] = f4(o.x_view, __uint_as_float(o.mat_idx));
-        S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
-        S[2 * n + pix] = f4(o.x_s, o.p_q);
-        S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
-        if (!alive) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
-        S[5 * n + pix] = f4(o.f0, 0.0f);
-        if (o.valid) {
-            fr.gpos[pix] = f4(o.x_view, 1.0f);
-            fr.gnrm[pix] = f4(o.n_view, 0.0f);
-            fr.gmatuv[pix] = make_float4((float)o.mat_idx, o.uv_u, o.uv_v, 0.0f);
-            const float cu = ((float)x + 0.5f) / (float)fr.width;
-            const float cv = ((float)y + 0.5f) / (float)fr.height;
-            float vp[4], cp[4];
-            mat_vec(ubo.viewPrev, o.x_view.x, o.x_view.y, o.x_view.z, 1.0f, vp);
-            mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
-            const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
-            const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
-            const uint32_t mx = f32_to_f16_bits(cu - pu), my = f32_to_f16_bits(cv - pv);
-            fr.motion[pix] = mx | (my << 16);
-        } else {
-            fr.gpos[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-            fr.gnrm[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-            fr.gmatuv[pix] = make_float4(-1.f, 0.f, 0.f, 0.f);
-            fr.motion[pix] = 0u;
-        }
         valid = o.valid ? 1u : 0u;
     }
-    // hand the live paths on: one global atomic per workgroup
-    if (threadIdx.x == 0) lds_count = 0;
-    __syncthreads();
-    const uint32_t local = alive ? atomicAdd(&lds_count, 1u) : 0u;
-    __syncthreads();
-    if (threadIdx.x == 0) lds_base = lds_count ? atomicAdd(fr.tail_count, lds_count) : 0u;
-    __syncthreads();
-    if (alive) store_path(fr.tail + 4 * (size_t)(lds_base + local), tc.y * fr.width + tc.x, ps, o);
+    hand_on(alive, fr.tail, fr.tail_count, tc.y * fr.width + tc.x, 0u, ps, o, &lds_count, &lds_base);
     count_rays(&fr.counters[0], rays, &lds_count);
     count_rays(&fr.counters[2], valid, &lds_count);
     if (cache.enabled) {  // uniform over the launch
@@ -328,6 +356,28 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
     }
 }
 
+// Remaining bounces of one handed-on path; writes its L_out.
+template <int kLds>
+__device__ __forceinline__ void initial_tail_path(const SceneDev& sc, const FrameDev& fr,
+                                                  const UniformBufferObject& ubo, const float4* rec,
+                                                  const CacheCfg& cache, int* col, uint32_t* rays) {
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const uint32_t pix = __float_as_uint(r0.w);
+    PathState ps;
+    ps.origin = xyz(r0);
+    ps.dir = xyz(r1);
+    ps.rng = __float_as_uint(r1.w);
+    ps.thr = xyz(r2);
+    ps.a0 = 0.0f;
+    PathOut o;
+    o.L = xyz(r3);
+    o.cache = __float_as_uint(r2.w);
+    o.valid = true;  // only paths with a diffuse primary hit get past bounce 0
+    path_bounces<kLds>(sc, pix % fr.width, pix / fr.width, fr.width, ubo.numberOfBounces, kHeadBounces,
+                       ubo.numberOfBounces, ubo.hasSkyBox != 0u, &ps, col, &o, cache, rays);
+    fr.samples[4 * (size_t)fr.n + pix] = f4(o.L, __uint_as_float(o.cache));
+}
+
 // Remaining bounces of the paths the head kernel handed on (persistent grid,
 // the path count is on the device).
 __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_tail(SceneDev sc, FrameDev fr,
@@ -337,24 +387,8 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_tail(
     __shared__ uint32_t lds_count;
     const uint32_t count = *fr.tail_count;
     uint32_t rays = 0;
-    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < count; q += gridDim.x * kBlock) {
-        const float4* rec = fr.tail + 4 * (size_t)q;
-        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
-        const uint32_t pix = __float_as_uint(r0.w);
-        PathState ps;
-        ps.origin = xyz(r0);
-        ps.dir = xyz(r1);
-        ps.rng = __float_as_uint(r1.w);
-        ps.thr = xyz(r2);
-        ps.a0 = 0.0f;
-        PathOut o;
-        o.L = xyz(r3);
-        o.cache = __float_as_uint(r2.w);
-        o.valid = true;  // only paths with a diffuse primary hit get past bounce 0
-        path_bounces(sc, pix % fr.width, pix / fr.width, fr.width, ubo.numberOfBounces, kHeadBounces,
-                     ubo.numberOfBounces, ubo.hasSkyBox != 0u, &ps, stack + threadIdx.x, &o, cache, &rays);
-        fr.samples[4 * (size_t)fr.n + pix] = f4(o.L, __uint_as_float(o.cache));
-    }
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < count; q += gridDim.x * kBlock)
+        initial_tail_path<kLdsDepth>(sc, fr, ubo, fr.tail + 4 * (size_t)q, cache, stack + threadIdx.x, &rays);
     count_rays(&fr.counters[0], rays, &lds_count);
 }
 
@@ -504,10 +538,8 @@ __device__ __forceinline__ v3 shade_reservoir(const Res& r, v3 x_curr, v3 n_curr
 // ---------------------------------------------------------------------------
 // Temporal reuse pass (HBM bound; no rays)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, PassParams pp,
-                                                  const float4* __restrict__ gpos_hist) {
-    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    if (pix >= fr.n) return;
+__device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t frame, const PassParams& pp,
+                                               const float4* __restrict__ gpos_hist, uint32_t pix) {
     const uint32_t n = fr.n, w = fr.width, h = fr.height;
     const uint32_t x = pix % w, y = pix / w;
     uint32_t rng = init_seed(x, y, w, frame);
@@ -575,6 +607,12 @@ __global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, P
     if (fr.radiance) fr.radiance[pix] = f4(color, 0.0f);
 }
 
+__global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, PassParams pp,
+                                                  const float4* __restrict__ gpos_hist) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix < fr.n) temporal_pixel(fr, frame, pp, gpos_hist, pix);
+}
+
 // ---------------------------------------------------------------------------
 // Spatial reuse pass
 // ---------------------------------------------------------------------------
@@ -593,15 +631,13 @@ __device__ __forceinline__ bool similar(float4 cx, float4 cn, float4 hx, float4 
 // sample's visible point and normal (the origin of the next visibility ray).
 // Per neighbour only the planes that p_hat and the merge weight read are
 // loaded (x_s, n_s, L_out + W, M).
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame,
-                                                                            uint32_t reuse, PassParams pp) {
-    __shared__ int stack[kLdsDepth * kBlock];
-    __shared__ uint32_t lds_count;
-    const TileCoord tc = tile_pixel(fr.width, fr.height);
+template <int kLds>
+__device__ __forceinline__ uint32_t spatial_pixel(const SceneDev& sc, const FrameDev& fr, uint32_t frame,
+                                                  uint32_t reuse, const PassParams& pp, uint32_t x, uint32_t y,
+                                                  int* col) {
     uint32_t rays = 0;
-    if (tc.in_image) {
+    {
         const uint32_t n = fr.n, w = fr.width, h = fr.height;
-        const uint32_t x = tc.x, y = tc.y;
         const uint32_t pix = y * w + x;
         const uint32_t ping = frame % 2u;
         const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
@@ -633,7 +669,6 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
                 }
                 m = 0u + mc;
             }
-            int* col = stack + threadIdx.x;
             for (uint32_t i = 0; i < pp.spatial_samples; ++i) {
                 const float ra = rand01(&rng) * 2.0f * XR_PI;
                 const float rr = sqrtf(rand01(&rng)) * pp.spatial_radius;
@@ -679,7 +714,8 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
                     const RayPre rp = make_ray_pre(o, dn);
                     Hit hh;
                     ++rays;
-                    if (trace_ray<kBlock, true>(sc.nodes, sc.tris, rp, 0.0f, dist - 0.01f, col, &hh)) ph = 0.0f;
+                    if (trace_ray<kBlock, true, kLds>(sc.nodes, sc.tris, rp, 0.0f, dist - 0.01f, col, &hh))
+                        ph = 0.0f;
                 }
                 // MergeReservoir(r, r_neighbor, ph): w = ph * W * M, update
                 // (one draw), m = M0 + M; then m -= M if ph == 0.
@@ -716,7 +752,154 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
             }
         }
     }
+    return rays;
+}
+
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame,
+                                                                            uint32_t reuse, PassParams pp) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image)
+        rays = spatial_pixel<kLdsDepth>(sc, fr, frame, reuse, pp, tc.x, tc.y, stack + threadIdx.x);
     count_rays(&fr.counters[1], rays, &lds_count);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched views (no reference counterpart): several cameras on one scene, each
+// with its own frame buffers; one launch per pass covers every view.  The
+// tiled kernels are instantiated for several tile shapes and LDS stack depths
+// (the autotuner times them on the actual views and keeps the fastest); all
+// give identical results.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t view_blocks(uint32_t w, uint32_t h, uint32_t tw, uint32_t th) {
+    const uint32_t tiles_x = (w + tw - 1) / tw;
+    const uint32_t tiles_y = (h + th - 1) / th;
+    return kXcds * ((tiles_y + kXcds - 1) / kXcds) * tiles_x;
+}
+
+// View of workgroup b of a tiled batched grid; *local = b within the view's grid.
+__device__ __forceinline__ uint32_t batch_view(const BatchDev& bd, uint32_t b, uint32_t tw, uint32_t th,
+                                               uint32_t* local) {
+    uint32_t v = 0;
+    for (; v + 1 < bd.nviews; ++v) {
+        const uint32_t g = view_blocks(bd.views[v].width, bd.views[v].height, tw, th);
+        if (b < g) break;
+        b -= g;
+    }
+    *local = b;
+    return v;
+}
+
+template <int kTw, int kTh, int kLds>
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_initial(SceneDev sc, BatchDev bd) {
+    __shared__ int stack[kLds * kBlock];
+    __shared__ uint32_t lds_count;
+    __shared__ uint32_t lds_base;
+    uint32_t local;
+    const uint32_t v = batch_view(bd, blockIdx.x, kTw, kTh, &local);
+    const FrameDev fr = bd.views[v];
+    const UniformBufferObject& ubo = bd.ubos[v];
+    const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
+    const CacheCfg nocache = {0u, 1u, 0u, 0.0f};
+    uint32_t rays = 0, valid = 0;
+    bool alive = false;
+    PathState ps;
+    PathOut o;
+    if (tc.in_image) {
+        alive = initial_head_pixel<kLds>(sc, fr, ubo, tc.x, tc.y, nocache, stack + threadIdx.x, &ps, &o, &rays);
+        valid = o.valid ? 1u : 0u;
+    }
+    hand_on(alive, bd.tail, bd.tail_count, tc.y * fr.width + tc.x, v, ps, o, &lds_count, &lds_base);
+    count_rays(&bd.counters[0], rays, &lds_count);
+    count_rays(&bd.counters[2], valid, &lds_count);
+}
+
+template <int kLds>
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_tail(SceneDev sc, BatchDev bd) {
+    __shared__ int stack[kLds * kBlock];
+    __shared__ uint32_t lds_count;
+    const uint32_t count = *bd.tail_count;
+    const CacheCfg nocache = {0u, 1u, 0u, 0.0f};
+    uint32_t rays = 0;
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < count; q += gridDim.x * kBlock) {
+        const float4* rec = bd.tail + 4 * (size_t)q;
+        const uint32_t v = __float_as_uint(rec[3].w);
+        initial_tail_path<kLds>(sc, bd.views[v], bd.ubos[v], rec, nocache, stack + threadIdx.x, &rays);
+    }
+    count_rays(&bd.counters[0], rays, &lds_count);
+}
+
+__global__ __launch_bounds__(256) void k_batch_temporal(BatchDev bd, PassParams pp) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    uint32_t v = 0;
+    for (; v < bd.nviews; ++v) {
+        if (p < bd.views[v].n) break;
+        p -= bd.views[v].n;
+    }
+    if (v == bd.nviews) return;
+    const FrameDev fr = bd.views[v];
+    temporal_pixel(fr, bd.ubos[v].currentFrame, pp, fr.gpos, p);
+}
+
+template <int kTw, int kTh, int kLds>
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_spatial(SceneDev sc, BatchDev bd,
+                                                                                  PassParams pp) {
+    __shared__ int stack[kLds * kBlock];
+    __shared__ uint32_t lds_count;
+    uint32_t local;
+    const uint32_t v = batch_view(bd, blockIdx.x, kTw, kTh, &local);
+    const FrameDev fr = bd.views[v];
+    const UniformBufferObject& ubo = bd.ubos[v];
+    const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
+    uint32_t rays = 0;
+    if (tc.in_image)
+        rays = spatial_pixel<kLds>(sc, fr, ubo.currentFrame, ubo.reuse, pp, tc.x, tc.y, stack + threadIdx.x);
+    count_rays(&bd.counters[1], rays, &lds_count);
+}
+
+// Tile shapes x LDS stack depths the autotuner chooses from.
+struct BatchKernels {
+    uint32_t tw, th, lds;
+    void (*initial)(SceneDev, BatchDev);
+    void (*tail)(SceneDev, BatchDev);
+    void (*spatial)(SceneDev, BatchDev, PassParams);
+};
+#define XR_BATCH_CFG(TW, TH, L) \
+    {TW, TH, L, k_batch_initial<TW, TH, L>, k_batch_tail<L>, k_batch_spatial<TW, TH, L>}
+static const BatchKernels kBatchKernels[] = {
+    XR_BATCH_CFG(16, 16, 24), XR_BATCH_CFG(16, 16, 16), XR_BATCH_CFG(16, 16, 32),
+    XR_BATCH_CFG(32, 8, 24),  XR_BATCH_CFG(32, 8, 16),  XR_BATCH_CFG(32, 8, 32),
+    XR_BATCH_CFG(8, 32, 24),  XR_BATCH_CFG(8, 32, 16),  XR_BATCH_CFG(8, 32, 32),
+};
+#undef XR_BATCH_CFG
+
+uint32_t batch_config_count() { return (uint32_t)(sizeof(kBatchKernels) / sizeof(kBatchKernels[0])); }
+
+void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds) {
+    *tw = kBatchKernels[cfg].tw;
+    *th = kBatchKernels[cfg].th;
+    *lds = kBatchKernels[cfg].lds;
+}
+
+// One ReSTIR GI frame of every view (initial head + tail, temporal, spatial).
+// views: host copy of the view table (sizes), bd: its device twin.
+hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                              const PassParams& pp, hipStream_t st) {
+    const BatchKernels& k = kBatchKernels[cfg];
+    uint32_t blocks = 0, pixels = 0;
+    for (uint32_t v = 0; v < bd.nviews; ++v) {
+        blocks += image_grid(views[v].width, views[v].height, k.tw, k.th);
+        pixels += views[v].n;
+    }
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, bd.tail_count, 1u);
+    hipLaunchKernelGGL(k.initial, dim3(blocks), dim3(kBlock), 0, st, sc, bd);
+    hipLaunchKernelGGL(k.tail, dim3(2048), dim3(kBlock), 0, st, sc, bd);
+    hipLaunchKernelGGL(k_batch_temporal, dim3((pixels + 255) / 256), dim3(256), 0, st, bd, pp);
+    hipLaunchKernelGGL(k.spatial, dim3(blocks), dim3(kBlock), 0, st, sc, bd, pp);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

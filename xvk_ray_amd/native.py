@@ -124,6 +124,10 @@ XR_API_SYMBOLS = [
     "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
     "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
     "xr_mlp_fp8_weights", "xr_mlp_params",
+    "xr_batch_create", "xr_batch_render", "xr_batch_frame_time", "xr_batch_set_graph", "xr_batch_config_count",
+    "xr_batch_config_info", "xr_batch_set_config", "xr_batch_config", "xr_batch_autotune", "xr_batch_reset",
+    "xr_batch_counters", "xr_batch_reset_counters", "xr_batch_buffer_bytes", "xr_batch_read_buffer",
+    "xr_batch_destroy",
     "xr_last_error", "xr_global_last_error",
 ]
 XR_HOST_SYMBOLS = [
@@ -186,6 +190,25 @@ def lib() -> C.CDLL:
         L.xr_mlp_params.argtypes = [P, P, I]
         L.xr_mlp_set_weight_format.argtypes = [P, I]
         L.xr_mlp_fp8_weights.argtypes = [P, P, P]
+        L.xr_batch_create.argtypes = [P, U32, P, P, C.POINTER(P)]
+        L.xr_batch_render.argtypes = [P, P, C.POINTER(XrRestirParams)]
+        L.xr_batch_frame_time.argtypes = [P, C.POINTER(C.c_float)]
+        L.xr_batch_set_graph.argtypes = [P, I]
+        L.xr_batch_config_count.argtypes = []
+        L.xr_batch_config_count.restype = U32
+        L.xr_batch_config_info.argtypes = [U32, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]
+        L.xr_batch_set_config.argtypes = [P, U32]
+        L.xr_batch_config.argtypes = [P]
+        L.xr_batch_config.restype = U32
+        L.xr_batch_autotune.argtypes = [P, P, C.POINTER(XrRestirParams), U32, P]
+        L.xr_batch_reset.argtypes = [P]
+        L.xr_batch_counters.argtypes = [P, C.POINTER(C.c_uint64), U32]
+        L.xr_batch_reset_counters.argtypes = [P]
+        L.xr_batch_buffer_bytes.argtypes = [P, U32, I, I]
+        L.xr_batch_buffer_bytes.restype = C.c_size_t
+        L.xr_batch_read_buffer.argtypes = [P, U32, I, P, C.c_size_t, I]
+        L.xr_batch_destroy.argtypes = [P]
+        L.xr_batch_destroy.restype = None
         L.xr_last_error.argtypes = [P]
         L.xr_last_error.restype = C.c_char_p
         L.xr_global_last_error.argtypes = []
@@ -377,6 +400,20 @@ def make_ubo(view, view_prev, fov, width, height, spp=1, total=1, bounces=4, sky
     return u
 
 
+def _typed(out: np.ndarray, which: int, layout: int) -> np.ndarray:
+    """View of raw buffer bytes with the element type of buffer `which`."""
+    if layout == XR_LAYOUT_REFERENCE:
+        if which == XR_BUF_SAMPLES:
+            return out.view(SAMPLE_DTYPE)
+        if which in (XR_BUF_TEMPORAL, XR_BUF_SPATIAL):
+            return out.view(RESERVOIR_DTYPE)
+    if which in (XR_BUF_OUTPUT, XR_BUF_MOTION):
+        return out.view(np.uint32)
+    if which in (XR_BUF_ACCUMULATION, XR_BUF_GBUF_POS, XR_BUF_GBUF_NRM, XR_BUF_GBUF_MATUV, XR_BUF_RADIANCE):
+        return out.view(np.float32).reshape(-1, 4)
+    return out.view(np.float32)
+
+
 class Context:
     """One xr_ctx: scene on the device, frame buffers, acceleration structure."""
 
@@ -420,16 +457,7 @@ class Context:
         n = self.buffer_bytes(which, layout)
         out = np.empty(n, np.uint8)
         self.check(lib().xr_read_buffer(self._h, which, out.ctypes.data, n, layout))
-        if layout == XR_LAYOUT_REFERENCE:
-            if which == XR_BUF_SAMPLES:
-                return out.view(SAMPLE_DTYPE)
-            if which in (XR_BUF_TEMPORAL, XR_BUF_SPATIAL):
-                return out.view(RESERVOIR_DTYPE)
-        if which in (XR_BUF_OUTPUT, XR_BUF_MOTION):
-            return out.view(np.uint32)
-        if which in (XR_BUF_ACCUMULATION, XR_BUF_GBUF_POS, XR_BUF_GBUF_NRM, XR_BUF_GBUF_MATUV, XR_BUF_RADIANCE):
-            return out.view(np.float32).reshape(-1, 4)
-        return out.view(np.float32)
+        return _typed(out, which, layout)
 
     def write(self, which: int, data: np.ndarray, layout: int = XR_LAYOUT_REFERENCE) -> None:
         a = np.ascontiguousarray(data)
@@ -619,3 +647,86 @@ class BorrowedContext(Context):
 
     def close(self):
         self._h = None
+
+
+class Batch:
+    """Views rendered together on the scene of a Context (xr_batch)."""
+
+    def __init__(self, ctx: "Context", sizes):
+        self.ctx = ctx
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        ws = np.array([w for w, _ in self.sizes], np.uint32)
+        hs = np.array([h for _, h in self.sizes], np.uint32)
+        h = C.c_void_p()
+        ctx.check(lib().xr_batch_create(ctx.handle, len(self.sizes), ws.ctypes.data, hs.ctypes.data, C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def config_count() -> int:
+        return int(lib().xr_batch_config_count())
+
+    @staticmethod
+    def config_info(k: int) -> tuple[int, int, int]:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        if lib().xr_batch_config_info(k, C.byref(a), C.byref(b), C.byref(c)):
+            raise XrError(XR_ERR_INVALID_ARGUMENT, lib().xr_global_last_error().decode())
+        return int(a.value), int(b.value), int(c.value)
+
+    def _ubos(self, ubos):
+        assert len(ubos) == len(self.sizes)
+        return (XrUbo * len(ubos))(*ubos)
+
+    def render(self, ubos, params: XrRestirParams | None = None) -> None:
+        arr = self._ubos(ubos)
+        self.ctx.check(lib().xr_batch_render(self._h, arr, C.byref(params) if params is not None else None))
+
+    def frame_time(self) -> float:
+        ms = C.c_float()
+        self.ctx.check(lib().xr_batch_frame_time(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def set_graph(self, enable: bool) -> None:
+        self.ctx.check(lib().xr_batch_set_graph(self._h, int(enable)))
+
+    def set_config(self, k: int) -> None:
+        self.ctx.check(lib().xr_batch_set_config(self._h, k))
+
+    @property
+    def config(self) -> int:
+        return int(lib().xr_batch_config(self._h))
+
+    def autotune(self, ubos, params: XrRestirParams | None = None, frames: int = 4) -> list[float]:
+        arr = self._ubos(ubos)
+        ms = (C.c_float * self.config_count())()
+        self.ctx.check(lib().xr_batch_autotune(self._h, arr, C.byref(params) if params is not None else None, frames,
+                                               ms))
+        return [float(x) for x in ms]
+
+    def reset(self) -> None:
+        self.ctx.check(lib().xr_batch_reset(self._h))
+
+    def counters(self) -> tuple[int, ...]:
+        c = (C.c_uint64 * 5)()
+        self.ctx.check(lib().xr_batch_counters(self._h, c, 5))
+        return tuple(int(c[i]) for i in range(5))
+
+    def reset_counters(self) -> None:
+        self.ctx.check(lib().xr_batch_reset_counters(self._h))
+
+    def read(self, view: int, which: int, layout: int = XR_LAYOUT_REFERENCE) -> np.ndarray:
+        n = int(lib().xr_batch_buffer_bytes(self._h, view, which, layout))
+        out = np.empty(n, np.uint8)
+        self.ctx.check(lib().xr_batch_read_buffer(self._h, view, which, out.ctypes.data, n, layout))
+        return _typed(out, which, layout)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().xr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
