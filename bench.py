@@ -47,8 +47,8 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 25641.8, "temporal": 236240.4, "spatial": 2686174.1}
-PMC_WRITE_KIB = {"initial": 301253.7, "temporal": 215801.4, "spatial": 190428.1}
+PMC_FETCH_KIB = {"initial": 196502.5 + 20108.5, "temporal": 236452.6, "spatial": 1087595.3}
+PMC_WRITE_KIB = {"initial": 818630.5 + 7162.6, "temporal": 215785.7, "spatial": 190619.6}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -68,9 +68,10 @@ def parse():
     ap.add_argument("--conv-spp", type=int, default=1024)
     ap.add_argument("--builder", choices=("lbvh", "ploc"), default="lbvh",
                     help="BVH builder (the build itself is outside the timed region)")
-    ap.add_argument("--workload", choices=("c2", "c1", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c5"), default="c2",
                     help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
-                         "Cornell box; c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain")
+                         "Cornell box; c3: 12 batched views of the 1080p scene (HIP graph, autotuned); "
+                         "c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain")
     return ap.parse_args()
 
 
@@ -307,6 +308,67 @@ def run_c1(args) -> dict:
     return res
 
 
+C3_SIZES = [(256, 256)] * 4 + [(960, 540)] * 4 + [(1920, 1080)] * 2 + [(1280, 720)] * 2
+
+
+def run_c3(args) -> dict:
+    """12 views of uneven size (4x 256x256, 4x 960x540, 2x 1920x1080,
+    2x 1280x720, 8,326,144 pixels) of the 1080p terrain scene, each on its own
+    seeded camera path (orbit seeds 2..13, 64 frames) with its own reservoir
+    buffers, ReSTIR GI as in the headline workload; one launch per pass for
+    all views, the frame replayed as a HIP graph; tile shape and LDS stack
+    depth chosen by the autotuner on the first frame's views."""
+    from xvk_ray_amd import native as N
+
+    frames = 64
+    scene = N.Scene.terrain(**SCENE)
+    ctx = N.Context(scene, 8, 8, builder=builder_id(args))
+    cams = [N.Camera.orbit(2 + v, 8, frames) for v in range(len(C3_SIZES))]
+    views = [[c.view(f) for f in range(frames)] for c in cams]
+    params = N.default_params(visibility=N.XR_VIS_TRACE, **PARAMS)
+
+    def ubos(f):
+        return [N.make_ubo(views[v][f], views[v][f - 1] if f > 0 else views[v][f], FOV, w, h, bounces=BOUNCES,
+                           frame=f + 1) for v, (w, h) in enumerate(C3_SIZES)]
+
+    all_ubos = [ubos(f) for f in range(frames)]
+    batch = N.Batch(ctx, C3_SIZES)
+    t = time.perf_counter()
+    tune = batch.autotune(all_ubos[0], params, frames=4)
+    cfg = batch.config
+    log(f"c3 autotune ({time.perf_counter() - t:.1f} s): {[round(x, 3) for x in tune]} -> config {cfg} "
+        f"{N.Batch.config_info(cfg)}")
+    warm = min(args.warmup, frames)
+    for f in range(warm):
+        batch.render(all_ubos[f], params)
+    ctx.sync()
+    batch.reset_counters()
+    t0 = time.perf_counter()
+    for f in range(warm, frames):
+        batch.render(all_ubos[f], params)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    timed = frames - warm
+    ext, shadow = batch.counters()[:2]
+    ms = wall * 1e3 / timed
+    pixels = sum(w * h for w, h in C3_SIZES)
+    tw, th, lds = N.Batch.config_info(cfg)
+    res = base_result(args, "ms/frame & Mrays/s, 12 batched ReSTIR GI views", round((ext + shadow) / wall / 1e6, 2),
+                      "Mrays/s", round(ms, 4), "f32",
+                      {"workload": "restir_gi_12_views_batched", "views": [f"{w}x{h}" for w, h in C3_SIZES],
+                       "pixels": pixels, "triangles": scene.triangle_count, "max_depth": BOUNCES,
+                       "frames": frames, "graph": True, "parallelism": "single_gpu"})
+    res["steps"], res["warmup"] = timed, warm
+    res.update({"ms_per_frame": round(ms, 4), "ns_per_pixel": round(ms * 1e6 / pixels, 3),
+                "rays_per_frame": (ext + shadow) // timed,
+                "autotune": {"ms_per_config": [round(x, 4) for x in tune],
+                             "configs": [N.Batch.config_info(k) for k in range(N.Batch.config_count())],
+                             "chosen": {"tile": f"{tw}x{th}", "lds_stack_depth": lds}}})
+    batch.close()
+    ctx.close()
+    return res
+
+
 def run_c5(args) -> dict:
     """3840x2160 ReSTIR GI, max depth 6, on the large terrain (2048x2048-vertex
     fBm heightfield + 4,096 icospheres of 1,280 tris + 256 quad lights, seed
@@ -403,6 +465,9 @@ def main():
         return
     if args.workload == "c5":
         print(json.dumps(run_c5(args)))
+        return
+    if args.workload == "c3":
+        print(json.dumps(run_c3(args)))
         return
 
     from xvk_ray_amd import native as N
