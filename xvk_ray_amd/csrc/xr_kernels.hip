@@ -631,127 +631,183 @@ __device__ __forceinline__ bool similar(float4 cx, float4 cn, float4 hx, float4 
 // sample's visible point and normal (the origin of the next visibility ray).
 // Per neighbour only the planes that p_hat and the merge weight read are
 // loaded (x_s, n_s, L_out + W, M).
+//
+// One pixel's pass is a resumable task: spatial_step runs the neighbour loop
+// until a neighbour needs a visibility ray (the ray and the merge it decides
+// are left pending) or the loop ends; the caller traces the ray and applies
+// the merge.  The draws and float operations are those of the loop in
+// Spatial.rgen:64-183, in the same order, whoever drives the task.
+struct SpatialTask {
+    uint32_t pix, x, y, rng, i, sel, m;
+    float ws;
+    float4 s0, s1;  // centre x_view / material, n_view
+    v3 sel_x, sel_n;
+    v3 ro, rdir;    // pending visibility ray
+    float tmax;
+    float pph, pW;  // pending merge: p_hat, W, M, source pixel
+    uint32_t pM, pnidx;
+};
+
+// Loads the centre and merges its reservoir; false when the pixel takes an
+// early out (written here) and needs nothing more.
+__device__ __forceinline__ bool spatial_begin(SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
+                                              uint32_t frame, uint32_t x, uint32_t y) {
+    const uint32_t n = fr.n;
+    const uint32_t pix = y * fr.width + x;
+    t.pix = pix;
+    t.x = x;
+    t.y = y;
+    t.s0 = fr.samples[pix];
+    t.s1 = fr.samples[n + pix];
+    const float4 s3 = fr.samples[3 * n + pix];
+    if (length(xyz(t.s1)) == 0.0f) {
+        fr.output[pix] = 0xFF000000u;
+        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return false;
+    }
+    if (length(xyz(s3)) == 0.0f) {
+        fr.output[pix] = 0xFFFFFFFFu;
+        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return false;
+    }
+    t.rng = init_seed(x, y, fr.width, frame);
+    t.sel = 0xFFFFFFFFu;  // none yet: InitReservoir's sample
+    t.sel_x = mk3(0.f, 0.f, 0.f);
+    t.sel_n = mk3(0.f, 0.f, 0.f);
+    t.ws = 0.0f;
+    t.i = 0;
+    // MergeReservoir(r, centerRes, |L|)
+    const float4 c4 = rd[4 * n + pix], c5 = rd[5 * n + pix];
+    const uint32_t mc = __float_as_uint(c5.w);
+    const float wc = length(xyz(c4)) * c4.w * (float)mc;
+    t.ws = t.ws + wc;
+    if (rand01(&t.rng) < wc / maxf(1e-10f, t.ws)) {
+        t.sel = pix;
+        t.sel_x = xyz(rd[pix]);
+        t.sel_n = xyz(rd[n + pix]);
+    }
+    t.m = 0u + mc;
+    return true;
+}
+
+// MergeReservoir(r, r_neighbor, ph): w = ph * W * M, update (one draw),
+// m = M0 + M; then m -= M if ph == 0.
+__device__ __forceinline__ void spatial_merge(SpatialTask& t, const float4* __restrict__ rd, uint32_t n, float ph,
+                                              float W, uint32_t M, uint32_t nidx) {
+    const uint32_t m0 = t.m;
+    const float wn = ph * W * (float)M;
+    t.ws = t.ws + wn;
+    if (rand01(&t.rng) < wn / maxf(1e-10f, t.ws)) {
+        t.sel = nidx;
+        t.sel_x = xyz(rd[nidx]);
+        t.sel_n = xyz(rd[n + nidx]);
+    }
+    t.m = m0 + M;
+    if (ph == 0.0f) t.m = t.m - M;
+}
+
+// Runs neighbours until one needs a visibility ray (returns true, ray and
+// merge pending) or the loop ends (returns false).
+__device__ __forceinline__ bool spatial_step(SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
+                                             const PassParams& pp) {
+    const uint32_t n = fr.n, w = fr.width, h = fr.height;
+    while (t.i < pp.spatial_samples) {
+        ++t.i;
+        const float ra = rand01(&t.rng) * 2.0f * XR_PI;
+        const float rr = sqrtf(rand01(&t.rng)) * pp.spatial_radius;
+        float sa, ca;
+        sincos_xr(ra, &sa, &ca);
+        int nx = (int)t.x + (int)floorf(ca * rr);
+        int ny = (int)t.y + (int)floorf(sa * rr);
+        nx = nx < 0 ? 0 : (nx > (int)w - 1 ? (int)w - 1 : nx);
+        ny = ny < 0 ? 0 : (ny > (int)h - 1 ? (int)h - 1 : ny);
+        const uint32_t nidx = (uint32_t)ny * w + (uint32_t)nx;
+        const float4 q0 = fr.samples[nidx];
+        const float4 q1 = fr.samples[n + nidx];
+        if (length(xyz(q1)) == 0.0f || !similar(t.s0, t.s1, q0, q1)) continue;
+        const float4 rn2 = rd[2 * n + nidx], rn3 = rd[3 * n + nidx], rn4 = rd[4 * n + nidx];
+        const uint32_t mn = __float_as_uint(rd[5 * n + nidx].w);
+        const v3 xs = xyz(rn2);
+        const v3 ns = xyz(rn3);
+        const v3 vo = sub(xyz(q0), xs);
+        const float d2o = dot(vo, vo);
+        const v3 vc = sub(xyz(t.s0), xs);
+        const float d2c = dot(vc, vc);
+        float jac = 1.0f;
+        if (d2o > 1e-6f && d2c > 1e-6f) {
+            const v3 dir_o = muls(vo, 1.0f / sqrtf(d2o));
+            const v3 dir_c = muls(vc, 1.0f / sqrtf(d2c));
+            const float co = maxf(0.0f, dot(ns, dir_o));
+            const float cc = maxf(0.0f, dot(ns, dir_c));
+            if (co > 1e-6f) jac = (cc * d2o) / (co * d2c);
+        }
+        jac = clampf(jac, 0.1f, 10.0f);
+        const float ph = length(xyz(rn4)) * jac;
+        // CheckVisibility (Spatial.rgen:44-62) from the currently selected
+        // sample's visible point, any-hit.  Its only effect is to zero ph,
+        // so it is traced only when ph is not already zero (same result,
+        // fewer rays: samples that found no light carry L_out = 0).
+        if (pp.visibility && ph != 0.0f) {
+            const v3 dv = sub(xs, t.sel_x);
+            const float dist = length(dv);
+            t.ro = add(t.sel_x, muls(t.sel_n, 0.01f));
+            t.rdir = normalize(dv);
+            t.tmax = dist - 0.01f;
+            t.pph = ph;
+            t.pW = rn4.w;
+            t.pM = mn;
+            t.pnidx = nidx;
+            return true;
+        }
+        spatial_merge(t, rd, n, ph, rn4.w, mn, nidx);
+    }
+    return false;
+}
+
+// After the loop: M cap, W, the selected sample copied in, shading, writes.
+__device__ __forceinline__ void spatial_finish(const SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
+                                               float4* __restrict__ wr, uint32_t reuse, const PassParams& pp) {
+    const uint32_t n = fr.n;
+    Res r;
+    res_init(&r);
+    if (t.sel != 0xFFFFFFFFu) {
+        float4 sp[kPlanes];
+        load_planes(rd, n, t.sel, sp);
+        take_sample(&r, sp);
+    }
+    set_wsum(&r, t.ws);
+    set_m(&r, t.m);
+    if (r_m(r) > pp.spatial_m_cap) set_m(&r, pp.spatial_m_cap);
+    const float phf = length(xyz(r.p[4]));
+    if (phf > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * phf));
+    else set_W(&r, 0.0f);
+    v3 color = mk3(0.f, 0.f, 0.f);
+    if (r_W(r) > 0.0f) color = shade_reservoir(r, xyz(t.s0), xyz(t.s1));
+    store_planes(wr, n, t.pix, r.p);
+    if (reuse) {
+        fr.output[t.pix] = resolve_bgra8(color, 1.0f);
+        if (fr.radiance) fr.radiance[t.pix] = f4(color, 0.0f);
+    }
+}
+
 template <int kLds>
 __device__ __forceinline__ uint32_t spatial_pixel(const SceneDev& sc, const FrameDev& fr, uint32_t frame,
                                                   uint32_t reuse, const PassParams& pp, uint32_t x, uint32_t y,
                                                   int* col) {
+    const uint32_t n = fr.n, ping = frame % 2u;
+    const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
+    float4* wr = fr.spatial + (size_t)(1u - ping) * kPlanes * n;
+    SpatialTask t;
+    if (!spatial_begin(t, fr, rd, frame, x, y)) return 0u;
     uint32_t rays = 0;
-    {
-        const uint32_t n = fr.n, w = fr.width, h = fr.height;
-        const uint32_t pix = y * w + x;
-        const uint32_t ping = frame % 2u;
-        const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
-        float4* wr = fr.spatial + (size_t)(1u - ping) * kPlanes * n;
-        const float4 s0 = fr.samples[pix];
-        const float4 s1 = fr.samples[n + pix];
-        const float4 s3 = fr.samples[3 * n + pix];
-        if (length(xyz(s1)) == 0.0f) {
-            fr.output[pix] = 0xFF000000u;
-            if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else if (length(xyz(s3)) == 0.0f) {
-            fr.output[pix] = 0xFFFFFFFFu;
-            if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-            uint32_t rng = init_seed(x, y, w, frame);
-            uint32_t sel = 0xFFFFFFFFu;  // none yet: InitReservoir's sample
-            float4 sel0 = make_float4(0.f, 0.f, 0.f, 0.f), sel1 = sel0;
-            float ws = 0.0f;
-            uint32_t m = 0u;
-            {  // MergeReservoir(r, centerRes, |L|)
-                const float4 c4 = rd[4 * n + pix], c5 = rd[5 * n + pix];
-                const uint32_t mc = __float_as_uint(c5.w);
-                const float wc = length(xyz(c4)) * c4.w * (float)mc;
-                ws = ws + wc;
-                if (rand01(&rng) < wc / maxf(1e-10f, ws)) {
-                    sel = pix;
-                    sel0 = rd[pix];
-                    sel1 = rd[n + pix];
-                }
-                m = 0u + mc;
-            }
-            for (uint32_t i = 0; i < pp.spatial_samples; ++i) {
-                const float ra = rand01(&rng) * 2.0f * XR_PI;
-                const float rr = sqrtf(rand01(&rng)) * pp.spatial_radius;
-                float sa, ca;
-                sincos_xr(ra, &sa, &ca);
-                int nx = (int)x + (int)floorf(ca * rr);
-                int ny = (int)y + (int)floorf(sa * rr);
-                nx = nx < 0 ? 0 : (nx > (int)w - 1 ? (int)w - 1 : nx);
-                ny = ny < 0 ? 0 : (ny > (int)h - 1 ? (int)h - 1 : ny);
-                const uint32_t nidx = (uint32_t)ny * w + (uint32_t)nx;
-                const float4 q0 = fr.samples[nidx];
-                const float4 q1 = fr.samples[n + nidx];
-                if (length(xyz(q1)) == 0.0f || !similar(s0, s1, q0, q1)) continue;
-                const float4 rn2 = rd[2 * n + nidx], rn3 = rd[3 * n + nidx], rn4 = rd[4 * n + nidx];
-                const uint32_t mn = __float_as_uint(rd[5 * n + nidx].w);
-                const v3 xs = xyz(rn2);
-                const v3 ns = xyz(rn3);
-                const v3 vo = sub(xyz(q0), xs);
-                const float d2o = dot(vo, vo);
-                const v3 vc = sub(xyz(s0), xs);
-                const float d2c = dot(vc, vc);
-                float jac = 1.0f;
-                if (d2o > 1e-6f && d2c > 1e-6f) {
-                    const v3 dir_o = muls(vo, 1.0f / sqrtf(d2o));
-                    const v3 dir_c = muls(vc, 1.0f / sqrtf(d2c));
-                    const float co = maxf(0.0f, dot(ns, dir_o));
-                    const float cc = maxf(0.0f, dot(ns, dir_c));
-                    if (co > 1e-6f) jac = (cc * d2o) / (co * d2c);
-                }
-                jac = clampf(jac, 0.1f, 10.0f);
-                float ph = length(xyz(rn4)) * jac;
-                // CheckVisibility (Spatial.rgen:44-62) from the currently
-                // selected sample's visible point, any-hit.  Its only effect
-                // is to zero ph, so it is traced only when ph is not already
-                // zero (same result, fewer rays: samples that found no light
-                // carry L_out = 0).
-                if (pp.visibility && ph != 0.0f) {
-                    const v3 from = xyz(sel0);
-                    const v3 dv = sub(xs, from);
-                    const float dist = length(dv);
-                    const v3 o = add(from, muls(xyz(sel1), 0.01f));
-                    const v3 dn = normalize(dv);
-                    const RayPre rp = make_ray_pre(o, dn);
-                    Hit hh;
-                    ++rays;
-                    if (trace_ray<kBlock, true, kLds>(sc.nodes, sc.tris, rp, 0.0f, dist - 0.01f, col, &hh))
-                        ph = 0.0f;
-                }
-                // MergeReservoir(r, r_neighbor, ph): w = ph * W * M, update
-                // (one draw), m = M0 + M; then m -= M if ph == 0.
-                const uint32_t m0 = m;
-                const float wn = ph * rn4.w * (float)mn;
-                ws = ws + wn;
-                if (rand01(&rng) < wn / maxf(1e-10f, ws)) {
-                    sel = nidx;
-                    sel0 = rd[nidx];
-                    sel1 = rd[n + nidx];
-                }
-                m = m0 + mn;
-                if (ph == 0.0f) m = m - mn;
-            }
-            Res r;
-            res_init(&r);
-            if (sel != 0xFFFFFFFFu) {
-                float4 sp[kPlanes];
-                load_planes(rd, n, sel, sp);
-                take_sample(&r, sp);
-            }
-            set_wsum(&r, ws);
-            set_m(&r, m);
-            if (r_m(r) > pp.spatial_m_cap) set_m(&r, pp.spatial_m_cap);
-            const float phf = length(xyz(r.p[4]));
-            if (phf > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * phf));
-            else set_W(&r, 0.0f);
-            v3 color = mk3(0.f, 0.f, 0.f);
-            if (r_W(r) > 0.0f) color = shade_reservoir(r, xyz(s0), xyz(s1));
-            store_planes(wr, n, pix, r.p);
-            if (reuse) {
-                fr.output[pix] = resolve_bgra8(color, 1.0f);
-                if (fr.radiance) fr.radiance[pix] = f4(color, 0.0f);
-            }
-        }
+    while (spatial_step(t, fr, rd, pp)) {
+        const RayPre rp = make_ray_pre(t.ro, t.rdir);
+        Hit hh;
+        ++rays;
+        const bool blocked = trace_ray<kBlock, true, kLds>(sc.nodes, sc.tris, rp, 0.0f, t.tmax, col, &hh);
+        spatial_merge(t, rd, n, blocked ? 0.0f : t.pph, t.pW, t.pM, t.pnidx);
     }
+    spatial_finish(t, fr, rd, wr, reuse, pp);
     return rays;
 }
 
