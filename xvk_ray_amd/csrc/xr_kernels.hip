@@ -72,6 +72,20 @@ __device__ __forceinline__ void wave_append(uint32_t* count, uint32_t* list, boo
     if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Slot in a device list for every thread of the workgroup that wants one: one
+// LDS atomic per thread, one global atomic per workgroup.  Called by every
+// thread of the workgroup.
+__device__ __forceinline__ uint32_t block_slot(bool want, uint32_t* count, uint32_t* lds_count,
+                                               uint32_t* lds_base) {
+    if (threadIdx.x == 0) *lds_count = 0;
+    __syncthreads();
+    const uint32_t local = want ? atomicAdd(lds_count, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_base = *lds_count ? atomicAdd(count, *lds_count) : 0u;
+    __syncthreads();
+    return *lds_base + local;
+}
+
 // One LDS atomic per lane, one global atomic per workgroup.
 __device__ __forceinline__ void count_rays(unsigned long long* counter, uint32_t n, uint32_t* lds_count) {
     if (threadIdx.x == 0) *lds_count = 0;
@@ -319,13 +333,8 @@ __device__ __forceinline__ bool initial_head_pixel(const SceneDev& sc, const Fra
 __device__ __forceinline__ void hand_on(bool alive, float4* tail, uint32_t* tail_count, uint32_t pix, uint32_t view,
                                         const PathState& ps, const PathOut& o, uint32_t* lds_count,
                                         uint32_t* lds_base) {
-    if (threadIdx.x == 0) *lds_count = 0;
-    __syncthreads();
-    const uint32_t local = alive ? atomicAdd(lds_count, 1u) : 0u;
-    __syncthreads();
-    if (threadIdx.x == 0) *lds_base = *lds_count ? atomicAdd(tail_count, *lds_count) : 0u;
-    __syncthreads();
-    if (alive) store_path(tail + 4 * (size_t)(*lds_base + local), pix, view, ps, o);
+    const uint32_t q = block_slot(alive, tail_count, lds_count, lds_base);
+    if (alive) store_path(tail + 4 * (size_t)q, pix, view, ps, o);
 }
 
 __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr,
@@ -821,7 +830,6 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(Scene
         rays = spatial_pixel<kLdsDepth>(sc, fr, frame, reuse, pp, tc.x, tc.y, stack + threadIdx.x);
     count_rays(&fr.counters[1], rays, &lds_count);
 }
-
 
 // ---------------------------------------------------------------------------
 // Batched views (no reference counterpart): several cameras on one scene, each
