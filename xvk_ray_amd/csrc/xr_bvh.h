@@ -130,4 +130,17 @@ XR_HD float slab(const RayPre& r, float lox, float hix, float loy, float hiy, fl
     return tn <= tf ? tn : __builtin_huge_valf();
 }
 
+// The same test returning the hit flag, with the entry distance in *tn_out.
+XR_HD bool slab_hit(const RayPre& r, float lox, float hix, float loy, float hiy, float loz, float hiz, float tmin,
+                    float tmax, float* tn_out) {
+    float t0x = (lox - r.org.x) * r.idir.x, t1x = (hix - r.org.x) * r.idir.x;
+    float t0y = (loy - r.org.y) * r.idir.y, t1y = (hiy - r.org.y) * r.idir.y;
+    float t0z = (loz - r.org.z) * r.idir.z, t1z = (hiz - r.org.z) * r.idir.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    tf = fminf(tf * 1.0000003576f, tmax);
+    *tn_out = tn;
+    return tn <= tf;
+}
+
 }  // namespace xr

@@ -78,6 +78,17 @@ constexpr uint32_t kMaxNodeVisits = 1u << 24;
 // latency-bound and the extra waves win (5.00 -> 4.63 ms per 1080p frame).
 #define XR_TRAVERSAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
 
+// Top-of-stack read.  Written as "sp < kLds ? lds[..] : spill[..]" the
+// compiler merges the two loads into one flat load through a selected
+// pointer, which waits on both the vector-memory and the LDS counters in
+// every iteration; the volatile private read (levels past kLds only, rare)
+// cannot be merged, so the common case stays a plain LDS read.
+template <int kBlock, int kLds>
+__device__ __forceinline__ int stack_top(const int* __restrict__ lds_col, const int* spill, int sp) {
+    if (sp < kLds) return lds_col[sp * kBlock];
+    return static_cast<const volatile int*>(spill)[sp - kLds];
+}
+
 // Triangles of leaf `code` against the ray (closest-hit rule above, or the
 // first hit for kAnyHit); true when an any-hit ray is done.
 template <bool kAnyHit>
@@ -125,16 +136,17 @@ template <int kBlock, bool kAnyHit, int kLds>
 __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
                                              const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
                                              Hit* hit) {
-    constexpr int kDone = 0x7FFFFFFF;  // what a pop of the empty stack returns
-    int spill[kMaxTraversalDepth - kLds];
-    int sp = 0;
+    // The bottom stack entry is a sentinel: popping it ends the ray.
+    constexpr int kDone = 0x7FFFFFFF;
+    int spill[kMaxTraversalDepth + 1 - kLds];
+    lds_col[0] = kDone;
+    int sp = 1;
     int node = 0;  // >= 0: inner node, < 0: leaf code, kDone: finished
     int leaf = 0;  // parked leaf code, or 0
     float best_t = tmax;
     uint32_t best_id = 0xFFFFFFFFu;
     uint32_t best_idx = 0xFFFFFFFFu;
     float best_u = 0.0f, best_v = 0.0f;
-    const float kInf = __builtin_huge_valf();
     // A well-formed tree visits each node at most once per ray; the cap only
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
@@ -146,36 +158,30 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
                 break;
             }
             const float4 na = nodes[node].a;
-            const float4 nb = nodes[node].b;
             const float4 nc = nodes[node].c;
+            const float4 nb = nodes[node].b;
             const int4 nd = nodes[node].d;
             const float tb = best_t * 1.0000003576f;
-            const float e0 = slab(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb);
-            const float e1 = slab(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb);
-            const bool h0 = e0 != kInf, h1 = e1 != kInf;
+            float e0, e1;
+            const bool h0 = slab_hit(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb, &e0);
+            const bool h1 = slab_hit(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb, &e1);
+            // nearer hit child next (the far one is pushed when both are hit)
+            const bool first1 = h1 && (!h0 || e1 < e0);
+            int next = first1 ? nd.y : nd.x;
             if (h0 && h1) {
-                int near = nd.x, far = nd.y;
-                if (e1 < e0) { near = nd.y; far = nd.x; }
+                const int far = first1 ? nd.x : nd.y;
                 if (sp < kLds) lds_col[sp * kBlock] = far;
                 else spill[sp - kLds] = far;
                 ++sp;
-                node = near;
-            } else if (h0 || h1) {
-                node = h0 ? nd.x : nd.y;
-            } else if (sp == 0) {
-                node = kDone;
-            } else {
+            } else if (!h0 && !h1) {
                 --sp;
-                node = sp < kLds ? lds_col[sp * kBlock] : spill[sp - kLds];
+                next = stack_top<kBlock, kLds>(lds_col, spill, sp);
             }
+            node = next;
             if (node < 0 && leaf == 0) {  // park the first leaf, keep descending
                 leaf = node;
-                if (sp == 0) {
-                    node = kDone;
-                } else {
-                    --sp;
-                    node = sp < kLds ? lds_col[sp * kBlock] : spill[sp - kLds];
-                }
+                --sp;
+                node = stack_top<kBlock, kLds>(lds_col, spill, sp);
             }
             if (__ballot(leaf == 0) == 0ull) break;  // every active lane holds a leaf
         }
@@ -185,12 +191,8 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
             leaf = 0;
             if (node < 0) {  // the current node is a leaf as well: take it
                 leaf = node;
-                if (sp == 0) {
-                    node = kDone;
-                } else {
-                    --sp;
-                    node = sp < kLds ? lds_col[sp * kBlock] : spill[sp - kLds];
-                }
+                --sp;
+                node = stack_top<kBlock, kLds>(lds_col, spill, sp);
             }
         }
     }
@@ -203,75 +205,56 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
     return best_idx != 0xFFFFFFFFu;
 }
 // Single loop: a lane intersects a leaf child's triangles as soon as it
-// tests the parent node.
+// tests the parent node (both leaf children, when both are hit, in one
+// triangle loop), then descends to the nearer hit inner child.
 template <int kBlock, bool kAnyHit, int kLds>
 __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
-                                          const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
-                                          Hit* hit) {
-    int spill[kMaxTraversalDepth - kLds];
-    int sp = 0;
+                                                 const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
+                                                 Hit* hit) {
+    // The bottom stack entry is a sentinel: popping it ends the ray.
+    constexpr int kDone = 0x7FFFFFFF;
+    int spill[kMaxTraversalDepth + 1 - kLds];
+    lds_col[0] = kDone;
+    int sp = 1;
     int node = 0;
     float best_t = tmax;
     uint32_t best_id = 0xFFFFFFFFu;
     uint32_t best_idx = 0xFFFFFFFFu;
     float best_u = 0.0f, best_v = 0.0f;
-    const float kInf = __builtin_huge_valf();
     // A well-formed tree visits each node at most once per ray; the cap only
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
-    while (++guard < kMaxNodeVisits) {
+    while (node != kDone && ++guard < kMaxNodeVisits) {
         const float4 na = nodes[node].a;
-        const float4 nb = nodes[node].b;
         const float4 nc = nodes[node].c;
+        const float4 nb = nodes[node].b;
         const int4 nd = nodes[node].d;
         const float tb = best_t * 1.0000003576f;
-        float e0 = slab(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb);
-        float e1 = slab(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb);
-        int c0 = nd.x, c1 = nd.y;
-        bool h0 = e0 != kInf, h1 = e1 != kInf;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const bool hs = side == 0 ? h0 : h1;
-            const int cs = side == 0 ? c0 : c1;
-            if (hs && cs < 0) {
-                const int code = ~cs;
-                const int first = code >> 4;
-                const int cnt = (code & 15) + 1;
-                for (int k = 0; k < cnt; ++k) {
-                    const BvhTri tr = tris[first + k];
-                    float t, u, v;
-                    if (intersect_tri(r, mk3(tr.v0.x, tr.v0.y, tr.v0.z), mk3(tr.v1.x, tr.v1.y, tr.v1.z),
-                                      mk3(tr.v2.x, tr.v2.y, tr.v2.z), tmin, best_t, &t, &u, &v)) {
-                        const uint32_t id = as_uint(tr.v0.w);
-                        if (kAnyHit) {
-                            hit->t = t; hit->u = u; hit->v = v; hit->id = id; hit->idx = first + k;
-                            return true;
-                        }
-                        // intersect_tri guarantees t <= best_t, so this is
-                        // "t < best_t, or t == best_t and a smaller id".
-                        if (t < best_t || id < best_id) {
-                            best_t = t; best_id = id; best_idx = first + k; best_u = u; best_v = v;
-                        }
-                    }
-                }
-                if (side == 0) h0 = false; else h1 = false;
-            }
+        float e0, e1;
+        const bool h0 = slab_hit(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb, &e0);
+        const bool h1 = slab_hit(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb, &e1);
+        const bool l0 = h0 && nd.x < 0, l1 = h1 && nd.y < 0;
+        int la = l0 ? nd.x : (l1 ? nd.y : 0);
+        int lb = (l0 && l1) ? nd.y : 0;
+        while (la < 0) {
+            if (leaf_hits<kAnyHit>(tris, r, tmin, la, &best_t, &best_id, &best_idx, &best_u, &best_v, hit))
+                return true;
+            la = lb;
+            lb = 0;
         }
-        if (h0 && h1) {
-            if (e1 < e0) { int tmp = c0; c0 = c1; c1 = tmp; }
-            if (sp < kLds) lds_col[sp * kBlock] = c1;
-            else spill[sp - kLds] = c1;
+        const bool i0 = h0 && nd.x >= 0, i1 = h1 && nd.y >= 0;
+        const bool first1 = i1 && (!i0 || e1 < e0);
+        int next = first1 ? nd.y : nd.x;
+        if (i0 && i1) {
+            const int far = first1 ? nd.x : nd.y;
+            if (sp < kLds) lds_col[sp * kBlock] = far;
+            else spill[sp - kLds] = far;
             ++sp;
-            node = c0;
-        } else if (h0) {
-            node = c0;
-        } else if (h1) {
-            node = c1;
-        } else {
-            if (sp == 0) break;
+        } else if (!i0 && !i1) {
             --sp;
-            node = sp < kLds ? lds_col[sp * kBlock] : spill[sp - kLds];
+            next = stack_top<kBlock, kLds>(lds_col, spill, sp);
         }
+        node = next;
     }
     if (kAnyHit) return false;
     hit->t = best_idx == 0xFFFFFFFFu ? -1.0f : best_t;
