@@ -78,11 +78,13 @@ XR_HD RayPre make_ray_pre(v3 org, v3 dir) {
     return p;
 }
 
-// Watertight ray/triangle test.  Returns true and (t, u, v) when the ray hits
-// the triangle with tmin <= t <= tmax.  Same operation sequence on host and
-// device (no contraction), so both produce identical bits.
-XR_HD bool intersect_tri(const RayPre& r, v3 p0, v3 p1, v3 p2, float tmin, float tmax,
-                         float* t_out, float* u_out, float* v_out) {
+// Watertight ray/triangle test.  Returns true and t when the ray hits the
+// triangle with tmin <= t <= tmax, plus the scaled barycentrics V, W and det:
+// the hit attributes are u = V / det, v = W / det, divided out by the caller
+// for the one hit it keeps.  Same operation sequence on host and device (no
+// contraction), so both produce identical bits.
+XR_HD bool intersect_tri(const RayPre& r, v3 p0, v3 p1, v3 p2, float tmin, float tmax, float* t_out,
+                           float* V_out, float* W_out, float* det_out) {
     v3 A = sub(p0, r.org), B = sub(p1, r.org), C = sub(p2, r.org);
     float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
     float Ax = fmaf(-r.sx, Akz, comp(A, r.kx));
@@ -111,8 +113,9 @@ XR_HD bool intersect_tri(const RayPre& r, v3 p0, v3 p1, v3 p2, float tmin, float
     float t = T / det;
     if (!(t >= tmin && t <= tmax)) return false;
     *t_out = t;
-    *u_out = V / det;
-    *v_out = W / det;
+    *V_out = V;
+    *W_out = W;
+    *det_out = det;
     return true;
 }
 

@@ -90,28 +90,29 @@ __device__ __forceinline__ int stack_top(const int* __restrict__ lds_col, const 
 }
 
 // Triangles of leaf `code` against the ray (closest-hit rule above, or the
-// first hit for kAnyHit); true when an any-hit ray is done.
+// first hit for kAnyHit); true when an any-hit ray is done.  The kept hit's
+// barycentrics are held as V, W and det (u = V / det, v = W / det).
 template <bool kAnyHit>
 __device__ __forceinline__ bool leaf_hits(const BvhTri* __restrict__ tris, const RayPre& r, float tmin, int code,
-                                          float* best_t, uint32_t* best_id, uint32_t* best_idx, float* best_u,
-                                          float* best_v, Hit* hit) {
+                                          float* best_t, uint32_t* best_id, uint32_t* best_idx, float* best_V,
+                                          float* best_W, float* best_det, Hit* hit) {
     const int c = ~code;
     const int first = c >> 4;
     const int cnt = (c & 15) + 1;
     for (int k = 0; k < cnt; ++k) {
         const BvhTri tr = tris[first + k];
-        float t, u, v;
+        float t, V, W, det;
         if (intersect_tri(r, mk3(tr.v0.x, tr.v0.y, tr.v0.z), mk3(tr.v1.x, tr.v1.y, tr.v1.z),
-                          mk3(tr.v2.x, tr.v2.y, tr.v2.z), tmin, *best_t, &t, &u, &v)) {
+                            mk3(tr.v2.x, tr.v2.y, tr.v2.z), tmin, *best_t, &t, &V, &W, &det)) {
             const uint32_t id = as_uint(tr.v0.w);
             if (kAnyHit) {
-                hit->t = t; hit->u = u; hit->v = v; hit->id = id; hit->idx = first + k;
+                hit->t = t; hit->u = V / det; hit->v = W / det; hit->id = id; hit->idx = first + k;
                 return true;
             }
             // intersect_tri guarantees t <= best_t, so this is
             // "t < best_t, or t == best_t and a smaller id".
             if (t < *best_t || id < *best_id) {
-                *best_t = t; *best_id = id; *best_idx = first + k; *best_u = u; *best_v = v;
+                *best_t = t; *best_id = id; *best_idx = first + k; *best_V = V; *best_W = W; *best_det = det;
             }
         }
     }
@@ -146,7 +147,7 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
     float best_t = tmax;
     uint32_t best_id = 0xFFFFFFFFu;
     uint32_t best_idx = 0xFFFFFFFFu;
-    float best_u = 0.0f, best_v = 0.0f;
+    float best_V = 0.0f, best_W = 0.0f, best_det = 1.0f;
     // A well-formed tree visits each node at most once per ray; the cap only
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
@@ -186,7 +187,7 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
             if (__ballot(leaf == 0) == 0ull) break;  // every active lane holds a leaf
         }
         while (leaf < 0) {
-            if (leaf_hits<kAnyHit>(tris, r, tmin, leaf, &best_t, &best_id, &best_idx, &best_u, &best_v, hit))
+            if (leaf_hits<kAnyHit>(tris, r, tmin, leaf, &best_t, &best_id, &best_idx, &best_V, &best_W, &best_det, hit))
                 return true;
             leaf = 0;
             if (node < 0) {  // the current node is a leaf as well: take it
@@ -198,8 +199,8 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
     }
     if (kAnyHit) return false;
     hit->t = best_idx == 0xFFFFFFFFu ? -1.0f : best_t;
-    hit->u = best_u;
-    hit->v = best_v;
+    hit->u = best_V / best_det;
+    hit->v = best_W / best_det;
     hit->id = best_id;
     hit->idx = best_idx;
     return best_idx != 0xFFFFFFFFu;
@@ -220,7 +221,7 @@ __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nod
     float best_t = tmax;
     uint32_t best_id = 0xFFFFFFFFu;
     uint32_t best_idx = 0xFFFFFFFFu;
-    float best_u = 0.0f, best_v = 0.0f;
+    float best_V = 0.0f, best_W = 0.0f, best_det = 1.0f;
     // A well-formed tree visits each node at most once per ray; the cap only
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
@@ -234,10 +235,11 @@ __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nod
         const bool h0 = slab_hit(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb, &e0);
         const bool h1 = slab_hit(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb, &e1);
         const bool l0 = h0 && nd.x < 0, l1 = h1 && nd.y < 0;
+        // (taking the nearer leaf first when both are hit measured slower)
         int la = l0 ? nd.x : (l1 ? nd.y : 0);
         int lb = (l0 && l1) ? nd.y : 0;
         while (la < 0) {
-            if (leaf_hits<kAnyHit>(tris, r, tmin, la, &best_t, &best_id, &best_idx, &best_u, &best_v, hit))
+            if (leaf_hits<kAnyHit>(tris, r, tmin, la, &best_t, &best_id, &best_idx, &best_V, &best_W, &best_det, hit))
                 return true;
             la = lb;
             lb = 0;
@@ -258,8 +260,8 @@ __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nod
     }
     if (kAnyHit) return false;
     hit->t = best_idx == 0xFFFFFFFFu ? -1.0f : best_t;
-    hit->u = best_u;
-    hit->v = best_v;
+    hit->u = best_V / best_det;
+    hit->v = best_W / best_det;
     hit->id = best_id;
     hit->idx = best_idx;
     return best_idx != 0xFFFFFFFFu;
