@@ -47,8 +47,8 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 196502.5 + 20108.5, "temporal": 236452.6, "spatial": 1087595.3}
-PMC_WRITE_KIB = {"initial": 818630.5 + 7162.6, "temporal": 215785.7, "spatial": 190619.6}
+PMC_FETCH_KIB = {"initial": 215319.1 + 20664.2, "temporal": 236352.4, "spatial": 730544.6}
+PMC_WRITE_KIB = {"initial": 833570.4 + 8196.5, "temporal": 215785.8, "spatial": 189815.7}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 

@@ -6,8 +6,8 @@
 //   k_plain           rgen.hlsl:31-151 (+ rhit.hlsl, sampling.hlsl)
 //   k_trace           TraceRay on a caller-supplied ray batch
 //
-// Image kernels run 256-lane workgroups on 16x16 pixel tiles; each wave64
-// covers a 16x4 strip.  Tiles are dealt to the eight XCDs in contiguous
+// Image kernels run 256-lane workgroups on 32x8 pixel tiles; each wave64
+// covers a 32x2 strip.  Tiles are dealt to the eight XCDs in contiguous
 // bands (the hardware hands workgroup i to XCD i % 8), so each XCD's L2 holds
 // one band of the image, its G-buffer/reservoir neighbourhoods and the BVH
 // nodes its rays touch.
@@ -16,8 +16,13 @@
 
 namespace xr {
 
-constexpr int kTile = 16;
-constexpr int kBlock = kTile * kTile;
+// Workgroups of 256 lanes cover 32 x 8 pixel tiles: a row of a tile reads
+// 128 contiguous bytes of each float4 plane (a whole cache line; 16 x 16 tiles
+// read half lines), which took the spatial pass from 1.91 to 1.85 ms and the
+// initial pass from 1.47 to 1.46 ms on the 1080p frame (64 x 4 was slower).
+constexpr int kTileW = 32;
+constexpr int kTileH = 8;
+constexpr int kBlock = kTileW * kTileH;
 constexpr int kXcds = 8;
 
 struct TileCoord {
@@ -49,7 +54,7 @@ __device__ __forceinline__ TileCoord tile_coord(uint32_t b, uint32_t width, uint
     return tc;
 }
 __device__ __forceinline__ TileCoord tile_pixel(uint32_t width, uint32_t height) {
-    return tile_coord<kTile, kTile>(blockIdx.x, width, height);
+    return tile_coord<kTileW, kTileH>(blockIdx.x, width, height);
 }
 
 uint32_t image_grid(uint32_t width, uint32_t height, uint32_t tw, uint32_t th) {
@@ -57,7 +62,7 @@ uint32_t image_grid(uint32_t width, uint32_t height, uint32_t tw, uint32_t th) {
     const uint32_t tiles_y = (height + th - 1) / th;
     return kXcds * ((tiles_y + kXcds - 1) / kXcds) * tiles_x;
 }
-uint32_t image_grid(uint32_t width, uint32_t height) { return image_grid(width, height, kTile, kTile); }
+uint32_t image_grid(uint32_t width, uint32_t height) { return image_grid(width, height, kTileW, kTileH); }
 
 // Appends `value` of every lane whose pred is set to list (one global atomic
 // per wave, slots in lane order).  Called by every lane of the wave.
@@ -820,14 +825,20 @@ __device__ __forceinline__ uint32_t spatial_pixel(const SceneDev& sc, const Fram
     return rays;
 }
 
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_spatial(SceneDev sc, FrameDev fr, uint32_t frame,
-                                                                            uint32_t reuse, PassParams pp) {
-    __shared__ int stack[kLdsDepth * kBlock];
+// The spatial pass keeps 20 stack levels in LDS (20 KiB per workgroup) and
+// runs at 7 waves per SIMD (72 VGPRs): its any-hit rays rarely go deeper, and
+// the extra wave took the 1080p pass from 1.99 to 1.91 ms (16 levels at 7
+// or 8 waves, 12 at 8 and 24 at 5 or 6 measured slower).
+constexpr int kSpatialLds = 20;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_spatial(SceneDev sc, FrameDev fr,
+                                                                                         uint32_t frame, uint32_t reuse,
+                                                                                         PassParams pp) {
+    __shared__ int stack[kSpatialLds * kBlock];
     __shared__ uint32_t lds_count;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
     uint32_t rays = 0;
     if (tc.in_image)
-        rays = spatial_pixel<kLdsDepth>(sc, fr, frame, reuse, pp, tc.x, tc.y, stack + threadIdx.x);
+        rays = spatial_pixel<kSpatialLds>(sc, fr, frame, reuse, pp, tc.x, tc.y, stack + threadIdx.x);
     count_rays(&fr.counters[1], rays, &lds_count);
 }
 
