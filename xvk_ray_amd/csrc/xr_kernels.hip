@@ -919,9 +919,11 @@ __global__ __launch_bounds__(256) void k_batch_temporal(BatchDev bd, PassParams 
     temporal_pixel(fr, bd.ubos[v].currentFrame, pp, fr.gpos, p);
 }
 
+// With at most 20 LDS stack levels (20 KiB) 7 workgroups fit a CU: ask for 7
+// waves, as the single-view spatial kernel does.
 template <int kTw, int kTh, int kLds>
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_spatial(SceneDev sc, BatchDev bd,
-                                                                                  PassParams pp) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLds <= 20 ? 7 : 6))) void k_batch_spatial(
+    SceneDev sc, BatchDev bd, PassParams pp) {
     __shared__ int stack[kLds * kBlock];
     __shared__ uint32_t lds_count;
     uint32_t local;
@@ -945,9 +947,9 @@ struct BatchKernels {
 #define XR_BATCH_CFG(TW, TH, L) \
     {TW, TH, L, k_batch_initial<TW, TH, L>, k_batch_tail<L>, k_batch_spatial<TW, TH, L>}
 static const BatchKernels kBatchKernels[] = {
-    XR_BATCH_CFG(16, 16, 24), XR_BATCH_CFG(16, 16, 16), XR_BATCH_CFG(16, 16, 32),
-    XR_BATCH_CFG(32, 8, 24),  XR_BATCH_CFG(32, 8, 16),  XR_BATCH_CFG(32, 8, 32),
-    XR_BATCH_CFG(8, 32, 24),  XR_BATCH_CFG(8, 32, 16),  XR_BATCH_CFG(8, 32, 32),
+    XR_BATCH_CFG(32, 8, 20),  XR_BATCH_CFG(32, 8, 16),  XR_BATCH_CFG(32, 8, 24),
+    XR_BATCH_CFG(16, 16, 20), XR_BATCH_CFG(16, 16, 16), XR_BATCH_CFG(16, 16, 24),
+    XR_BATCH_CFG(8, 32, 20),  XR_BATCH_CFG(8, 32, 16),  XR_BATCH_CFG(8, 32, 24),
 };
 #undef XR_BATCH_CFG
 
