@@ -13,6 +13,7 @@
 #ifndef XR_HOST_H
 #define XR_HOST_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include "xr_api.h"
 
@@ -37,6 +38,28 @@ int xrh_scene_cornell(uint64_t seed, xrh_scene** out);
  * U[0.6,0.9]) of radiance U[5,50] above it (one
  * instance). */
 int xrh_scene_terrain(uint64_t seed, uint32_t grid, uint32_t n_spheres, uint32_t n_lights, xrh_scene** out);
+
+/* glTF 2.0 files (.gltf JSON with external or base64 data-URI buffers; GLB
+ * is not read), one instance per file, loaded as AglTFModel::loadglTFFile
+ * and AScene do (src/Asset/Model.cpp:69-260, src/Asset/Scene.cpp:16-50):
+ * scene 0's nodes depth first, children before the node's own mesh; node
+ * matrix T*R*S or `matrix`, composed with its parents in glm's float order;
+ * positions world*(p,1); normals transpose(inverse(world))*(normalize(n),1),
+ * not renormalised (NaN when a primitive has no NORMAL, as in the
+ * reference); TEXCOORD_0 or 0; material index of the primitive (-1 when
+ * absent) plus the file's material offset; baseColorFactor, else zeros;
+ * baseColorTexture index, else -1; material model DiffuseLight for a
+ * material named "light", else Lambertian.
+ * XRH_GLTF_UNTEXTURED sets every texture index to -1, so the base colour
+ * factor is shaded where the reference samples a texture (textures are
+ * outside this layer: xr_create rejects materials that name one).
+ * Returns XR_ERR_INVALID_ARGUMENT with a message in err (when err_len > 0)
+ * for unreadable or malformed files and for inputs the reference would read
+ * out of bounds (no indices, index types other than u8/u16/u32, non-float
+ * attributes, accessors past their buffer). */
+#define XRH_GLTF_UNTEXTURED 0x1u
+int xrh_scene_gltf(const char* const* paths, uint32_t count, uint32_t flags, xrh_scene** out, char* err,
+                   size_t err_len);
 
 /* Pointers into the scene's arrays (valid until xrh_scene_destroy). */
 int xrh_scene_desc(const xrh_scene* s, xr_scene_desc* out);
@@ -93,6 +116,17 @@ int xrh_app_last_ubo(const xrh_app* app, xr_ubo* out);
 xr_ctx* xrh_app_ctx(xrh_app* app);
 const char* xrh_app_last_error(const xrh_app* app);
 void xrh_app_destroy(xrh_app* app);
+
+/* ---- image output ---- */
+
+/* PNG (8-bit RGB, stored deflate blocks) of a B8G8R8A8_UNORM image as
+ * xr_read_image returns it: the file-side counterpart of the reference's copy
+ * of its output image to the swapchain (ApplicationReSTIRGI.cpp:247-267).
+ * Alpha is dropped.  xrh_png_encode writes the file's bytes to dst (dst NULL:
+ * only *written, the size, is set); xrh_write_png writes them to path. */
+int xrh_png_encode(const uint32_t* bgra8, uint32_t width, uint32_t height, uint8_t* dst, size_t dst_bytes,
+                   size_t* written);
+int xrh_write_png(const char* path, const uint32_t* bgra8, uint32_t width, uint32_t height);
 
 /* Fills ubo for a given view and frame counter exactly as the driver does
  * (prev matrices from the arguments). */

@@ -9,6 +9,8 @@ to 1 LSB per channel.
 """
 from __future__ import annotations
 
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -169,6 +171,51 @@ def test_restir_terrain_sky():
     eyes = [(0.2, 0.4, 0.2), (0.21, 0.4, 0.2)]
     params = N.default_params(visibility=N.XR_VIS_TRACE)
     _frames(scene, 64, 48, 45.0, eyes, (0.5, 0.0, 0.5), 4, params, _strict, sky=1)
+
+
+# The reference's own cb2 scene (asset/model/cb2, CC-BY-4.0, see
+# tests/golden/gltf/cb2/license.txt) through the glTF loader, at the
+# reference's default view (Scene.cpp:29), fov and bounce count (main.cpp:45-47).
+CB2 = Path(__file__).parent / "golden" / "gltf" / "cb2" / "scene.gltf"
+CB2_EYE, CB2_TARGET = (0.0, 1.0, 3.0), (0.0, 1.0, 0.0)
+
+
+@pytest.mark.parametrize("vis", [N.XR_VIS_REFERENCE, N.XR_VIS_TRACE])
+def test_restir_cb2_gltf(vis):
+    scene = N.Scene.gltf(CB2)
+    eyes = [CB2_EYE, (0.02, 1.0, 3.0), (0.03, 1.01, 2.99)]
+    params = N.default_params(visibility=vis)
+    _frames(scene, 96, 64, 45.0, eyes, CB2_TARGET, 10, params, _strict)
+
+
+def test_plain_cb2_gltf(tmp_path):
+    scene = N.Scene.gltf(CB2)
+    W, H = 80, 48
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    view = look_view(CB2_EYE, CB2_TARGET)
+    accum = np.zeros((W * H, 4), np.float32)
+    out = np.zeros(W * H, np.uint32)
+    for frame in range(1, 3):
+        ubo = N.make_ubo(view, view, 45.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
+        say(f"[plain cb2] frame {frame}")
+        ctx.render(ubo, N.XR_MODE_PLAIN, N.default_params())
+        ctx.sync()
+        orc.plain(N.ubo_to_bytes(ubo), W, H, 0, accum, out)
+        ga = ctx.read(N.XR_BUF_ACCUMULATION)
+        eq = bytes_equal_rows(ga, accum)
+        assert eq.all(), describe_mismatch("accumulation", ga, accum, eq)
+        assert _max_lsb_diff(ctx.read(N.XR_BUF_OUTPUT), out) <= 1
+    assert accum[:, :3].max() > 0
+    # the image dump holds the output image's R, G, B bytes
+    import zlib
+    ctx.save_png(tmp_path / "cb2.png")
+    png = (tmp_path / "cb2.png").read_bytes()
+    idat = png[8 + 25 + 8:]  # IHDR chunk is 25 bytes; IDAT data follows its 8-byte header
+    raw = np.frombuffer(zlib.decompress(idat[:int.from_bytes(png[33:37], "big")]), np.uint8)
+    rgb = raw.reshape(H, 1 + 3 * W)[:, 1:].reshape(H * W, 3)
+    img = ctx.read_image()
+    assert np.array_equal(rgb[:, 0], (img >> 16) & 0xFF) and np.array_equal(rgb[:, 2], img & 0xFF)
 
 
 # ---------------------------------------------------------------------------

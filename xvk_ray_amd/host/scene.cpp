@@ -14,14 +14,7 @@
 #include <new>
 #include <vector>
 
-#include "xr_host.h"
-
-struct xrh_scene {
-    std::vector<xr_vertex> vertices;
-    std::vector<uint32_t> indices;
-    std::vector<xr_material> materials;
-    std::vector<uint32_t> offsets;
-};
+#include "scene_internal.h"
 
 namespace {
 

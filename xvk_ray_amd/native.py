@@ -9,6 +9,7 @@ fallback for any computation.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -29,6 +30,7 @@ XR_VIS_TRACE = 1
 XR_PASS_INITIAL, XR_PASS_TEMPORAL, XR_PASS_SPATIAL = 1, 2, 4
 XR_BUILDER_LBVH = 0
 XR_BUILDER_PLOC = 1
+XRH_GLTF_UNTEXTURED = 0x1
 
 XR_BUF_ACCUMULATION = 1
 XR_BUF_OUTPUT = 2
@@ -131,10 +133,11 @@ XR_API_SYMBOLS = [
     "xr_last_error", "xr_global_last_error",
 ]
 XR_HOST_SYMBOLS = [
-    "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_desc", "xrh_scene_triangle_count", "xrh_scene_destroy",
+    "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_gltf", "xrh_scene_desc", "xrh_scene_triangle_count",
+    "xrh_scene_destroy",
     "xrh_camera_static", "xrh_camera_orbit", "xrh_camera_view", "xrh_camera_destroy", "xrh_perspective",
     "xrh_inverse", "xrh_app_create", "xrh_app_draw_frame", "xrh_app_last_ubo", "xrh_app_ctx", "xrh_app_last_error",
-    "xrh_app_destroy", "xrh_make_ubo",
+    "xrh_app_destroy", "xrh_make_ubo", "xrh_png_encode", "xrh_write_png",
 ]
 
 _lib = None
@@ -225,6 +228,9 @@ def host() -> C.CDLL:
         P, U32, U64, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
         H.xrh_scene_cornell.argtypes = [U64, C.POINTER(P)]
         H.xrh_scene_terrain.argtypes = [U64, U32, U32, U32, C.POINTER(P)]
+        H.xrh_scene_gltf.argtypes = [C.POINTER(C.c_char_p), U32, U32, C.POINTER(P), C.c_char_p, C.c_size_t]
+        H.xrh_png_encode.argtypes = [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t)]
+        H.xrh_write_png.argtypes = [C.c_char_p, P, U32, U32]
         H.xrh_scene_desc.argtypes = [P, C.POINTER(XrSceneDesc)]
         H.xrh_scene_triangle_count.argtypes = [P]
         H.xrh_scene_triangle_count.restype = U32
@@ -303,6 +309,21 @@ class Scene:
         return cls(handle=h)
 
     @classmethod
+    def gltf(cls, *paths, untextured: bool = False) -> "Scene":
+        """glTF 2.0 files, one instance each, loaded as the reference's
+        AglTFModel / AScene do (see xrh_scene_gltf in include/xr_host.h)."""
+        if not paths:
+            raise ValueError("at least one path")
+        arr = (C.c_char_p * len(paths))(*[os.fsencode(os.fspath(p)) for p in paths])
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = host().xrh_scene_gltf(arr, len(paths), XRH_GLTF_UNTEXTURED if untextured else 0, C.byref(h), err,
+                                   len(err))
+        if rc:
+            raise XrError(rc, "xrh_scene_gltf: " + err.value.decode(errors="replace"))
+        return cls(handle=h)
+
+    @classmethod
     def from_arrays(cls, vertices: np.ndarray, indices: np.ndarray, materials: np.ndarray,
                     offsets: np.ndarray) -> "Scene":
         v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
@@ -330,12 +351,17 @@ class Scene:
         if self._arrays is not None:
             return self._arrays
         d = self.desc()
-        v = np.ctypeslib.as_array((C.c_uint8 * (36 * d.vertex_count)).from_address(d.vertices)).view(VERTEX_DTYPE)
-        i = np.ctypeslib.as_array((C.c_uint32 * d.index_count).from_address(d.indices))
-        m = np.ctypeslib.as_array((C.c_uint8 * (32 * d.material_count)).from_address(d.materials)).view(
-            MATERIAL_DTYPE)
-        o = np.ctypeslib.as_array((C.c_uint32 * (2 * d.instance_count)).from_address(d.offsets))
-        return v.copy(), i.copy(), m.copy(), o.copy()
+
+        def view(ptr, n, ctype, dtype):
+            if n == 0 or not ptr:  # an empty std::vector has no storage
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array((ctype * n).from_address(ptr)).view(dtype).copy()
+
+        v = view(d.vertices, 36 * d.vertex_count, C.c_uint8, VERTEX_DTYPE)
+        i = view(d.indices, d.index_count, C.c_uint32, np.uint32)
+        m = view(d.materials, 32 * d.material_count, C.c_uint8, MATERIAL_DTYPE)
+        o = view(d.offsets, 2 * d.instance_count, C.c_uint32, np.uint32)
+        return v, i, m, o
 
     @property
     def triangle_count(self) -> int:
@@ -458,6 +484,20 @@ class Context:
         out = np.empty(n, np.uint8)
         self.check(lib().xr_read_buffer(self._h, which, out.ctypes.data, n, layout))
         return _typed(out, which, layout)
+
+    def read_image(self) -> np.ndarray:
+        """The tone-mapped B8G8R8A8 output image (xr_read_image), one uint32 per pixel."""
+        n = self.buffer_bytes(XR_BUF_OUTPUT, XR_LAYOUT_REFERENCE)
+        out = np.empty(n // 4, np.uint32)
+        self.check(lib().xr_read_image(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def save_png(self, path) -> None:
+        """The output image as an 8-bit RGB PNG (xrh_write_png)."""
+        img = self.read_image()
+        rc = host().xrh_write_png(os.fsencode(os.fspath(path)), img.ctypes.data, self.width, self.height)
+        if rc:
+            raise XrError(rc, f"xrh_write_png {path}")
 
     def write(self, which: int, data: np.ndarray, layout: int = XR_LAYOUT_REFERENCE) -> None:
         a = np.ascontiguousarray(data)
