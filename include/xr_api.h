@@ -53,7 +53,7 @@ typedef struct xr_vertex {
 #define XR_MATERIAL_DIFFUSE_LIGHT 4u
 typedef struct xr_material {
     float base_color[4];
-    int32_t base_color_texture_index; /* must be -1: texture sampling is not in this layer */
+    int32_t base_color_texture_index; /* -1, or an index into the textures given to xr_set_textures */
     float fuzziness;
     float refraction_index;
     uint32_t material_model;
@@ -187,6 +187,25 @@ int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ct
 #define XR_BUILDER_LBVH 0
 #define XR_BUILDER_PLOC 1
 int xr_build_accel(xr_ctx* ctx, int builder);
+
+/* Base-colour textures (descriptor binding 8, TextureSamplers[],
+ * BSDF.hlsl:4-5; images uploaded as R8G8B8A8_UNORM, TextureImage.cpp:25, and
+ * sampled with a LINEAR, CLAMP_TO_EDGE sampler at level 0, XVKSampler.h:9-26,
+ * BSDF.hlsl:22).  Copies `count` RGBA8 images (rows top to bottom, texel
+ * (x, y) at rgba8[4 * (y * width + x)]) to the device, replacing earlier
+ * ones.  A frame whose scene names a texture index >= count fails with
+ * XR_ERR_STATE.  Sampling: x = u * width - 0.5, i0 = floor(x), a = x - i0
+ * (x first clamped to [-1, width]), likewise y / j0 / b; the four texels
+ * clamped to the edge; colour = ((w00 t00 + w10 t10) + w01 t01) + w11 t11
+ * with w00 = (1-a)(1-b), w10 = a(1-b), w01 = (1-a)b, w11 = ab and texels as
+ * byte / 255, in FP32 without contraction (the oracle's definition; the
+ * reference's hardware sampler rounds its weights to a fixed-point grid of
+ * its own). */
+typedef struct xr_texture {
+    uint32_t width, height;
+    const uint8_t* rgba8;
+} xr_texture;
+int xr_set_textures(xr_ctx* ctx, const xr_texture* textures, uint32_t count);
 
 /* Reallocates the frame buffers (swapchain recreation,
  * ApplicationReSTIRGI.cpp:115-148); reservoir history is cleared. */

@@ -50,9 +50,11 @@ int xrh_scene_terrain(uint64_t seed, uint32_t grid, uint32_t n_spheres, uint32_t
  * absent) plus the file's material offset; baseColorFactor, else zeros;
  * baseColorTexture index, else -1; material model DiffuseLight for a
  * material named "light", else Lambertian.
- * XRH_GLTF_UNTEXTURED sets every texture index to -1, so the base colour
- * factor is shaded where the reference samples a texture (textures are
- * outside this layer: xr_create rejects materials that name one).
+ * Base-colour textures: the file's `textures` in order, each the PNG image
+ * its `source` names (uri relative to the file, data URI, or bufferView),
+ * decoded to RGBA8 (xrh_scene_textures); other image formats (JPEG) fail.
+ * XRH_GLTF_UNTEXTURED loads no images and sets every texture index to -1,
+ * so the base colour factor is shaded where the reference samples a texture.
  * Returns XR_ERR_INVALID_ARGUMENT with a message in err (when err_len > 0)
  * for unreadable or malformed files and for inputs the reference would read
  * out of bounds (no indices, index types other than u8/u16/u32, non-float
@@ -63,6 +65,16 @@ int xrh_scene_gltf(const char* const* paths, uint32_t count, uint32_t flags, xrh
 
 /* Pointers into the scene's arrays (valid until xrh_scene_destroy). */
 int xrh_scene_desc(const xrh_scene* s, xr_scene_desc* out);
+/* The scene's textures (glTF scenes loaded without XRH_GLTF_UNTEXTURED: the
+ * PNG images of every file's textures, in file order, as AScene collects them,
+ * Scene.cpp:56-63; material texture indices are file-local, as in the
+ * reference).  *textures is valid until xrh_scene_destroy; count may be 0. */
+int xrh_scene_textures(const xrh_scene* s, const xr_texture** textures, uint32_t* count);
+/* PNG file bytes -> RGBA8 (every colour type and depth except interlaced
+ * images; grey replicated, alpha 255 unless the image carries one, 16-bit
+ * samples keep their high byte).  rgba8 NULL: only the size is returned. */
+int xrh_png_decode(const uint8_t* data, size_t bytes, uint32_t* width, uint32_t* height, uint8_t* rgba8,
+                   size_t rgba_bytes);
 uint32_t xrh_scene_triangle_count(const xrh_scene* s);
 void xrh_scene_destroy(xrh_scene* s);
 

@@ -56,6 +56,8 @@ class Oracle:
             L.orc_merge_reservoir.restype = None
             L.orc_scene_create.argtypes = [P, U32, P, U32, P, U32, P, U32]
             L.orc_scene_create.restype = P
+            L.orc_scene_set_textures.argtypes = [P, P, U32, P, C.c_size_t]
+            L.orc_scene_set_textures.restype = C.c_int
             L.orc_scene_destroy.argtypes = [P]
             L.orc_scene_destroy.restype = None
             L.orc_trace32.argtypes = [P, P, U32, P, I]
@@ -102,7 +104,8 @@ def _ptr(a):
 class OracleScene:
     """Oracle copy of a scene (its own FP64-box BVH) plus the pass restatements."""
 
-    def __init__(self, vertices, indices, materials, offsets):
+    def __init__(self, vertices, indices, materials, offsets, textures=()):
+        """textures: sequence of (width, height, RGBA8 uint8 array of width*height*4)."""
         self.L = Oracle.lib()
         self.v = np.ascontiguousarray(vertices)
         self.i = np.ascontiguousarray(indices, np.uint32)
@@ -112,6 +115,17 @@ class OracleScene:
                                          self.m.ctypes.data, len(self.m), self.o.ctypes.data, len(self.o) // 2)
         if not self.h:
             raise ValueError("oracle rejected the scene: an index is out of range for its instance")
+        if len(textures):
+            info, chunks, first = [], [], 0
+            for w, h, rgba in textures:
+                info += [first, w, h]
+                chunks.append(np.ascontiguousarray(rgba, np.uint8).reshape(-1))
+                first += w * h
+            self.tex_info = np.array(info, np.uint32)
+            self.texels = np.concatenate(chunks)
+            if self.L.orc_scene_set_textures(self.h, self.tex_info.ctypes.data, len(textures),
+                                             self.texels.ctypes.data, self.texels.nbytes):
+                raise ValueError("oracle rejected the textures")
 
     def close(self):
         if self.h:

@@ -278,7 +278,47 @@ float3 SampleHemisphere(float2 xi) {
     float sin_theta = sqrtf(xi.y);
     return F3(sin_theta * hcos(phi), sin_theta * hsin(phi), cos_theta);
 }
-float3 SampleLambertian(const BSDFMaterial& mat, float3& wi, float& pdf, float& cos_theta, uint32_t& seed) {
+// TextureSamplers[] (binding 8) with LinearSampler (BSDF.hlsl:4-5, 22):
+// R8G8B8A8_UNORM images (TextureImage.cpp:25) sampled at level 0 with a
+// LINEAR, CLAMP_TO_EDGE sampler (XVKSampler.h:9-26).  The texture-space
+// position, the two taps per axis, their clamp to the edge and the weights
+// (1-a)(1-b), a(1-b), (1-a)b, ab follow the Vulkan specification's linear
+// filtering; the FP32 sum order and the clamp of x to [-1, width] before the
+// floor (beyond it both taps are the edge texel) are this restatement's
+// definition, which the device follows.  Parity with the reference's sampler
+// hardware, which rounds its weights to a fixed-point grid, is unpinned.
+struct TextureSet {
+    std::vector<uint32_t> info;    // (first texel, width, height) per texture
+    std::vector<uint8_t> texels;   // RGBA8
+    uint32_t count() const { return (uint32_t)(info.size() / 3); }
+};
+float unorm8(uint8_t b) { return (float)b / 255.0f; }
+float3 TextureSample(const TextureSet& ts, int t, float2 uv) {
+    if ((uint32_t)t >= ts.count()) return F3(0, 0, 0);  // not reached: the device refuses such scenes
+    const uint32_t first = ts.info[3 * t], w = ts.info[3 * t + 1], h = ts.info[3 * t + 2];
+    const float x = fminf(fmaxf(uv.x * (float)w - 0.5f, -1.0f), (float)w);
+    const float y = fminf(fmaxf(uv.y * (float)h - 0.5f, -1.0f), (float)h);
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int i0 = (int)fx, j0 = (int)fy;
+    auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+    const int x0 = cl(i0, (int)w - 1), x1 = cl(i0 + 1, (int)w - 1);
+    const int y0 = cl(j0, (int)h - 1), y1 = cl(j0 + 1, (int)h - 1);
+    auto tex = [&](int xx, int yy, int c) { return unorm8(ts.texels[4 * ((size_t)first + (size_t)yy * w + xx) + c]); };
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    float r[3];
+    for (int c = 0; c < 3; ++c)
+        r[c] = ((w00 * tex(x0, y0, c) + w10 * tex(x1, y0, c)) + w01 * tex(x0, y1, c)) + w11 * tex(x1, y1, c);
+    return F3(r[0], r[1], r[2]);
+}
+// (mat.baseColorTextureIndex >= 0) ? texColor.rgb : mat.baseColor.rgb (BSDF.hlsl:22-23, 29-30)
+float3 MaterialColor(const TextureSet& ts, const BSDFMaterial& mat, float2 uv) {
+    if (mat.baseColorTextureIndex >= 0) return TextureSample(ts, mat.baseColorTextureIndex, uv);
+    return F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]);
+}
+
+float3 SampleLambertian(const TextureSet& ts, const BSDFMaterial& mat, float2 uv, float3& wi, float& pdf,
+                        float& cos_theta, uint32_t& seed) {
     float x0 = RandomFloat(seed);
     float x1 = RandomFloat(seed);
     wi = SampleHemisphere(float2{x0, x1});
@@ -288,18 +328,18 @@ float3 SampleLambertian(const BSDFMaterial& mat, float3& wi, float& pdf, float& 
         pdf = 0;
         return F3(0, 0, 0);
     }
-    float3 albedo = F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]);
+    float3 albedo = MaterialColor(ts, mat, uv);
     return albedo * INV_PI;
 }
-float3 SampleBSDF(float3 n, const BSDFMaterial& mat, float3& wi_world, float& pdf_w, float& cos_theta,
-                  uint32_t& seed) {
+float3 SampleBSDF(const TextureSet& ts, float3 n, float2 uv, const BSDFMaterial& mat, float3& wi_world,
+                  float& pdf_w, float& cos_theta, uint32_t& seed) {
     float3 f = F3(0, 0, 0);
     pdf_w = 0;
     cos_theta = 0;
     float3 T, B;
     BranchlessONB(n, T, B);
     float3 wi_local = F3(0, 0, 0);
-    if (mat.materialModel == BSDFLambertian) f = SampleLambertian(mat, wi_local, pdf_w, cos_theta, seed);
+    if (mat.materialModel == BSDFLambertian) f = SampleLambertian(ts, mat, uv, wi_local, pdf_w, cos_theta, seed);
     wi_world = to_world(wi_local, T, B, n);
     return f;
 }
@@ -324,6 +364,7 @@ struct Scene {
     std::vector<Vertex> vertices;
     std::vector<uint32_t> indices;
     std::vector<BSDFMaterial> materials;
+    TextureSet textures;
     std::vector<uint32_t> offsets;      // 2 per instance
     std::vector<uint32_t> first;        // first canonical triangle per instance (+ total)
     std::vector<float> tv;              // 9 floats per canonical triangle (v0, v1, v2)
@@ -650,7 +691,7 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
             break;
         }
         const BSDFMaterial& mat = s.materials[pl.MaterialIndex];
-        float3 bc = F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]);
+        float3 bc = MaterialColor(s.textures, mat, pl.UV);
         if (mat.materialModel == BSDFDiffuseLight) {
             L_out = L_out + throughput * bc * 4.0f;
             if (bounce == 1) { x_sample = pl.Position; n_sample = pl.Normal; }
@@ -675,7 +716,7 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
             }
         }
         uint32_t seedForThisBounce = rngState;
-        float3 f = SampleBSDF(pl.Normal, mat, wi, pdf, cos_theta, rngState);
+        float3 f = SampleBSDF(s.textures, pl.Normal, pl.UV, mat, wi, pdf, cos_theta, rngState);
         if (bounce == 0) {
             primitiveHitValid = true;
             x_view = pl.Position;
@@ -920,12 +961,12 @@ uint32_t plain_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32_t y, uin
             }
             const BSDFMaterial& mat = s.materials[pl.MaterialIndex];
             if (mat.materialModel == BSDFDiffuseLight) {
-                rayColor = rayColor + throughput * F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]) * 4.0f;
+                rayColor = rayColor + throughput * MaterialColor(s.textures, mat, pl.UV) * 4.0f;
                 break;
             }
             float pdf = 0, cos_theta = 0;
             float3 wi;
-            float3 f = SampleBSDF(pl.Normal, mat, wi, pdf, cos_theta, rngState);
+            float3 f = SampleBSDF(s.textures, pl.Normal, pl.UV, mat, wi, pdf, cos_theta, rngState);
             if (pdf <= 1e-6f) break;
             throughput = throughput * (f * cos_theta / pdf);
             if (bounce > 3) {
@@ -1000,7 +1041,8 @@ void orc_sample_lambert(const float* n, const float* albedo, uint32_t seed, floa
     m.baseColorTextureIndex = -1;
     m.materialModel = BSDFLambertian;
     float3 w;
-    float3 ff = SampleBSDF(F3(n[0], n[1], n[2]), m, w, *pdf, *cos_theta, seed);
+    const TextureSet none;
+    float3 ff = SampleBSDF(none, F3(n[0], n[1], n[2]), float2{0.0f, 0.0f}, m, w, *pdf, *cos_theta, seed);
     wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
     f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
     *seed_out = seed;
@@ -1019,6 +1061,18 @@ void orc_merge_reservoir(void* r, const void* src, float p_hat_, uint32_t* seed)
 }
 
 // ---- scene ----
+// Textures of the scene: info = (first texel, width, height) per texture,
+// texels RGBA8.  Returns 0, or -1 when a texture runs past the texels.
+int orc_scene_set_textures(void* scene, const uint32_t* info, uint32_t count, const uint8_t* texels,
+                           size_t texel_bytes) {
+    Scene* s = (Scene*)scene;
+    for (uint32_t t = 0; t < count; ++t)
+        if (((size_t)info[3 * t] + (size_t)info[3 * t + 1] * info[3 * t + 2]) * 4 > texel_bytes) return -1;
+    s->textures.info.assign(info, info + 3 * (size_t)count);
+    s->textures.texels.assign(texels, texels + texel_bytes);
+    return 0;
+}
+
 void* orc_scene_create(const void* vertices, uint32_t vcount, const uint32_t* indices, uint32_t icount,
                        const void* materials, uint32_t mcount, const uint32_t* offsets, uint32_t ninst) {
     // reject what the reference would read out of bounds

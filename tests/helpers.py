@@ -55,7 +55,7 @@ def rays_f64(r: np.ndarray) -> np.ndarray:
 def oracle_scene(scene: N.Scene):
     from oracle.binding import OracleScene
     v, i, m, o = scene.arrays()
-    return OracleScene(v, i, m, o)
+    return OracleScene(v, i, m, o, scene.textures())
 
 
 def bytes_equal_rows(a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -67,9 +67,9 @@ def test_create_rejects_bad_arguments_before_device_work():
     rc, _, msg = _create(v, i[:2], m, o)
     assert rc == N.XR_ERR_INVALID_ARGUMENT
     m2 = m.copy()
-    m2["base_color_texture_index"] = 0
+    m2["base_color_texture_index"] = -2
     rc, _, msg = _create(v, i, m2, o)
-    assert rc == N.XR_ERR_UNSUPPORTED and "texture" in msg
+    assert rc == N.XR_ERR_INVALID_ARGUMENT and "texture" in msg
     v2 = v.copy()
     v2["material_index"] = 5
     rc, _, msg = _create(v2, i, m, o)

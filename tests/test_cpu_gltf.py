@@ -283,10 +283,13 @@ def test_untextured_flag(tmp_path):
     assert (m["base_color_texture_index"] == -1).all()
 
 
-def test_textured_scene_is_refused_by_the_context(tmp_path):
+def test_texture_index_without_textures_is_kept(tmp_path):
+    """Material 1 names texture 2 but the file has no textures: the index is
+    kept as the reference keeps it (a frame then fails with XR_ERR_STATE)."""
     path, _ = _synthetic(tmp_path)
-    with pytest.raises(N.XrError):
-        N.Context(N.Scene.gltf(path), 8, 8)
+    scene = N.Scene.gltf(path)
+    assert scene.textures() == []
+    assert scene.arrays()[2]["base_color_texture_index"][1] == 2
 
 
 @pytest.mark.parametrize("case", ["missing", "json", "float_indices", "no_indices", "short_buffer", "past_buffer"])

@@ -218,6 +218,68 @@ def test_plain_cb2_gltf(tmp_path):
     assert np.array_equal(rgb[:, 0], (img >> 16) & 0xFF) and np.array_equal(rgb[:, 2], img & 0xFF)
 
 
+# Textured materials: the Cornell scene with planar texture coordinates that
+# run past [0, 1] (clamp to edge), alternate diffuse materials on a 37x23 or
+# a 5x9 noise texture and the emitter on a 1x1 texture (emission = texel * 4).
+def textured_cornell() -> N.Scene:
+    v, i, m, o = cornell_scene().arrays()
+    v, m = v.copy(), m.copy()
+    v["tex_coord"][:, 0] = v["pos"][:, 0] * 1.7 - 0.2
+    v["tex_coord"][:, 1] = v["pos"][:, 1] * 1.3 + v["pos"][:, 2] * 0.5 - 0.1
+    rng = np.random.default_rng(5)
+    tex = [(37, 23, rng.integers(0, 256, 37 * 23 * 4, dtype=np.uint8)),
+           (5, 9, rng.integers(0, 256, 5 * 9 * 4, dtype=np.uint8)),
+           (1, 1, np.array([200, 100, 50, 255], np.uint8))]
+    lam = np.flatnonzero(m["material_model"] == N.MATERIAL_LAMBERTIAN)
+    m["base_color_texture_index"][lam[::2]] = 0
+    m["base_color_texture_index"][lam[1::2]] = 1
+    m["base_color_texture_index"][m["material_model"] == N.MATERIAL_DIFFUSE_LIGHT] = 2
+    return N.Scene.from_arrays(v, i, m, o, tex)
+
+
+@pytest.mark.parametrize("vis", [N.XR_VIS_REFERENCE, N.XR_VIS_TRACE])
+def test_restir_textured(vis):
+    eyes = [CORNELL_EYE, (0.52, 0.5, -1.2)]
+    params = N.default_params(visibility=vis, spatial_samples=5, spatial_radius=30.0)
+    _frames(textured_cornell(), 72, 56, 60.0, eyes, CORNELL_TARGET, 4, params, _strict)
+
+
+def test_plain_textured():
+    scene = textured_cornell()
+    W, H = 64, 48
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    accum = np.zeros((W * H, 4), np.float32)
+    out = np.zeros(W * H, np.uint32)
+    for frame in range(1, 3):
+        ubo = N.make_ubo(view, view, 60.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
+        say(f"[plain textured] frame {frame}")
+        ctx.render(ubo, N.XR_MODE_PLAIN, N.default_params(plain_max_depth=0))
+        ctx.sync()
+        orc.plain(N.ubo_to_bytes(ubo), W, H, 0, accum, out)
+        ga = ctx.read(N.XR_BUF_ACCUMULATION)
+        eq = bytes_equal_rows(ga, accum)
+        assert eq.all(), describe_mismatch("accumulation", ga, accum, eq)
+    assert accum[:, :3].max() > 0
+
+
+def test_missing_texture_refuses_the_frame():
+    v, i, m, o = cornell_scene().arrays()
+    m = m.copy()
+    m["base_color_texture_index"][0] = 1
+    scene = N.Scene.from_arrays(v, i, m, o, [(1, 1, np.full(4, 255, np.uint8))])  # only texture 0
+    ctx = N.Context(scene, 16, 16)
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    ubo = N.make_ubo(view, view, 60.0, 16, 16)
+    with pytest.raises(N.XrError) as e:
+        ctx.render(ubo, N.XR_MODE_RESTIR, N.default_params())
+    assert e.value.code == N.XR_ERR_STATE
+    ctx.set_textures([(1, 1, np.full(4, 255, np.uint8)), (2, 1, np.full(8, 9, np.uint8))])
+    ctx.render(ubo, N.XR_MODE_RESTIR, N.default_params())
+    ctx.sync()
+
+
 # ---------------------------------------------------------------------------
 # Plain path tracer
 # ---------------------------------------------------------------------------

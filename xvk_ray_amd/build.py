@@ -44,7 +44,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 HIP_SOURCES = ["xr_api.hip", "xr_kernels.hip", "xr_bvh_build.hip", "xr_mlp.hip"]
-HOST_SOURCES = ["scene.cpp", "gltf.cpp", "png.cpp", "camera.cpp", "driver.cpp"]
+HOST_SOURCES = ["scene.cpp", "gltf.cpp", "png.cpp", "png_decode.cpp", "camera.cpp", "driver.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

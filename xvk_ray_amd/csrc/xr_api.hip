@@ -57,6 +57,11 @@ struct xr_ctx {
     uint2* d_offsets = nullptr;
     uint32_t* d_inst_first = nullptr;
     uint32_t vertex_count = 0, index_count = 0, material_count = 0, instance_count = 0, tri_count = 0;
+    // base-colour textures (xr_set_textures)
+    uint4* d_tex_info = nullptr;  // (first texel, width, height, 0) per texture
+    uint32_t* d_texels = nullptr;  // RGBA8, R in the low byte
+    uint32_t texture_count = 0;
+    int32_t max_texture_index = -1;  // largest index a material names
 
     // acceleration structure
     BvhNode* d_nodes = nullptr;
@@ -220,6 +225,8 @@ SceneDev scene_dev(const xr_ctx* c) {
     s.indices = c->d_indices;
     s.materials = c->d_materials;
     s.offsets = c->d_offsets;
+    s.tex_info = c->d_tex_info;
+    s.texels = c->d_texels;
     s.nodes = c->d_nodes;
     s.tris = c->d_tris;
     s.material_count = c->material_count;
@@ -323,7 +330,15 @@ int ensure_mlp_io(xr_ctx* c, size_t floats) {
     return XR_OK;
 }
 
+int check_textures(xr_ctx* c) {
+    if (c->max_texture_index >= (int32_t)c->texture_count)
+        return set_err(c, XR_ERR_STATE, "a material names texture %d; xr_set_textures has given %u",
+                       c->max_texture_index, c->texture_count);
+    return XR_OK;
+}
+
 int check_params(xr_ctx* c, const xr_restir_params* p) {
+    if (int rc = check_textures(c)) return rc;
     for (int i = 0; i < 4; ++i)
         if (p->reserved[i] != 0) return set_err(c, XR_ERR_INVALID_ARGUMENT, "params.reserved[%d] must be zero", i);
     if (p->flags & ~(XR_FLAG_RADIANCE | XR_FLAG_MLP_CACHE | XR_FLAG_MLP_FP8))
@@ -375,6 +390,8 @@ void xr_destroy(xr_ctx* c) {
     dev_free(&c->d_indices);
     dev_free(&c->d_materials);
     dev_free(&c->d_offsets);
+    dev_free(&c->d_tex_info);
+    dev_free(&c->d_texels);
     dev_free(&c->d_inst_first);
     dev_free(&c->d_nodes);
     dev_free(&c->d_tris);
@@ -403,9 +420,11 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     if (s->index_count % 3 != 0) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "index_count %% 3 != 0");
     if (s->index_count / 3 >= (1u << 27))
         return set_err(nullptr, XR_ERR_UNSUPPORTED, "more than 2^27 triangles");
+    int32_t max_tex = -1;
     for (uint32_t m = 0; m < s->material_count; ++m) {
-        if (s->materials[m].base_color_texture_index >= 0)
-            return set_err(nullptr, XR_ERR_UNSUPPORTED, "material %u is textured; texture sampling is not supported", m);
+        const int32_t t = s->materials[m].base_color_texture_index;
+        if (t < -1) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "material %u texture index %d", m, t);
+        max_tex = t > max_tex ? t : max_tex;
     }
     for (uint32_t v = 0; v < s->vertex_count; ++v) {
         const int32_t mi = s->vertices[v].material_index;
@@ -462,6 +481,7 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     c->vertex_count = s->vertex_count;
     c->index_count = s->index_count;
     c->material_count = s->material_count;
+    c->max_texture_index = max_tex;
     c->instance_count = s->instance_count;
     c->tri_count = s->index_count / 3;
     for (int a = 0; a < 3; ++a) {
@@ -490,6 +510,34 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
     rc = alloc_frame(c, width, height);
     if (rc != XR_OK) return fail(rc);
     *out = c;
+    return XR_OK;
+}
+
+int xr_set_textures(xr_ctx* c, const xr_texture* t, uint32_t count) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (count && !t) return set_err(c, XR_ERR_INVALID_ARGUMENT, "textures is NULL");
+    std::vector<uint4> info(count);
+    size_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!t[i].rgba8 || t[i].width == 0 || t[i].height == 0 || t[i].width > 16384 || t[i].height > 16384)
+            return set_err(c, XR_ERR_INVALID_ARGUMENT, "texture %u: bad size or NULL data", i);
+        info[i] = make_uint4((uint32_t)total, t[i].width, t[i].height, 0u);
+        total += (size_t)t[i].width * t[i].height;
+        if (total >= (1ull << 32)) return set_err(c, XR_ERR_UNSUPPORTED, "more than 2^32 texels");
+    }
+    XR_HIP(c, hipStreamSynchronize(c->stream));  // frames in flight may still read the old textures
+    dev_free(&c->d_tex_info);
+    dev_free(&c->d_texels);
+    c->texture_count = 0;
+    if (count) {
+        XR_HIP(c, dev_alloc(&c->d_tex_info, sizeof(uint4) * count));
+        XR_HIP(c, dev_alloc(&c->d_texels, sizeof(uint32_t) * total));
+        XR_HIP(c, hipMemcpy(c->d_tex_info, info.data(), sizeof(uint4) * count, hipMemcpyHostToDevice));
+        for (uint32_t i = 0; i < count; ++i)
+            XR_HIP(c, hipMemcpy(c->d_texels + info[i].x, t[i].rgba8, 4ull * t[i].width * t[i].height,
+                                hipMemcpyHostToDevice));
+    }
+    c->texture_count = count;
     return XR_OK;
 }
 
@@ -734,7 +782,9 @@ int xr_converge(xr_ctx* c, const xr_ubo* ubo, uint32_t spp, uint32_t frame_base)
     if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!ubo) return set_err(c, XR_ERR_INVALID_ARGUMENT, "ubo is NULL");
     if (!c->built) return set_err(c, XR_ERR_STATE, "xr_build_accel has not run");
-    int rc = ensure_conv(c);
+    int rc = check_textures(c);
+    if (rc) return rc;
+    rc = ensure_conv(c);
     if (rc) return rc;
     UniformBufferObject u;
     memcpy(&u, ubo, sizeof(u));

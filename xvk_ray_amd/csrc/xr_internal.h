@@ -19,6 +19,8 @@ struct SceneDev {
     const BvhNode* nodes;
     const BvhTri* tris;
     uint32_t material_count;
+    const uint4* tex_info;    // (first texel, width, height, 0) per texture (binding 8)
+    const uint32_t* texels;   // RGBA8, R in the low byte
 };
 
 struct FrameDev {
@@ -338,10 +340,45 @@ XR_HD void onb(v3 n, v3* b1, v3* b2) {
     *b2 = mk3(b, sign + n.y * n.y * a, -n.y);
 }
 
-// SampleBSDF (BSDF.hlsl:34-57) with SampleLambertian (:7-25).  Untextured
-// materials only (the scene loader rejects texture indices >= 0).
-__device__ __forceinline__ v3 sample_bsdf(v3 n, const float4 mat0, const float4 mat1, v3* wi_world, float* pdf,
-                                          float* cos_theta, uint32_t* seed) {
+// TextureSamplers[t].SampleLevel(LinearSampler, uv, 0).rgb (BSDF.hlsl:22):
+// bilinear, clamp to edge, R8G8B8A8_UNORM texels as byte / 255.  The weights
+// and the sum order are this layer's definition (the oracle's is the same);
+// the reference's sampler hardware rounds its weights to a fixed-point grid.
+__device__ __forceinline__ float unorm8(uint32_t p, int shift) { return (float)((p >> shift) & 0xFFu) / 255.0f; }
+__device__ __forceinline__ v3 tex_bilinear(const SceneDev& sc, int t, float u, float v) {
+    const uint4 ti = sc.tex_info[t];
+    const int w = (int)ti.y, h = (int)ti.z;
+    // x clamped to [-1, w] first (NaN -> -1): beyond it both taps are the edge texel
+    const float x = fminf(fmaxf(u * (float)w - 0.5f, -1.0f), (float)w);
+    const float y = fminf(fmaxf(v * (float)h - 0.5f, -1.0f), (float)h);
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int i0 = (int)fx, j0 = (int)fy;
+    const int x0 = min(max(i0, 0), w - 1), x1 = min(max(i0 + 1, 0), w - 1);
+    const int y0 = min(max(j0, 0), h - 1), y1 = min(max(j0 + 1, 0), h - 1);
+    const uint32_t* tx = sc.texels + ti.x;
+    const uint32_t p00 = tx[y0 * w + x0], p10 = tx[y0 * w + x1], p01 = tx[y1 * w + x0], p11 = tx[y1 * w + x1];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    float c[3];
+    for (int k = 0; k < 3; ++k) {
+        const int sh = 8 * k;
+        c[k] = ((w00 * unorm8(p00, sh) + w10 * unorm8(p10, sh)) + w01 * unorm8(p01, sh)) + w11 * unorm8(p11, sh);
+    }
+    return mk3(c[0], c[1], c[2]);
+}
+
+// Base colour of a material at texture coordinates (u, v): its texture when
+// it names one, else baseColor.rgb (BSDF.hlsl:22-23, 29-30).
+__device__ __forceinline__ v3 material_color(const SceneDev& sc, const float4 m0, const float4 m1, float u,
+                                             float v) {
+    const int t = __float_as_int(m1.x);
+    return t >= 0 ? tex_bilinear(sc, t, u, v) : mk3(m0.x, m0.y, m0.z);
+}
+
+// SampleBSDF (BSDF.hlsl:34-57) with SampleLambertian (:7-25); the albedo is
+// read only for a sample above the surface, as the reference reads it.
+__device__ __forceinline__ v3 sample_bsdf(const SceneDev& sc, v3 n, const float4 mat0, const float4 mat1, float tu,
+                                          float tv, v3* wi_world, float* pdf, float* cos_theta, uint32_t* seed) {
     v3 T, B;
     onb(n, &T, &B);
     v3 f = mk3(0.0f, 0.0f, 0.0f);
@@ -363,7 +400,8 @@ __device__ __forceinline__ v3 sample_bsdf(v3 n, const float4 mat0, const float4 
         if (wl.z <= 0.0f) {
             *pdf = 0.0f;
         } else {
-            f = mk3(mat0.x * XR_INV_PI, mat0.y * XR_INV_PI, mat0.z * XR_INV_PI);
+            const v3 al = material_color(sc, mat0, mat1, tu, tv);
+            f = mk3(al.x * XR_INV_PI, al.y * XR_INV_PI, al.z * XR_INV_PI);
         }
     }
     *wi_world = mk3(wl.x * T.x + wl.y * B.x + wl.z * n.x, wl.x * T.y + wl.y * B.y + wl.z * n.y,

@@ -194,7 +194,7 @@ __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uin
         const float4 m0 = sc.materials[2 * pl.material];
         const float4 m1 = sc.materials[2 * pl.material + 1];
         if (__float_as_uint(m1.w) == kDiffuseLight) {
-            o->L = add(o->L, muls(mul(ps->thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+            o->L = add(o->L, muls(mul(ps->thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
             if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
             if (bounce == 0) { o->x_view = pl.position; o->n_view = pl.normal; }
             return false;
@@ -219,7 +219,7 @@ __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uin
         const uint32_t seed_here = ps->rng;
         v3 wi;
         float pdf, cos_t;
-        const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &ps->rng);
+        const v3 f = sample_bsdf(sc, pl.normal, m0, m1, pl.u, pl.v, &wi, &pdf, &cos_t, &ps->rng);
         if (bounce == 0) {
             o->valid = true;
             o->x_view = pl.position;
@@ -1023,12 +1023,12 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain(SceneDe
                 const float4 m0 = sc.materials[2 * pl.material];
                 const float4 m1 = sc.materials[2 * pl.material + 1];
                 if (__float_as_uint(m1.w) == kDiffuseLight) {
-                    color = add(color, muls(mul(thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+                    color = add(color, muls(mul(thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
                     break;
                 }
                 v3 wi;
                 float pdf, cos_t;
-                const v3 f = sample_bsdf(pl.normal, m0, m1, &wi, &pdf, &cos_t, &rng);
+                const v3 f = sample_bsdf(sc, pl.normal, m0, m1, pl.u, pl.v, &wi, &pdf, &cos_t, &rng);
                 if (pdf <= 1e-6f) break;
                 thr = mul(thr, divs(muls(f, cos_t), pdf));
                 if (bounce > 3) {

@@ -464,6 +464,7 @@ struct Model {
     std::vector<xr_vertex> vertices;
     std::vector<uint32_t> indices;
     std::vector<xr_material> materials;
+    std::vector<xrh_texture_data> textures;
 };
 
 class GltfLoader {
@@ -480,6 +481,7 @@ public:
         if (root_.kind != Json::kObject) fail(path_ + ": top level is not an object");
         load_buffers();
         load_materials();
+        if (!untextured_) load_textures();
         const Json& scenes = array_of(root_, "scenes");
         if (scenes.items.empty()) fail(path_ + ": no scenes");
         const Json& nodes = array_of(root_, "nodes");
@@ -554,6 +556,54 @@ private:
                                      : XR_MATERIAL_LAMBERTIAN;
             model_.materials.push_back(mat);
         }
+    }
+
+    // AglTFModel::loadImages / loadTextures (Model.cpp:19-67): texture i is
+    // the image its `source` names, decoded to RGBA8.
+    void load_textures() {
+        const Json& images = array_of(root_, "images");
+        std::vector<xrh_texture_data> decoded(images.items.size());
+        std::vector<bool> done(images.items.size(), false);
+        for (const Json& t : array_of(root_, "textures").items) {
+            const size_t src = index_of(t.get("source"), images.items.size(), "textures[].source");
+            if (!done[src]) {
+                decoded[src] = decode_image(images.items[src], src);
+                done[src] = true;
+            }
+            model_.textures.push_back(decoded[src]);
+        }
+    }
+
+    xrh_texture_data decode_image(const Json& im, size_t index) {
+        const std::string what = path_ + ": image " + std::to_string(index);
+        std::string bytes;
+        const Json* uri = im.get("uri");
+        if (uri && uri->kind == Json::kString) {
+            const std::string& u = uri->str;
+            if (u.compare(0, 5, "data:") == 0) {
+                const size_t comma = u.find(',');
+                if (comma == std::string::npos || u.find(";base64") > comma) fail(what + ": data URI is not base64");
+                bytes = base64_decode(u.data() + comma + 1, u.data() + u.size());
+            } else {
+                bytes = read_file(dir() + percent_decode(u));
+            }
+        } else if (const Json* bv = im.get("bufferView")) {
+            const Json& views = array_of(root_, "bufferViews");
+            const Json& view = views.items[index_of(bv, views.items.size(), "image.bufferView")];
+            const size_t bi = index_of(view.get("buffer"), buffers_.size(), "bufferView.buffer");
+            const double off = num_or(view.get("byteOffset"), 0.0, "bufferView.byteOffset");
+            const double len = num(view.get("byteLength"), "bufferView.byteLength");
+            if (off < 0.0 || len < 0.0 || off + len > (double)buffers_[bi].size()) fail(what + ": bufferView past its buffer");
+            bytes = buffers_[bi].substr((size_t)off, (size_t)len);
+        } else {
+            fail(what + ": neither uri nor bufferView");
+        }
+        xrh_texture_data t;
+        std::string err;
+        if (!xrh_internal::png_to_rgba(reinterpret_cast<const uint8_t*>(bytes.data()), bytes.size(), &t.width,
+                                       &t.height, &t.rgba8, &err))
+            fail(what + ": " + err + " (PNG images only; XRH_GLTF_UNTEXTURED loads the scene without them)");
+        return t;
     }
 
     // Float32 attribute data of an accessor: count * comps floats, tightly
@@ -715,7 +765,10 @@ int xrh_scene_gltf(const char* const* paths, uint32_t count, uint32_t flags, xrh
             s->vertices.insert(s->vertices.end(), m.vertices.begin(), m.vertices.end());
             s->indices.insert(s->indices.end(), m.indices.begin(), m.indices.end());
             s->materials.insert(s->materials.end(), m.materials.begin(), m.materials.end());
+            // textures in file order, indices not offset (Scene.cpp:56-63)
+            for (auto& t : m.textures) s->textures.push_back(std::move(t));
         }
+        for (const auto& t : s->textures) s->texture_desc.push_back(xr_texture{t.width, t.height, t.rgba8.data()});
     } catch (const LoadError& e) {
         set_err(err, err_len, e.msg);
         return XR_ERR_INVALID_ARGUMENT;

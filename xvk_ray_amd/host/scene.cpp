@@ -316,6 +316,13 @@ int xrh_scene_desc(const xrh_scene* s, xr_scene_desc* d) {
     return XR_OK;
 }
 
+int xrh_scene_textures(const xrh_scene* s, const xr_texture** textures, uint32_t* count) {
+    if (!s || !textures || !count) return XR_ERR_INVALID_ARGUMENT;
+    *textures = s->texture_desc.empty() ? nullptr : s->texture_desc.data();
+    *count = (uint32_t)s->texture_desc.size();
+    return XR_OK;
+}
+
 uint32_t xrh_scene_triangle_count(const xrh_scene* s) { return s ? (uint32_t)(s->indices.size() / 3) : 0; }
 
 void xrh_scene_destroy(xrh_scene* s) { delete s; }

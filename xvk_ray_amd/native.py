@@ -82,6 +82,10 @@ class XrSceneDesc(C.Structure):
                 ("offsets", C.c_void_p), ("instance_count", C.c_uint32)]
 
 
+class XrTexture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba8", C.c_void_p)]
+
+
 class XrRestirParams(C.Structure):
     _fields_ = [("temporal_m_cap", C.c_uint32), ("spatial_m_cap", C.c_uint32), ("spatial_samples", C.c_uint32),
                 ("spatial_radius", C.c_float), ("visibility", C.c_uint32), ("plain_max_depth", C.c_uint32),
@@ -130,11 +134,11 @@ XR_API_SYMBOLS = [
     "xr_batch_config_info", "xr_batch_set_config", "xr_batch_config", "xr_batch_autotune", "xr_batch_reset",
     "xr_batch_counters", "xr_batch_reset_counters", "xr_batch_buffer_bytes", "xr_batch_read_buffer",
     "xr_batch_destroy",
-    "xr_last_error", "xr_global_last_error",
+    "xr_last_error", "xr_global_last_error", "xr_set_textures",
 ]
 XR_HOST_SYMBOLS = [
-    "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_gltf", "xrh_scene_desc", "xrh_scene_triangle_count",
-    "xrh_scene_destroy",
+    "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_gltf", "xrh_scene_desc", "xrh_scene_textures",
+    "xrh_scene_triangle_count", "xrh_scene_destroy", "xrh_png_decode",
     "xrh_camera_static", "xrh_camera_orbit", "xrh_camera_view", "xrh_camera_destroy", "xrh_perspective",
     "xrh_inverse", "xrh_app_create", "xrh_app_draw_frame", "xrh_app_last_ubo", "xrh_app_ctx", "xrh_app_last_error",
     "xrh_app_destroy", "xrh_make_ubo", "xrh_png_encode", "xrh_write_png",
@@ -168,6 +172,7 @@ def lib() -> C.CDLL:
         L.xr_read_buffer.argtypes = [P, I, P, C.c_size_t, I]
         L.xr_write_buffer.argtypes = [P, I, P, C.c_size_t, I]
         L.xr_read_image.argtypes = [P, P, C.c_size_t]
+        L.xr_set_textures.argtypes = [P, P, U32]
         L.xr_buffer_bytes.argtypes = [P, I, I]
         L.xr_buffer_bytes.restype = C.c_size_t
         L.xr_trace_rays.argtypes = [P, P, U32, P, I]
@@ -230,6 +235,8 @@ def host() -> C.CDLL:
         H.xrh_scene_terrain.argtypes = [U64, U32, U32, U32, C.POINTER(P)]
         H.xrh_scene_gltf.argtypes = [C.POINTER(C.c_char_p), U32, U32, C.POINTER(P), C.c_char_p, C.c_size_t]
         H.xrh_png_encode.argtypes = [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t)]
+        H.xrh_png_decode.argtypes = [P, C.c_size_t, C.POINTER(U32), C.POINTER(U32), P, C.c_size_t]
+        H.xrh_scene_textures.argtypes = [P, C.POINTER(C.POINTER(XrTexture)), C.POINTER(U32)]
         H.xrh_write_png.argtypes = [C.c_char_p, P, U32, U32]
         H.xrh_scene_desc.argtypes = [P, C.POINTER(XrSceneDesc)]
         H.xrh_scene_triangle_count.argtypes = [P]
@@ -325,12 +332,14 @@ class Scene:
 
     @classmethod
     def from_arrays(cls, vertices: np.ndarray, indices: np.ndarray, materials: np.ndarray,
-                    offsets: np.ndarray) -> "Scene":
+                    offsets: np.ndarray, textures=()) -> "Scene":
+        """textures: [(width, height, RGBA8 uint8 array)], given to xr_set_textures by Context."""
         v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
         i = np.ascontiguousarray(indices, dtype=np.uint32)
         m = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
         o = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
-        return cls(arrays=(v, i, m, o))
+        t = [(int(w), int(h), np.ascontiguousarray(px, np.uint8).reshape(-1)) for w, h, px in textures]
+        return cls(arrays=(v, i, m, o, t))
 
     def desc(self) -> XrSceneDesc:
         d = XrSceneDesc()
@@ -339,7 +348,7 @@ class Scene:
             if rc:
                 raise XrError(rc, "xrh_scene_desc")
         else:
-            v, i, m, o = self._arrays
+            v, i, m, o = self._arrays[:4]
             d.vertices, d.vertex_count = v.ctypes.data, len(v)
             d.indices, d.index_count = i.ctypes.data, len(i)
             d.materials, d.material_count = m.ctypes.data, len(m)
@@ -349,7 +358,7 @@ class Scene:
     def arrays(self):
         """(vertices, indices, materials, offsets) as numpy views/copies."""
         if self._arrays is not None:
-            return self._arrays
+            return self._arrays[:4]
         d = self.desc()
 
         def view(ptr, n, ctype, dtype):
@@ -362,6 +371,23 @@ class Scene:
         m = view(d.materials, 32 * d.material_count, C.c_uint8, MATERIAL_DTYPE)
         o = view(d.offsets, 2 * d.instance_count, C.c_uint32, np.uint32)
         return v, i, m, o
+
+    def textures(self) -> list:
+        """[(width, height, RGBA8 uint8 array)] of a loaded glTF scene (empty otherwise)."""
+        if self._h is None:
+            return list(self._arrays[4]) if len(self._arrays) > 4 else []
+        ptr = C.POINTER(XrTexture)()
+        n = C.c_uint32()
+        rc = host().xrh_scene_textures(self._h, C.byref(ptr), C.byref(n))
+        if rc:
+            raise XrError(rc, "xrh_scene_textures")
+        out = []
+        for k in range(n.value):
+            t = ptr[k]
+            nb = 4 * t.width * t.height
+            px = np.ctypeslib.as_array((C.c_uint8 * nb).from_address(t.rgba8)).copy()
+            out.append((t.width, t.height, px))
+        return out
 
     @property
     def triangle_count(self) -> int:
@@ -452,8 +478,19 @@ class Context:
             raise XrError(rc, lib().xr_global_last_error().decode())
         self._h = h
         self.width, self.height = width, height
+        tex = scene.textures()
+        if tex:
+            self.set_textures(tex)
         if build:
             self.build_accel(builder)
+
+    def set_textures(self, textures) -> None:
+        """[(width, height, RGBA8 uint8 array)] -> xr_set_textures."""
+        self._tex_keep = [np.ascontiguousarray(px, np.uint8) for _, _, px in textures]
+        arr = (XrTexture * len(textures))()
+        for k, ((w, h, _), px) in enumerate(zip(textures, self._tex_keep)):
+            arr[k].width, arr[k].height, arr[k].rgba8 = w, h, px.ctypes.data
+        self.check(lib().xr_set_textures(self._h, arr, len(textures)))
 
     def build_accel(self, builder: int = XR_BUILDER_LBVH) -> None:
         self.check(lib().xr_build_accel(self._h, builder))
