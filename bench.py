@@ -68,10 +68,11 @@ def parse():
     ap.add_argument("--conv-spp", type=int, default=1024)
     ap.add_argument("--builder", choices=("lbvh", "ploc"), default="lbvh",
                     help="BVH builder (the build itself is outside the timed region)")
-    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c5", "scenes"), default="c2",
                     help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
                          "Cornell box; c3: 12 batched views of the 1080p scene (HIP graph, autotuned); "
-                         "c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain")
+                         "c5: 3840x2160 ReSTIR GI + FP8 MLP cache on the large terrain; scenes: the "
+                         "reference's own glTF scenes cb2 and sb2 at 1920x1080 (context for its README)")
     return ap.parse_args()
 
 
@@ -470,6 +471,71 @@ def run_c5(args) -> dict:
     return res
 
 
+GLTF_DIR = Path(__file__).resolve().parent / "tests" / "golden" / "gltf"
+# The reference's README (README.md:7-9): frame cost of three unnamed scenes
+# on unstated hardware, CPU wall time between frames; the ReSTIR GI columns are
+# the UI value, which divides by 3.
+README_MS = {"scene1": {"pt_1spp": 2.8, "restir_temporal": 2.9, "restir_temporal_spatial": 3.3},
+             "scene2": {"pt_1spp": 1.2, "restir_temporal": 1.7, "restir_temporal_spatial": 2.3},
+             "scene3": {"pt_1spp": 2.6, "restir_temporal": 2.8, "restir_temporal_spatial": 2.8}}
+
+
+def run_scenes(args) -> dict:
+    """The reference's own scenes through the glTF loader at 1920x1080 with the
+    reference app's settings (src/main.cpp:41-50: 10 bounces, fov 45, no
+    accumulation, no sky) at its default view (Scene.cpp:29): plain 1-spp path
+    tracing (unbounded depth, rgen.hlsl) and ReSTIR GI in reference mode
+    (M caps 30 / 300, 6 neighbours in 5 px, CheckVisibility always true).
+    cb2: 32 triangles, untextured; sb2: 75,277 triangles, PNG textures.  Wall
+    time per frame with the scene resident; the README's numbers are context
+    only (unnamed scenes, unstated hardware, CPU frame deltas)."""
+    from xvk_ray_amd import native as N
+
+    out = {}
+    total_rays, total_s = 0, 0.0
+    for name in ("cb2", "sb2"):
+        scene = N.Scene.gltf(GLTF_DIR / name / "scene.gltf")
+        camera = N.Camera.static((0.0, 1.0, 3.0), (0.0, 1.0, 0.0))
+        row = {"triangles": scene.triangle_count, "textures": len(scene.textures())}
+        for mode_name, mode in (("pt_1spp", N.XR_MODE_PLAIN), ("restir", N.XR_MODE_RESTIR)):
+            app = N.App(scene, camera, W, H, mode=mode, spp=1, bounces=10, fov=45.0, accumulate=False, sky=False,
+                        reuse=True)
+            ctx = app.context()
+            ctx.build_accel(builder_id(args))
+            params = N.default_params()
+            for _ in range(args.warmup):
+                app.draw_frame(params)
+            ctx.sync()
+            ctx.reset_counters()
+            ctx.reset_pass_times()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                app.draw_frame(params)
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            ext, shadow, _ = ctx.counters3()
+            pass_ms, nfr = ctx.pass_time_totals()
+            ms = wall * 1e3 / args.steps
+            entry = {"ms_per_frame": round(ms, 4), "rays_per_frame": (ext + shadow) // args.steps}
+            if mode == N.XR_MODE_RESTIR:
+                entry["pass_ms"] = {k: round(v / nfr, 4) for k, v in zip(("initial", "temporal", "spatial", "cache"),
+                                                                            pass_ms)}
+            info = ctx.accel_info()
+            row["bvh"] = {"builder": args.builder, "nodes": info.node_count, "depth": info.max_depth}
+            row[mode_name] = entry
+            total_rays += ext + shadow
+            total_s += wall
+            log(f"scenes: {name} {mode_name} {ms:.4f} ms/frame")
+            del ctx, app
+        out[name] = row
+    res = base_result(args, "ms/frame on the reference's glTF scenes @1920x1080 (context)",
+                      round(total_rays / total_s / 1e6, 2), "Mrays/s", round(total_s * 1e3 / (4 * args.steps), 4),
+                      "f32", {"workload": "reference_scenes_1080p", "width": W, "height": H, "scenes": ["cb2", "sb2"],
+                              "bounces": 10, "parallelism": "single_gpu"})
+    res.update({"scenes": out, "readme_ms_context": README_MS})
+    return res
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -485,6 +551,9 @@ def main():
         return
     if args.workload == "c3":
         print(json.dumps(run_c3(args)))
+        return
+    if args.workload == "scenes":
+        print(json.dumps(run_scenes(args)))
         return
 
     from xvk_ray_amd import native as N

@@ -58,6 +58,8 @@ class Oracle:
             L.orc_scene_create.restype = P
             L.orc_scene_set_textures.argtypes = [P, P, U32, P, C.c_size_t]
             L.orc_scene_set_textures.restype = C.c_int
+            L.orc_texture_sample.argtypes = [P, I, F, F, P]
+            L.orc_texture_sample.restype = None
             L.orc_scene_destroy.argtypes = [P]
             L.orc_scene_destroy.restype = None
             L.orc_trace32.argtypes = [P, P, U32, P, I]
@@ -126,6 +128,11 @@ class OracleScene:
             if self.L.orc_scene_set_textures(self.h, self.tex_info.ctypes.data, len(textures),
                                              self.texels.ctypes.data, self.texels.nbytes):
                 raise ValueError("oracle rejected the textures")
+
+    def texture_sample(self, t: int, u: float, v: float) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        self.L.orc_texture_sample(self.h, t, u, v, out.ctypes.data)
+        return out
 
     def close(self):
         if self.h:

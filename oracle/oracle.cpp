@@ -1073,6 +1073,12 @@ int orc_scene_set_textures(void* scene, const uint32_t* info, uint32_t count, co
     return 0;
 }
 
+// TextureSample of the scene's texture t at (u, v) (for tests).
+void orc_texture_sample(const void* scene, int t, float u, float v, float* rgb) {
+    const float3 c = TextureSample(((const Scene*)scene)->textures, t, float2{u, v});
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+
 void* orc_scene_create(const void* vertices, uint32_t vcount, const uint32_t* indices, uint32_t icount,
                        const void* materials, uint32_t mcount, const uint32_t* offsets, uint32_t ninst) {
     // reject what the reference would read out of bounds

@@ -19,7 +19,7 @@ import pytest
 from xvk_ray_amd import native as N
 
 PIL = pytest.importorskip("PIL.Image")
-REF_SB2 = Path("/root/reference/asset/model/sb2/scene.gltf")
+SB2 = Path(__file__).parent / "golden" / "gltf" / "sb2" / "scene.gltf"
 
 
 def _decode(data: bytes):
@@ -159,16 +159,55 @@ def test_gltf_non_png_image_is_refused(tmp_path):
     N.Scene.gltf(path, untextured=True)  # loads without images
 
 
-@pytest.mark.skipif(not REF_SB2.exists(), reason="the reference's sb2 asset is not on this machine")
 def test_reference_sb2_scene():
     """The reference's sb2 scene (src/main.cpp:56-68) with its PNG textures:
     75,277 triangles, every texture equal to Pillow's decoding of its image."""
-    scene = N.Scene.gltf(REF_SB2)
+    scene = N.Scene.gltf(SB2)
     assert scene.triangle_count == 75277
-    doc = json.loads(REF_SB2.read_text())
+    doc = json.loads(SB2.read_text())
     tex = scene.textures()
     assert len(tex) == len(doc["textures"])
     for (w, h, px), t in zip(tex, doc["textures"]):
         img = doc["images"][t["source"]]
-        ref = _pillow_rgba((REF_SB2.parent / img["uri"]).read_bytes())
+        ref = _pillow_rgba((SB2.parent / img["uri"]).read_bytes())
         assert np.array_equal(px.reshape(h, w, 4), ref)
+
+
+# ------------------------------------------------- the oracle's sampling ----
+def _oracle_with(textures):
+    from oracle.binding import OracleScene
+    v = np.zeros(3, N.VERTEX_DTYPE)
+    v["pos"] = [[0, 0, 1], [1, 0, 1], [0, 1, 1]]
+    m = np.zeros(1, N.MATERIAL_DTYPE)
+    m["base_color_texture_index"] = 0
+    return OracleScene(v, np.arange(3, dtype=np.uint32), m, np.array([0, 0], np.uint32), textures)
+
+
+def test_oracle_bilinear_closed_forms():
+    """Texel centres give the texel exactly (power-of-two sizes: no rounding in
+    u * w - 0.5), midpoints the mean of two or four texels, coordinates past
+    the edge the edge texel (clamp to edge), bytes read as byte / 255."""
+    rng = np.random.default_rng(9)
+    W, H = 8, 4
+    px = rng.integers(0, 256, size=(H, W, 4), dtype=np.uint8)
+    o = _oracle_with([(W, H, px.reshape(-1))])
+    t = px[..., :3].astype(np.float64) / 255.0
+    for j in range(H):
+        for i in range(W):
+            got = o.texture_sample(0, (i + 0.5) / W, (j + 0.5) / H)
+            assert np.array_equal(got, (px[j, i, :3] / np.float32(255)).astype(np.float32))
+    mid = o.texture_sample(0, 3.0 / W, 1.5 / H)  # halfway between texels 2 and 3 of row 1
+    assert np.allclose(mid, (t[1, 2] + t[1, 3]) / 2, atol=1e-6)
+    quad = o.texture_sample(0, 5.0 / W, 2.0 / H)  # the corner of texels (4..5, 1..2)
+    assert np.allclose(quad, (t[1, 4] + t[1, 5] + t[2, 4] + t[2, 5]) / 4, atol=1e-6)
+    assert np.allclose(o.texture_sample(0, -3.0, -0.2), t[0, 0], atol=1e-6)
+    assert np.allclose(o.texture_sample(0, 7.0, 0.5 / H), t[0, W - 1], atol=1e-6)
+    assert np.allclose(o.texture_sample(0, float("nan"), 1e30), t[H - 1, 0], atol=1e-6)
+    # a weighted mix at an arbitrary point, against the float64 formula
+    u, v = 0.3712, 0.6409
+    x, y = u * W - 0.5, v * H - 0.5
+    i0, j0 = int(np.floor(x)), int(np.floor(y))
+    a, b = x - i0, y - j0
+    want = ((1 - a) * (1 - b) * t[j0, i0] + a * (1 - b) * t[j0, i0 + 1] + (1 - a) * b * t[j0 + 1, i0]
+            + a * b * t[j0 + 1, i0 + 1])
+    assert np.allclose(o.texture_sample(0, u, v), want, atol=2e-6)

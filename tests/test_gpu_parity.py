@@ -188,6 +188,17 @@ def test_restir_cb2_gltf(vis):
     _frames(scene, 96, 64, 45.0, eyes, CB2_TARGET, 10, params, _strict)
 
 
+SB2 = Path(__file__).parent / "golden" / "gltf" / "sb2" / "scene.gltf"
+
+
+def test_restir_sb2_gltf_textured():
+    """The reference's sb2 scene (75,277 triangles, PNG textures) at its default view."""
+    scene = N.Scene.gltf(SB2)
+    assert scene.textures()
+    eyes = [CB2_EYE, (0.05, 1.0, 3.0)]
+    _frames(scene, 64, 48, 45.0, eyes, CB2_TARGET, 10, N.default_params(visibility=N.XR_VIS_TRACE), _strict)
+
+
 def test_plain_cb2_gltf(tmp_path):
     scene = N.Scene.gltf(CB2)
     W, H = 80, 48

@@ -195,9 +195,15 @@ __global__ void k_karras(const unsigned long long* __restrict__ keys, int n, int
 // Surface-area-heuristic constants of the leaf collapse: relative cost of
 // visiting a node (two slab tests, one 64-byte fetch) and of one triangle
 // test (one 48-byte fetch, the watertight test).
+#ifndef XR_SAH_TRI
+#define XR_SAH_TRI 1.0f
+#endif
+#ifndef XR_MAX_LEAF
+#define XR_MAX_LEAF 8
+#endif
 constexpr float kSahNode = 1.0f;
-constexpr float kSahTri = 1.0f;
-constexpr int kMaxLeafTris = 8;
+constexpr float kSahTri = XR_SAH_TRI;
+constexpr int kMaxLeafTris = XR_MAX_LEAF;
 
 __device__ __forceinline__ float box_area(const Box& b) {
     const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];

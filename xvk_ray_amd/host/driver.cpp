@@ -51,7 +51,12 @@ int xrh_app_create(const xrh_settings* st, const xrh_scene* scene, const xrh_cam
         return rc;
     }
     // CreateAccelerationStructures (pathTracer.cpp:39 -> ApplicationRT.cpp:55-66)
-    rc = xr_build_accel(a->ctx, XR_BUILDER_LBVH);
+    // the scene's textures (descriptor binding 8), when it has any
+    const xr_texture* tex = nullptr;
+    uint32_t ntex = 0;
+    rc = xrh_scene_textures(scene, &tex, &ntex);
+    if (rc == XR_OK && ntex) rc = xr_set_textures(a->ctx, tex, ntex);
+    if (rc == XR_OK) rc = xr_build_accel(a->ctx, XR_BUILDER_LBVH);
     if (rc != XR_OK) {
         a->err = xr_last_error(a->ctx);
         xr_destroy(a->ctx);
