@@ -66,10 +66,9 @@ def parse():
     ap.add_argument("--no-mlp-study", action="store_true")
     ap.add_argument("--study-frames", type=int, default=256)
     ap.add_argument("--conv-spp", type=int, default=1024)
-    ap.add_argument("--builder", choices=("lbvh", "ploc"), default=None,
-                    help="BVH builder (the build itself is outside the timed region); default LBVH for the "
-                         "synthetic workloads, PLOC for the glTF scenes (architectural models, where it "
-                         "builds markedly better trees)")
+    ap.add_argument("--builder", choices=("lbvh", "ploc", "sah"), default="sah",
+                    help="BVH builder (the build itself is outside the timed region): binned SAH on host "
+                         "threads (default: the best trees), or the GPU builders LBVH / PLOC")
     ap.add_argument("--workload", choices=("c2", "c1", "c3", "c5", "scenes"), default="c2",
                     help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
                          "Cornell box; c3: 12 batched views of the 1080p scene (HIP graph, autotuned); "
@@ -255,7 +254,7 @@ def log(msg: str) -> None:
 
 def builder_id(args) -> int:
     from xvk_ray_amd import native as N
-    return {"lbvh": N.XR_BUILDER_LBVH, "ploc": N.XR_BUILDER_PLOC}[args.builder or "lbvh"]
+    return {"lbvh": N.XR_BUILDER_LBVH, "ploc": N.XR_BUILDER_PLOC, "sah": N.XR_BUILDER_SAH}[args.builder]
 
 
 def base_result(args, metric, value, unit, ms, dtype, config) -> dict:
@@ -483,12 +482,6 @@ README_MS = {"scene1": {"pt_1spp": 2.8, "restir_temporal": 2.9, "restir_temporal
 
 
 def run_scenes(args) -> dict:
-    if args.builder is None:
-        args.builder = "ploc"
-    return _run_scenes(args)
-
-
-def _run_scenes(args) -> dict:
     """The reference's own scenes through the glTF loader at 1920x1080 with the
     reference app's settings (src/main.cpp:41-50: 10 bounces, fov 45, no
     accumulation, no sky) at its default view (Scene.cpp:29): plain 1-spp path
@@ -634,7 +627,7 @@ def main():
             "width": W, "height": H, "triangles": info.triangle_count, "max_depth": BOUNCES,
             "temporal_m_cap": PARAMS["temporal_m_cap"], "spatial_samples": PARAMS["spatial_samples"],
             "spatial_radius_px": PARAMS["spatial_radius"], "visibility": "traced",
-            "bvh": {"builder": args.builder or "lbvh", "nodes": info.node_count, "depth": info.max_depth,
+            "bvh": {"builder": args.builder, "nodes": info.node_count, "depth": info.max_depth,
                     "build_ms": round(info.build_ms, 2)},
             "parallelism": "single_gpu",
         },

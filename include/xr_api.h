@@ -182,10 +182,13 @@ int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ct
 
 /* Builds the acceleration structure (CreateAccelerationStructures,
  * ApplicationRT.cpp:55-66).  builder: XR_BUILDER_LBVH (Morton-order radix
- * tree, fastest build) or XR_BUILDER_PLOC (locally-ordered clustering,
- * higher tree quality, faster traversal). */
+ * tree on the GPU, fastest build), XR_BUILDER_PLOC (locally-ordered
+ * clustering on the GPU) or XR_BUILDER_SAH (binned surface-area heuristic,
+ * top down on host threads: the slowest build, usually the best trees).
+ * Every builder gives the same ray/triangle hits. */
 #define XR_BUILDER_LBVH 0
 #define XR_BUILDER_PLOC 1
+#define XR_BUILDER_SAH 2
 int xr_build_accel(xr_ctx* ctx, int builder);
 
 /* Base-colour textures (descriptor binding 8, TextureSamplers[],

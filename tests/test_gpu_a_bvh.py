@@ -65,7 +65,7 @@ def check_bvh(nodes: np.ndarray, tris: np.ndarray, tri_count: int) -> int:
     return max_depth
 
 
-@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC])
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH])
 @pytest.mark.parametrize("which", ["cornell", "terrain"])
 def test_bvh_structure(which, builder):
     scene = cornell_scene() if which == "cornell" else small_terrain()
@@ -88,7 +88,7 @@ def test_bvh_structure(which, builder):
     assert np.array_equal(canon, v["pos"][vi])
 
 
-@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC])
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH])
 def test_bvh_two_triangles_and_rebuild(builder):
     """Smallest clustered tree, and a rebuild of one context with the other builder."""
     verts = np.zeros(6, N.VERTEX_DTYPE)

@@ -30,7 +30,7 @@ def _max_lsb_diff(a: np.ndarray, b: np.ndarray) -> int:
 # ---------------------------------------------------------------------------
 # TraceRay
 # ---------------------------------------------------------------------------
-BUILDERS = [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC]
+BUILDERS = [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH]
 
 
 @pytest.mark.parametrize("builder", BUILDERS)

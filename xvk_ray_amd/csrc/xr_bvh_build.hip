@@ -24,7 +24,15 @@
 // and the linked nodes are stored in depth-first preorder (a node's left
 // child follows it), with no storage for nodes inside collapsed subtrees.
 #include <hipcub/hipcub.hpp>
+#include <math.h>
 #include <stdio.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <vector>
+
 #include "xr_kernels.h"
 
 namespace xr {
@@ -819,6 +827,335 @@ done:
     hipFree(tris);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Binned SAH, top down on the host (the driver builders behind the
+// reference's PREFER_FAST_TRACE flag, ApplicationRT.cpp:186-224, are of this
+// family).  The canonical triangles come from k_gather on the device; boxes
+// are padded as in k_refit; each node is split at the best of 32 centroid bins
+// per axis by SAH with the leaf-collapse constants above (a range of at most
+// kMaxLeafTris triangles becomes a leaf when that is cheaper); ranges whose
+// centroids fall in one bin are split at the centroid median of their widest
+// axis.  Nodes are stored in depth-first preorder (a node's left child follows
+// it) within each subtree; on scenes above kSahParallel triangles the top
+// levels are split on the calling thread and the subtrees below them built by
+// a thread pool and appended (each spliced subtree is contiguous, preorder).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kSahBins = 32;
+constexpr uint32_t kSahParallel = 1u << 16;
+
+float ord_dec_host(int o) {
+    const int i = o >= 0 ? o : o ^ 0x7FFFFFFF;
+    float f;
+    memcpy(&f, &i, sizeof(f));
+    return f;
+}
+
+struct HBox {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const HBox& b) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], b.lo[a]);
+            hi[a] = fmaxf(hi[a], b.hi[a]);
+        }
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct SahPrim {
+    HBox box;   // padded triangle box
+    float c[3]; // centroid
+};
+
+struct SahTree {
+    std::vector<BvhNode> nodes;
+    uint32_t depth = 0;  // edges on the longest root-to-leaf path
+};
+
+struct SahJob {
+    uint32_t b, e;  // triangle range [b, e) of ids
+    HBox box;
+    int parent;     // node index in the tree being built, -1 for its root
+    int side;
+    uint32_t depth; // edges from the tree's root to this job's node
+};
+
+void set_link(BvhNode& n, int side, int v) {
+    if (side == 0) n.d.x = v; else n.d.y = v;
+}
+
+void set_child_box(BvhNode& n, int side, const HBox& b) {
+    if (side == 0) {
+        n.a = make_float4(b.lo[0], b.hi[0], b.lo[1], b.hi[1]);
+        n.c.x = b.lo[2];
+        n.c.y = b.hi[2];
+    } else {
+        n.b = make_float4(b.lo[0], b.hi[0], b.lo[1], b.hi[1]);
+        n.c.z = b.lo[2];
+        n.c.w = b.hi[2];
+    }
+}
+
+// Splits ids[b, e) in place; returns the split position m (b < m < e) and
+// the two sides' boxes.
+uint32_t sah_split(const std::vector<SahPrim>& prims, std::vector<uint32_t>& ids, uint32_t b, uint32_t e,
+                   HBox* left, HBox* right) {
+    HBox cb;
+    for (uint32_t i = b; i < e; ++i) {
+        const SahPrim& p = prims[ids[i]];
+        for (int a = 0; a < 3; ++a) {
+            cb.lo[a] = fminf(cb.lo[a], p.c[a]);
+            cb.hi[a] = fmaxf(cb.hi[a], p.c[a]);
+        }
+    }
+    float best = INFINITY;
+    int best_axis = -1, best_bin = -1;
+    for (int a = 0; a < 3; ++a) {
+        const float ext = cb.hi[a] - cb.lo[a];
+        if (!(ext > 0.0f)) continue;
+        const float scale = (float)kSahBins / ext;
+        HBox bb[kSahBins];
+        uint32_t cnt[kSahBins] = {};
+        for (uint32_t i = b; i < e; ++i) {
+            const SahPrim& p = prims[ids[i]];
+            const int k = std::min(kSahBins - 1, (int)((p.c[a] - cb.lo[a]) * scale));
+            bb[k].grow(p.box);
+            ++cnt[k];
+        }
+        float right_area[kSahBins];
+        uint32_t right_cnt[kSahBins];
+        HBox acc;
+        uint32_t n = 0;
+        for (int k = kSahBins - 1; k > 0; --k) {
+            acc.grow(bb[k]);
+            n += cnt[k];
+            right_area[k] = acc.area();
+            right_cnt[k] = n;
+        }
+        acc = HBox();
+        n = 0;
+        for (int k = 0; k < kSahBins - 1; ++k) {
+            acc.grow(bb[k]);
+            n += cnt[k];
+            if (n == 0 || right_cnt[k + 1] == 0) continue;
+            const float c = (float)n * acc.area() + (float)right_cnt[k + 1] * right_area[k + 1];
+            if (c < best) {
+                best = c;
+                best_axis = a;
+                best_bin = k;
+            }
+        }
+    }
+    uint32_t m;
+    if (best_axis >= 0) {
+        const int a = best_axis;
+        const float scale = (float)kSahBins / (cb.hi[a] - cb.lo[a]);
+        const float lo = cb.lo[a];
+        auto it = std::partition(ids.begin() + b, ids.begin() + e, [&](uint32_t id) {
+            return std::min(kSahBins - 1, (int)((prims[id].c[a] - lo) * scale)) <= best_bin;
+        });
+        m = (uint32_t)(it - ids.begin());
+    } else {  // every centroid in one place: median of the widest vertex-box axis
+        HBox vb;
+        for (uint32_t i = b; i < e; ++i) vb.grow(prims[ids[i]].box);
+        int a = 0;
+        for (int k = 1; k < 3; ++k)
+            if (vb.hi[k] - vb.lo[k] > vb.hi[a] - vb.lo[a]) a = k;
+        m = b + (e - b) / 2;
+        std::nth_element(ids.begin() + b, ids.begin() + m, ids.begin() + e,
+                         [&](uint32_t x, uint32_t y) { return prims[x].c[a] < prims[y].c[a]; });
+    }
+    *left = HBox();
+    *right = HBox();
+    for (uint32_t i = b; i < m; ++i) left->grow(prims[ids[i]].box);
+    for (uint32_t i = m; i < e; ++i) right->grow(prims[ids[i]].box);
+    return m;
+}
+
+// Builds the subtree of ids[b, e) (an internal node: the caller has decided
+// not to make it a leaf) into `tree`, nodes numbered from 0.  With chunk > 0,
+// subtrees of at most `chunk` triangles are built by a pool of threads and
+// spliced in afterwards.
+void sah_subtree(const std::vector<SahPrim>& prims, std::vector<uint32_t>& ids, uint32_t b, uint32_t e,
+                 const HBox& box, SahTree* tree, uint32_t chunk) {
+    std::vector<SahJob> stack;
+    stack.push_back(SahJob{b, e, box, -1, 0, 0});
+    struct Deferred {
+        uint32_t b, e;
+        HBox box;
+        int parent, side;
+        uint32_t depth;
+        SahTree sub;
+    };
+    std::vector<Deferred> deferred;
+    while (!stack.empty()) {
+        SahJob j = stack.back();
+        stack.pop_back();
+        const uint32_t cnt = j.e - j.b;
+        const float area = j.box.area();
+        if (j.parent >= 0 && cnt <= (uint32_t)kMaxLeafTris) {
+            // leaf if cheaper than the best split (always for one triangle)
+            bool leaf = cnt == 1;
+            if (!leaf) {
+                std::vector<uint32_t> save(ids.begin() + j.b, ids.begin() + j.e);
+                HBox l, r;
+                const uint32_t m = sah_split(prims, ids, j.b, j.e, &l, &r);
+                const float split = kSahNode * area + kSahTri * ((float)(m - j.b) * l.area() + (float)(j.e - m) * r.area());
+                leaf = kSahTri * (float)cnt * area <= split;
+                std::copy(save.begin(), save.end(), ids.begin() + j.b);
+            }
+            if (leaf) {
+                set_link(tree->nodes[j.parent], j.side, ~(int)(j.b * 16u + (cnt - 1u)));
+                tree->depth = std::max(tree->depth, j.depth);
+                continue;
+            }
+        }
+        if (chunk && j.parent >= 0 && cnt <= chunk) {
+            // built on its own thread below, spliced in afterwards
+            deferred.push_back(Deferred{j.b, j.e, j.box, j.parent, j.side, j.depth, SahTree()});
+            continue;
+        }
+        const int k = (int)tree->nodes.size();
+        BvhNode nd;
+        memset(&nd, 0, sizeof(nd));
+        tree->nodes.push_back(nd);
+        if (j.parent >= 0) set_link(tree->nodes[j.parent], j.side, k);
+        HBox l, r;
+        const uint32_t m = sah_split(prims, ids, j.b, j.e, &l, &r);
+        set_child_box(tree->nodes[k], 0, l);
+        set_child_box(tree->nodes[k], 1, r);
+        stack.push_back(SahJob{m, j.e, r, k, 1, j.depth + 1});
+        stack.push_back(SahJob{j.b, m, l, k, 0, j.depth + 1});
+    }
+    if (deferred.empty()) return;
+    std::atomic<size_t> next{0};
+    const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < std::min<size_t>(hw, deferred.size()); ++t)
+        pool.emplace_back([&] {
+            for (size_t k = next++; k < deferred.size(); k = next++) {
+                Deferred& d = deferred[k];
+                sah_subtree(prims, ids, d.b, d.e, d.box, &d.sub, 0);
+            }
+        });
+    for (auto& t : pool) t.join();
+    for (auto& d : deferred) {
+        const int base = (int)tree->nodes.size();
+        set_link(tree->nodes[d.parent], d.side, base);
+        for (BvhNode n : d.sub.nodes) {
+            if (n.d.x >= 0) n.d.x += base;
+            if (n.d.y >= 0) n.d.y += base;
+            tree->nodes.push_back(n);
+        }
+        tree->depth = std::max(tree->depth, d.depth + d.sub.depth);
+    }
+}
+
+}  // namespace
+
+hipError_t build_sah(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                     const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count, hipStream_t st,
+                     BvhBuildResult* out, char* err, size_t err_len) {
+    if (tri_count < 2)
+        return build_lbvh(vertices, indices, offsets, inst_first_tri, instance_count, tri_count, st, out, err,
+                          err_len);
+    hipError_t rc = hipSuccess;
+    const uint32_t n = tri_count;
+    BvhTri* ctris = nullptr;
+    float4* cent = nullptr;
+    int* bounds = nullptr;
+    BvhNode* nodes = nullptr;
+    BvhTri* tris = nullptr;
+    const int init_bounds[12] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
+                                 (int)0x80000000, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000,
+                                 (int)0x80000000, (int)0x80000000};
+    int hb[12];
+    std::vector<BvhTri> h;
+    std::vector<SahPrim> prims;
+    std::vector<uint32_t> ids;
+    std::vector<BvhTri> sorted;
+    SahTree tree;
+    HBox all;
+    float pad = 0.0f, ext = 0.0f, mag = 0.0f;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        h.resize(n);
+        prims.resize(n);
+        ids.resize(n);
+        sorted.resize(n);
+    } catch (...) {
+        snprintf(err, err_len, "host memory for the SAH build");
+        return hipErrorOutOfMemory;
+    }
+    XR_TRY(hipMalloc(&ctris, sizeof(BvhTri) * n));
+    XR_TRY(hipMalloc(&cent, sizeof(float4) * n));
+    XR_TRY(hipMalloc(&bounds, sizeof(int) * 12));
+    XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
+                       instance_count, n, ctris, cent, bounds);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipMemcpyAsync(h.data(), ctris, sizeof(BvhTri) * n, hipMemcpyDeviceToHost, st));
+    XR_TRY(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    // box_pad, on the host
+    for (int a = 0; a < 3; ++a) {
+        const float lo = ord_dec_host(hb[6 + a]), hi = ord_dec_host(hb[9 + a]);
+        ext = fmaxf(ext, hi - lo);
+        mag = fmaxf(mag, fmaxf(fabsf(lo), fabsf(hi)));
+    }
+    pad = fmaxf(ext * (1.0f / 65536.0f), mag * (1.0f / 1048576.0f));
+    for (uint32_t i = 0; i < n; ++i) {
+        const BvhTri& t = h[i];
+        SahPrim& p = prims[i];
+        const float v[3][3] = {{t.v0.x, t.v0.y, t.v0.z}, {t.v1.x, t.v1.y, t.v1.z}, {t.v2.x, t.v2.y, t.v2.z}};
+        for (int a = 0; a < 3; ++a) {
+            p.box.lo[a] = fminf(fminf(v[0][a], v[1][a]), v[2][a]) - pad;
+            p.box.hi[a] = fmaxf(fmaxf(v[0][a], v[1][a]), v[2][a]) + pad;
+            p.c[a] = (v[0][a] + v[1][a] + v[2][a]) * (1.0f / 3.0f);
+        }
+        ids[i] = i;
+        all.grow(p.box);
+    }
+    try {
+        // subtrees handed to threads: about 256 of them on large scenes
+        const uint32_t chunk = n > kSahParallel ? std::max(n / 256u, 4096u) : 0u;
+        sah_subtree(prims, ids, 0, n, all, &tree, chunk);
+    } catch (...) {
+        snprintf(err, err_len, "SAH build failed (host memory or threads)");
+        goto fail;
+    }
+    for (uint32_t i = 0; i < n; ++i) sorted[i] = h[ids[i]];
+    XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * tree.nodes.size()));
+    XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * n));
+    XR_TRY(hipMemcpyAsync(nodes, tree.nodes.data(), sizeof(BvhNode) * tree.nodes.size(), hipMemcpyHostToDevice, st));
+    XR_TRY(hipMemcpyAsync(tris, sorted.data(), sizeof(BvhTri) * n, hipMemcpyHostToDevice, st));
+    XR_TRY(hipStreamSynchronize(st));
+    out->nodes = nodes;
+    out->tris = tris;
+    out->node_count = (uint32_t)tree.nodes.size();
+    out->tri_count = n;
+    out->max_depth = tree.depth;
+    out->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    nodes = nullptr;
+    tris = nullptr;
+    rc = hipSuccess;
+    goto done;
+fail:
+    rc = hipErrorUnknown;
+done:
+    hipFree(ctris);
+    hipFree(cent);
+    hipFree(bounds);
+    hipFree(nodes);
+    hipFree(tris);
     return rc;
 }
 

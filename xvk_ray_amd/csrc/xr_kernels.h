@@ -93,5 +93,9 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
 hipError_t build_ploc(const float* vertices, const uint32_t* indices, const uint2* offsets,
                       const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
                       hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
+// Binned SAH on the host (canonical triangles gathered on the device).
+hipError_t build_sah(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                     const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                     hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
 
 }  // namespace xr
