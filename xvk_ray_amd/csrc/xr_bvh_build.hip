@@ -834,7 +834,7 @@ done:
 // Binned SAH, top down on the host (the driver builders behind the
 // reference's PREFER_FAST_TRACE flag, ApplicationRT.cpp:186-224, are of this
 // family).  The canonical triangles come from k_gather on the device; boxes
-// are padded as in k_refit; each node is split at the best of 32 centroid bins
+// are padded as in k_refit; each node is split at the best of 64 centroid bins
 // per axis by SAH with the leaf-collapse constants above (a range of at most
 // kMaxLeafTris triangles becomes a leaf when that is cheaper); ranges whose
 // centroids fall in one bin are split at the centroid median of their widest
@@ -845,7 +845,9 @@ done:
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kSahBins = 32;
+// 64 bins per axis (32: 0.5 % slower frames, 16: 3 % slower; build time
+// grows a few percent per doubling).
+constexpr int kSahBins = 64;
 constexpr uint32_t kSahParallel = 1u << 16;
 
 float ord_dec_host(int o) {
