@@ -266,8 +266,9 @@ def _features(samples, mats, lo, inv, idx):
 @pytest.mark.parametrize("fp8", [False, True])
 def test_cache_frame_matches_oracle(fp8):
     """One cache frame: termination flags and every traced field bit-exact vs
-    the oracle; query pixels carry max(0, network(features)) computed with the
-    weights before the frame's training step (BF16 or FP8 weights)."""
+    the oracle; query and training pixels carry max(0, network(features))
+    computed with the weights after the frame's training step (BF16 or FP8
+    weights)."""
     scene = N.Scene.cornell(1)
     W, H = 64, 48
     ctx = N.Context(scene, W, H)
@@ -298,12 +299,15 @@ def test_cache_frame_matches_oracle(fp8):
     assert query.sum() > 0.1 * W * H and (kind == 2).sum() > 0
     assert counts[3] == query.sum() and counts[4] == (kind == 2).sum()
     # everything but the cached radiance is bit-exact
+    train = kind == 2
     g2, r2 = gpu.copy(), ref.copy()
-    g2["l_out"][query] = 0
+    g2["l_out"][query | train] = 0
+    r2["l_out"][train] = 0  # the traced suffix radiance, consumed by the training step
     eq = bytes_equal_rows(g2, r2)
     assert eq.all(), describe_mismatch("cache-frame samples", g2, r2, eq)
-    idx = np.nonzero(query)[0]
-    pred = np.maximum(B.mlp_forward(w_before, _features(ref, mats, lo, inv, idx), mode=2 if fp8 else 1), 0.0)
+    # query and training pixels carry the network's radiance after the step
+    idx = np.nonzero(query | train)[0]
+    pred = np.maximum(B.mlp_forward(w_after, _features(ref, mats, lo, inv, idx), mode=2 if fp8 else 1), 0.0)
     got = gpu["l_out"][idx].astype(np.float64)
     scale = max(1.0, np.abs(pred).max())
     assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())

@@ -657,12 +657,12 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
         if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, cache, c->stream));
         XR_HIP(c, hipEventRecord(ev[1], c->stream));
         if (cache_on && (passes & XR_PASS_INITIAL)) {
-            // infer with the current weights, then one training step on this
-            // frame's path suffixes
-            XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
+            // one training step on this frame's path suffixes, then the
+            // updated network's radiance for query and training pixels alike
             XR_HIP(c, launch_mlp_train_frame(c->mlp, fr, c->d_materials, p.mlp_loss_scale, c->stream));
             c->mlp.steps++;
             XR_HIP(c, launch_mlp_adam(c->mlp, p.mlp_lr, 1.0f / p.mlp_loss_scale, true, c->stream));
+            XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
         }
         XR_HIP(c, hipEventRecord(ev[2], c->stream));
         if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));

@@ -192,16 +192,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const 
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = fr.n;
-    // the initial pass listed this frame's query pixels
-    const uint32_t count = fr.cache_list[0];
-    const uint32_t* list = fr.cache_list + 2;
+    // the initial pass listed this frame's query pixels and training pixels;
+    // both take the network's prediction (training pixels after the step has
+    // read their traced radiance, as the training paths of neural radiance
+    // caching only feed the network)
+    const uint32_t nq = fr.cache_list[0], nt = fr.cache_list[1];
+    const uint32_t count = nq + nt;
     const uint32_t ntiles = (count + 63) / 64;
     for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
         const uint32_t idx = tile * 64 + lane;
-        const uint32_t pix = idx < count ? list[idx] : 0u;
+        const uint32_t kind = idx < nq ? 1u : 2u;
+        const uint32_t pix = idx < nq ? fr.cache_list[2 + idx] : (idx < count ? fr.cache_list[2 + n + (idx - nq)] : 0u);
         bool query = false;
         float f[kMlpIn];
-        if (idx < count) query = __float_as_uint(fr.samples[4 * n + pix].w) == 1u;
+        if (idx < count) query = __float_as_uint(fr.samples[4 * n + pix].w) == kind;
         if (query) pixel_features(m, fr.samples, n, pix, materials, f);
         else
             for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
