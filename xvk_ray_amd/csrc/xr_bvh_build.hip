@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <new>
 #include <chrono>
 #include <thread>
 #include <vector>
@@ -1039,16 +1040,26 @@ void sah_subtree(const std::vector<SahPrim>& prims, std::vector<uint32_t>& ids, 
     }
     if (deferred.empty()) return;
     std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
     const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
-    for (unsigned t = 0; t < std::min<size_t>(hw, deferred.size()); ++t)
-        pool.emplace_back([&] {
-            for (size_t k = next++; k < deferred.size(); k = next++) {
+    auto work = [&] {
+        try {
+            for (size_t k = next++; k < deferred.size() && !failed; k = next++) {
                 Deferred& d = deferred[k];
                 sah_subtree(prims, ids, d.b, d.e, d.box, &d.sub, 0);
             }
-        });
+        } catch (...) {  // out of host memory: reported by the caller
+            failed = true;
+        }
+    };
+    try {
+        for (unsigned t = 0; t < std::min<size_t>(hw, deferred.size()); ++t) pool.emplace_back(work);
+    } catch (...) {  // no more threads: the calling thread does the rest
+    }
+    work();
     for (auto& t : pool) t.join();
+    if (failed) throw std::bad_alloc();
     for (auto& d : deferred) {
         const int base = (int)tree->nodes.size();
         set_link(tree->nodes[d.parent], d.side, base);

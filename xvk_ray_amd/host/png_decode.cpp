@@ -30,7 +30,10 @@ class Inflater {
 public:
     Inflater(const uint8_t* p, size_t n) : p_(p), n_(n) {}
 
+    // expect: the size the image needs; a stream that inflates past it is
+    // refused rather than followed.
     std::vector<uint8_t> run(size_t expect) {
+        limit_ = expect;
         out_.reserve(expect);
         int last;
         do {
@@ -57,6 +60,11 @@ private:
     uint32_t buf_ = 0;
     int cnt_ = 0;
     std::vector<uint8_t> out_;
+    size_t limit_ = 0;
+
+    void room(size_t more) {
+        if (out_.size() + more > limit_) fail("deflate: more data than the image holds");
+    }
 
     uint32_t bits(int need) {
         uint32_t v = buf_;
@@ -113,6 +121,7 @@ private:
         pos_ += 4;
         if (len != (~nlen & 0xFFFFu)) fail("deflate: stored length mismatch");
         if (pos_ + len > n_) fail("deflate: short stored block");
+        room(len);
         out_.insert(out_.end(), p_ + pos_, p_ + pos_ + len);
         pos_ += len;
     }
@@ -130,6 +139,7 @@ private:
         for (;;) {
             int sym = decode(lit);
             if (sym < 256) {
+                room(1);
                 out_.push_back((uint8_t)sym);
                 continue;
             }
@@ -141,15 +151,15 @@ private:
             if (ds >= 30) fail("deflate: bad distance symbol");
             const size_t d = kDistBase[ds] + bits(kDistExtra[ds]);
             if (d > out_.size()) fail("deflate: distance too far back");
+            room(len);
             const size_t from = out_.size() - d;
             for (uint32_t k = 0; k < len; ++k) out_.push_back(out_[from + k]);
         }
     }
 
-    void fixed() {
-        static Huffman lit, dist;
-        static bool ready = false;
-        if (!ready) {
+    struct FixedCodes {
+        Huffman lit, dist;
+        FixedCodes() {
             uint8_t l[288];
             int s = 0;
             for (; s < 144; ++s) l[s] = 8;
@@ -160,9 +170,12 @@ private:
             uint8_t d[30];
             for (int k = 0; k < 30; ++k) d[k] = 5;
             build(&dist, d, 30);
-            ready = true;
         }
-        codes(lit, dist);
+    };
+
+    void fixed() {
+        static const FixedCodes c;  // thread-safe one-time initialisation
+        codes(c.lit, c.dist);
     }
 
     void dynamic() {
@@ -205,19 +218,21 @@ private:
     }
 };
 
-uint32_t crc32_of(const uint8_t* p, size_t n) {
-    static uint32_t table[256];
-    static bool ready = false;
-    if (!ready) {
+struct CrcTable {
+    uint32_t t[256];
+    CrcTable() {
         for (uint32_t k = 0; k < 256; ++k) {
             uint32_t c = k;
             for (int j = 0; j < 8; ++j) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[k] = c;
+            t[k] = c;
         }
-        ready = true;
     }
+};
+
+uint32_t crc32_of(const uint8_t* p, size_t n) {
+    static const CrcTable table;  // thread-safe one-time initialisation
     uint32_t c = 0xFFFFFFFFu;
-    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFFu] ^ (c >> 8);
+    for (size_t i = 0; i < n; ++i) c = table.t[(c ^ p[i]) & 0xFFu] ^ (c >> 8);
     return c ^ 0xFFFFFFFFu;
 }
 
@@ -273,7 +288,7 @@ Image decode_png(const uint8_t* data, size_t n) {
         }
         pos += 12 + (size_t)len;
     }
-    if (w == 0 || h == 0 || w > (1u << 16) || h > (1u << 16)) fail("PNG: image size out of range");
+    if (w == 0 || h == 0 || w > 16384 || h > 16384) fail("PNG: image size out of range (1..16384)");
     if (interlace != 0) fail("PNG: interlaced images are not read");
     int channels;
     switch (ctype) {
