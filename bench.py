@@ -47,8 +47,8 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 157270.2 + 23199.5, "temporal": 236381.8, "spatial": 737775.1}
-PMC_WRITE_KIB = {"initial": 711547.5 + 15439.6, "temporal": 215788.2, "spatial": 189815.3}
+PMC_FETCH_KIB = {"initial": 171835.3 + 22985.5, "temporal": 236424.2, "spatial": 806323.7}
+PMC_WRITE_KIB = {"initial": 750570.4 + 24241.3, "temporal": 215781.0, "spatial": 254193.8}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 

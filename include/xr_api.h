@@ -312,11 +312,16 @@ typedef struct xr_accel_info {
 int xr_get_accel_info(const xr_ctx* ctx, xr_accel_info* info);
 
 /* Copies the acceleration structure to host memory for validation: node_count
- * 64-byte nodes {float4 a, b, c; int4 d} with a = (c0.lo.x, c0.hi.x, c0.lo.y,
- * c0.hi.y), b = the same for child 1, c = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z),
- * d = (link0, link1, 0, 0) where a link >= 0 names a node and a link < 0 a
- * leaf ~(first * 16 + count - 1); and triangle_count 48-byte BVH-ordered
- * triangles {v0.xyz, canonical id; v1.xyz, instance; v2.xyz, primitive}. */
+ * 64-byte nodes of up to four children {float4 q; uint4 p; uint4 r; int4
+ * link}.  q.xyz is the node's grid origin and the bits of q.w hold the biased
+ * exponents e_x, e_y, e_z of the grid steps (step = 2^(e - 127)) in bytes 0-2
+ * and the child count in byte 3; child k's box is origin + g * step per axis
+ * with the planes g in byte k of p.x (lo.x), p.y (hi.x), p.z (lo.y), p.w
+ * (hi.y), r.x (lo.z), r.y (hi.z).  link[k] >= 0 names a node, link[k] < 0 a
+ * leaf ~(first * 16 + count - 1); slots past the child count are unused.
+ * Node 0 is the root and max_depth counts the nodes on the longest
+ * root-to-leaf path.  Triangles are triangle_count 48-byte BVH-ordered
+ * records {v0.xyz, canonical id; v1.xyz, instance; v2.xyz, primitive}. */
 int xr_read_accel(xr_ctx* ctx, void* nodes, size_t node_bytes, void* triangles, size_t triangle_bytes);
 
 /* HIP stream the context records on (a hipStream_t). */

@@ -1,5 +1,5 @@
 """Structure of the GPU-built BVH, checked on the host before any traversal
-test runs (this file is collected first).  SURVEY-style validity criteria:
+test runs (this file is collected first).  Validity criteria:
 every triangle sits in exactly one leaf, every node is reached exactly once,
 and each stored child box contains everything below it."""
 from __future__ import annotations
@@ -13,49 +13,75 @@ from xvk_ray_amd import native as N
 pytestmark = pytest.mark.gpu
 
 
+def decode_nodes(nodes: np.ndarray):
+    """(lo, hi, links, counts) of the 64-byte four-child nodes: lo / hi as
+    (n, 4, 3) float64 boxes of the children, decoded as origin + g * step."""
+    origin = nodes[:, 0:3].astype(np.float64)
+    meta = nodes[:, 3].copy().view(np.uint32)
+    exps = np.stack([(meta >> (8 * a)) & 0xFF for a in range(3)], axis=1).astype(np.int64)
+    step = np.ldexp(1.0, exps - 127)  # (n, 3)
+    counts = (meta >> 24).astype(np.int64)
+    planes = nodes[:, 4:10].copy().view(np.uint32)  # lox, hix, loy, hiy, loz, hiz
+    g = np.stack([(planes >> (8 * k)) & 0xFF for k in range(4)], axis=1).astype(np.float64)  # (n, 4, 6)
+    lo = origin[:, None, :] + g[:, :, 0::2] * step[:, None, :]
+    hi = origin[:, None, :] + g[:, :, 1::2] * step[:, None, :]
+    links = nodes[:, 12:16].copy().view(np.int32)
+    return lo, hi, links, counts
+
+
 def check_bvh(nodes: np.ndarray, tris: np.ndarray, tri_count: int) -> int:
-    links = nodes[:, 12:14].copy().view(np.int32)
+    """Walks the four-child tree from the root: every stored child box holds
+    every triangle below it, every triangle sits in exactly one leaf, every
+    node is reached once.  Returns the number of nodes on the longest
+    root-to-leaf path."""
+    assert nodes.shape[1] == 16
+    lo, hi, links, counts = decode_nodes(nodes)
     n_nodes = len(nodes)
     assert len(tris) == tri_count
-    tv = tris.reshape(-1, 3, 4)[:, :, :3]
+    assert ((counts >= 2) & (counts <= 4)).all(), "child counts"
+    tv = tris.reshape(-1, 3, 4)[:, :, :3].astype(np.float64)
     tlo, thi = tv.min(axis=1), tv.max(axis=1)
     covered = np.zeros(tri_count, np.int64)
     seen = np.zeros(n_nodes, np.int64)
     seen[0] = 1
-    boxes = {}
-
-    def child_box(i, side):
-        a, b, c = nodes[i, 0:4], nodes[i, 4:8], nodes[i, 8:12]
-        if side == 0:
-            return np.array([a[0], a[2], c[0]]), np.array([a[1], a[3], c[1]])
-        return np.array([b[0], b[2], c[2]]), np.array([b[1], b[3], c[3]])
-
-    # iterative DFS with depth
-    stack = [(0, 0)]
+    # subtree triangle bounds, children before parents (levels are stored in order)
+    sub_lo = np.full((n_nodes, 3), np.inf)
+    sub_hi = np.full((n_nodes, 3), -np.inf)
+    depth = np.zeros(n_nodes, np.int64)
+    depth[0] = 1
     max_depth = 0
+    order = []
+    stack = [0]
     while stack:
-        i, d = stack.pop()
-        for side in range(2):
-            link = int(links[i, side])
-            lo, hi = child_box(i, side)
+        i = stack.pop()
+        order.append(i)
+        for k in range(int(counts[i])):
+            link = int(links[i, k])
             if link >= 0:
                 assert link < n_nodes, f"node {i} links to {link}"
                 seen[link] += 1
-                assert seen[link] == 1 or tri_count == 1, f"node {link} reached twice"
-                # child's own children boxes must be inside this box
-                for s2 in range(2):
-                    l2, h2 = child_box(link, s2)
-                    assert np.all(l2 >= lo) and np.all(h2 <= hi), f"box of node {link} not contained"
-                stack.append((link, d + 1))
+                assert seen[link] == 1, f"node {link} reached twice"
+                depth[link] = depth[i] + 1
+                stack.append(link)
             else:
                 code = ~link
-                first, cnt = code >> 4, (code & 15) + 1
-                assert 0 <= first and first + cnt <= tri_count
-                covered[first:first + cnt] += 1
-                assert np.all(tlo[first:first + cnt] >= lo) and np.all(thi[first:first + cnt] <= hi)
-                max_depth = max(max_depth, d + 1)
-    # collapsed subtrees leave their internal nodes unlinked; no node is shared
-    assert (seen <= 1).all() or tri_count == 1, "shared nodes"
+                first, n = code >> 4, (code & 15) + 1
+                assert 0 <= first and first + n <= tri_count
+                covered[first:first + n] += 1
+                assert np.all(tlo[first:first + n] >= lo[i, k]) and np.all(thi[first:first + n] <= hi[i, k]), \
+                    f"leaf box {k} of node {i} does not hold its triangles"
+                sub_lo[i] = np.minimum(sub_lo[i], tlo[first:first + n].min(axis=0))
+                sub_hi[i] = np.maximum(sub_hi[i], thi[first:first + n].max(axis=0))
+                max_depth = max(max_depth, int(depth[i]))
+    for i in reversed(order):  # children are finished before their parent
+        for k in range(int(counts[i])):
+            link = int(links[i, k])
+            if link >= 0:
+                assert np.all(sub_lo[link] >= lo[i, k]) and np.all(sub_hi[link] <= hi[i, k]), \
+                    f"box {k} of node {i} does not hold the triangles below it"
+                sub_lo[i] = np.minimum(sub_lo[i], sub_lo[link])
+                sub_hi[i] = np.maximum(sub_hi[i], sub_hi[link])
+    assert (seen == 1).all(), "nodes not reached exactly once"
     if tri_count == 1:
         assert covered[0] == 2
     else:
@@ -75,8 +101,27 @@ def test_bvh_structure(which, builder):
     assert info.builder == builder
     nodes, tris = ctx.read_accel()
     depth = check_bvh(nodes, tris, scene.triangle_count)
-    # the builder reports the uncollapsed depth, an upper bound on the linked tree's
-    assert 0 < depth <= info.max_depth
+    assert depth == info.max_depth
+    assert info.node_bytes == 64 * len(nodes)
+    # the collapse opens inner children until a node has four: a node with
+    # fewer children has leaves only
+    lo, hi, links, counts = decode_nodes(nodes)
+    short = counts < 4
+    assert short.mean() < 0.9
+    assert (links[short][:, :2] < 0).all() and ((links[short][:, 2] < 0) | (counts[short] < 3)).all()
+    # the grid spans the children: per axis the lowest plane is 0 (the origin
+    # is the children's min corner) and, the step being the smallest power of
+    # two that reaches the max corner, the highest plane lies in the grid's
+    # upper half
+    planes = nodes[:, 4:10].copy().view(np.uint32)
+    g = np.stack([(planes >> (8 * k)) & 0xFF for k in range(4)], axis=1)  # (n, 4, 6)
+    live = np.arange(4)[None, :] < counts[:, None]
+    exps = np.stack([(nodes[:, 3].copy().view(np.uint32) >> (8 * a)) & 0xFF for a in range(3)], axis=1)
+    for a in range(3):
+        g_lo = np.where(live, g[:, :, 2 * a], 255).min(axis=1)
+        g_hi = np.where(live, g[:, :, 2 * a + 1], 0).max(axis=1)
+        assert (g_lo == 0).all()
+        assert (g_hi[exps[:, a] > 1] >= 127).all()
     # canonical triangle vertices match the scene's index buffer
     v, idx, _, off = scene.arrays()
     ids = tris[:, 3].copy().view(np.uint32)

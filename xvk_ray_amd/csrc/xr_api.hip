@@ -64,7 +64,7 @@ struct xr_ctx {
     int32_t max_texture_index = -1;  // largest index a material names
 
     // acceleration structure
-    BvhNode* d_nodes = nullptr;
+    WideNode* d_nodes = nullptr;
     BvhTri* d_tris = nullptr;
     bool built = false;
     xr_accel_info accel{};
@@ -554,20 +554,35 @@ int xr_build_accel(xr_ctx* c, int builder) {
         c->d_vertices, c->d_indices, c->d_offsets, c->d_inst_first, c->instance_count, c->tri_count, c->stream, &r,
         msg, sizeof(msg));
     if (e != hipSuccess) return set_err(c, XR_ERR_DEVICE, "BVH build failed: %s", msg);
-    c->d_nodes = r.nodes;
     c->d_tris = r.tris;
     if (r.max_depth > (uint32_t)kMaxTraversalDepth) {
-        dev_free(&c->d_nodes);
+        hipFree(r.nodes);
         dev_free(&c->d_tris);
         return set_err(c, XR_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack (%d)", r.max_depth,
                        kMaxTraversalDepth);
     }
+    // Four children per node where the tree's stack bound allows it; the
+    // binary tree's own bound is its depth, checked above, so width 2 fits.
+    WideBuildResult w;
+    for (int width = 4; width >= 2; --width) {
+        e = build_wide(r.nodes, r.node_count, width, c->stream, &w, msg, sizeof(msg));
+        if (e != hipSuccess) break;
+        if (w.stack_bound <= (uint32_t)kMaxTraversalDepth) break;
+        hipFree(w.nodes);
+        w.nodes = nullptr;
+    }
+    hipFree(r.nodes);
+    if (e != hipSuccess || !w.nodes) {
+        dev_free(&c->d_tris);
+        return set_err(c, XR_ERR_DEVICE, "wide BVH build failed: %s", msg);
+    }
+    c->d_nodes = w.nodes;
     c->accel.triangle_count = r.tri_count;
-    c->accel.node_count = r.node_count;
-    c->accel.max_depth = r.max_depth;
+    c->accel.node_count = w.node_count;
+    c->accel.max_depth = w.depth;
     c->accel.builder = (uint32_t)builder;
-    c->accel.build_ms = r.build_ms;
-    c->accel.node_bytes = (uint64_t)r.node_count * sizeof(BvhNode);
+    c->accel.build_ms = r.build_ms + w.build_ms;
+    c->accel.node_bytes = (uint64_t)w.node_count * sizeof(WideNode);
     c->accel.triangle_bytes = (uint64_t)r.tri_count * sizeof(BvhTri);
     c->built = true;
     return XR_OK;

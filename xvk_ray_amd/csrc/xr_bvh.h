@@ -3,10 +3,10 @@
 // Replaces the driver-side TraceRay of the reference (KHR ray tracing; BLAS
 // built in ApplicationRT.cpp:186-224, TLAS with identity instances and
 // CULL_DISABLE in XVKTLAccelerationStructure.cpp:68-93).  CDNA has no RT
-// cores, so traversal is plain VALU code: a binary BVH whose node holds both
-// child boxes (one 64-byte line per visited node), a per-lane traversal stack
-// kept in LDS, and the watertight ray/triangle test of Woop, Benthin and Wald
-// (JCGT 2013).
+// cores, so traversal is plain VALU code: a four-child BVH with quantised
+// child boxes (64 bytes per visited node, collapsed from the binary tree the
+// builders make), a per-lane traversal stack kept in LDS, and the watertight
+// ray/triangle test of Woop, Benthin and Wald (JCGT 2013).
 //
 // Closest-hit rule (order independent, so any BVH and the brute-force check
 // agree): the hit with the smallest t in [tmin, tmax]; equal t goes to the
@@ -32,6 +32,31 @@ struct alignas(16) BvhNode {
     int4 d;
 };
 static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte line");
+
+// The tree the kernels traverse: the binary tree above with up to four
+// children per node (each node opens its largest-area inner children until it
+// has four), in 64 bytes per node: the child boxes are quantised to 8 bits
+// per plane on a per-node grid (Ylitie, Karras and Laine, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017).
+//   q.xyz = grid origin (floats, the min corner of the union of the children)
+//   q.w   = biased exponents ex | ey << 8 | ez << 16 | child count << 24: the
+//           grid step on axis a is 2^(e_a - 127)
+//   p     = (lox, hix, loy, hiy): byte k holds child k's plane on that grid,
+//   r.xy  = (loz, hiz), r.zw unused
+//   link  = (link0 .. link3): >= 0 inner node, < 0 leaf ~(first*16 + count-1),
+//           unused slots (k >= child count) are never tested
+// A child's box is [origin + lo * step, origin + hi * step] per axis, rounded
+// outwards by the builder so it holds the child's padded box.  Traversal reads
+// four 16-byte words per node; its cost follows the bytes fetched per visit
+// (the 128-byte uncompressed form halved the visits of the binary tree and
+// gained 3 %), and the quantised node halves those bytes.
+struct alignas(64) WideNode {
+    float4 q;
+    uint4 p;
+    uint4 r;
+    int4 link;
+};
+static_assert(sizeof(WideNode) == 64, "wide node must be 64 bytes");
 
 // BVH-ordered triangle: three float4.  v0.w = canonical triangle id (bits),
 // v1.w = instance index (bits), v2.w = primitive index within the instance.

@@ -1172,4 +1172,234 @@ done:
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Wide nodes.  The binary tree is collapsed top-down, one level of wide nodes
+// per pass: each item (a binary node and the stack entries its ancestors can
+// leave) opens its children greedily, always the inner child with the largest
+// box surface, until it holds `width` children; a first kernel counts the
+// inner children, a scan gives every node's first child slot on the next
+// level, and a second kernel writes the node (child boxes rounded outwards
+// onto the node's 8-bit grid) and the next level's items.
+// Children of one node are consecutive, levels are stored one after another.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct WideSlots {
+    float lo[4][3], hi[4][3];
+    int link[4];
+    int count;
+};
+
+__device__ __forceinline__ void binary_child(const BvhNode& n, int side, float lo[3], float hi[3], int* link) {
+    if (side == 0) {
+        lo[0] = n.a.x; hi[0] = n.a.y; lo[1] = n.a.z; hi[1] = n.a.w; lo[2] = n.c.x; hi[2] = n.c.y;
+        *link = n.d.x;
+    } else {
+        lo[0] = n.b.x; hi[0] = n.b.y; lo[1] = n.b.z; hi[1] = n.b.w; lo[2] = n.c.z; hi[2] = n.c.w;
+        *link = n.d.y;
+    }
+}
+
+__device__ void open_wide(const BvhNode* __restrict__ bn, int b, int width, WideSlots* s) {
+    const BvhNode n = bn[b];
+    binary_child(n, 0, s->lo[0], s->hi[0], &s->link[0]);
+    binary_child(n, 1, s->lo[1], s->hi[1], &s->link[1]);
+    int cnt = 2;
+    while (cnt < width) {
+        int best = -1;
+        float area = -1.0f;
+        for (int k = 0; k < cnt; ++k) {
+            if (s->link[k] < 0) continue;
+            const float dx = s->hi[k][0] - s->lo[k][0], dy = s->hi[k][1] - s->lo[k][1],
+                        dz = s->hi[k][2] - s->lo[k][2];
+            const float a = dx * dy + dy * dz + dz * dx;
+            if (a > area) {
+                area = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        const BvhNode m = bn[s->link[best]];
+        binary_child(m, 0, s->lo[best], s->hi[best], &s->link[best]);
+        binary_child(m, 1, s->lo[cnt], s->hi[cnt], &s->link[cnt]);
+        ++cnt;
+    }
+    s->count = cnt;
+}
+
+// items[i] = (binary node, stack entries above it)
+__global__ void k_wide_count(const BvhNode* __restrict__ bn, const int2* __restrict__ items, int count, int width,
+                             int* __restrict__ inner) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    WideSlots s;
+    open_wide(bn, items[i].x, width, &s);
+    int m = 0;
+    for (int k = 0; k < s.count; ++k) m += s.link[k] >= 0 ? 1 : 0;
+    inner[i] = m;
+}
+
+// Grid of one axis of a wide node: origin lo and the smallest power-of-two
+// step with lo + 255 * step >= hi in float arithmetic; returns the step's
+// biased exponent (1..254, so the step is a normal float).
+__device__ uint32_t grid_exponent(float lo, float hi) {
+    uint32_t eb = 1;
+    const float ext = hi - lo;
+    if (ext > 0.0f) {
+        const uint32_t bits = __float_as_uint(ext * (1.0f / 255.0f));
+        eb = bits >> 23;
+        if (bits & 0x7FFFFFu) ++eb;  // round up to a power of two
+        if (eb < 1) eb = 1;
+    }
+    while (eb < 254 && lo + 255.0f * __uint_as_float(eb << 23) < hi) ++eb;
+    return eb;
+}
+
+// Grid planes of a child box, rounded outwards and checked in the arithmetic
+// that decodes them (origin + g * step; g * step is exact).
+__device__ uint32_t grid_below(float v, float origin, float step) {
+    float g = fminf(fmaxf(floorf((v - origin) / step), 0.0f), 255.0f);
+    while (g > 0.0f && origin + g * step > v) g -= 1.0f;
+    return (uint32_t)g;
+}
+__device__ uint32_t grid_above(float v, float origin, float step) {
+    float g = fminf(fmaxf(ceilf((v - origin) / step), 0.0f), 255.0f);
+    while (g < 255.0f && origin + g * step < v) g += 1.0f;
+    return (uint32_t)g;
+}
+
+__global__ void k_wide_emit(const BvhNode* __restrict__ bn, const int2* __restrict__ items, int count, int width,
+                            uint32_t self_base, uint32_t next_base, const int* __restrict__ inner,
+                            const int* __restrict__ incl, WideNode* __restrict__ out, int2* __restrict__ next,
+                            uint32_t* __restrict__ bound) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    WideSlots s;
+    open_wide(bn, items[i].x, width, &s);
+    const int acc = items[i].y + s.count - 1;
+    int slot = incl[i] - inner[i];
+    float nlo[3], nhi[3];
+    for (int a = 0; a < 3; ++a) {
+        nlo[a] = s.lo[0][a];
+        nhi[a] = s.hi[0][a];
+        for (int k = 1; k < s.count; ++k) {
+            nlo[a] = fminf(nlo[a], s.lo[k][a]);
+            nhi[a] = fmaxf(nhi[a], s.hi[k][a]);
+        }
+    }
+    uint32_t eb[3];
+    float step[3];
+    for (int a = 0; a < 3; ++a) {
+        eb[a] = grid_exponent(nlo[a], nhi[a]);
+        step[a] = __uint_as_float(eb[a] << 23);
+    }
+    uint32_t planes[6] = {0, 0, 0, 0, 0, 0};  // lox, hix, loy, hiy, loz, hiz; byte k = child k
+    int link[4];
+    for (int k = 0; k < 4; ++k) {
+        if (k < s.count) {
+            for (int a = 0; a < 3; ++a) {
+                planes[2 * a] |= grid_below(s.lo[k][a], nlo[a], step[a]) << (8 * k);
+                planes[2 * a + 1] |= grid_above(s.hi[k][a], nlo[a], step[a]) << (8 * k);
+            }
+            if (s.link[k] >= 0) {
+                next[slot] = make_int2(s.link[k], acc);
+                link[k] = (int)(next_base + (uint32_t)slot);
+                ++slot;
+            } else {
+                link[k] = s.link[k];
+            }
+        } else {
+            link[k] = 0;
+        }
+    }
+    WideNode w;
+    w.q = make_float4(nlo[0], nlo[1], nlo[2],
+                      __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16) | ((uint32_t)s.count << 24)));
+    w.p = make_uint4(planes[0], planes[1], planes[2], planes[3]);
+    w.r = make_uint4(planes[4], planes[5], 0u, 0u);
+    w.link = make_int4(link[0], link[1], link[2], link[3]);
+    out[self_base + (uint32_t)i] = w;
+    atomicMax(bound, (uint32_t)acc);
+}
+
+}  // namespace
+
+hipError_t build_wide(const BvhNode* binary, uint32_t binary_count, int width, hipStream_t st, WideBuildResult* out,
+                      char* err, size_t err_len) {
+    hipError_t rc = hipSuccess;
+    const uint32_t cap = binary_count > 0 ? binary_count : 1u;
+    WideNode* wide = nullptr;
+    WideNode* exact = nullptr;
+    int2* items[2] = {nullptr, nullptr};
+    int* inner = nullptr;
+    int* incl = nullptr;
+    uint32_t* bound = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    const int2 root = make_int2(0, 0);
+    uint32_t self_base = 0, total = 1, count = 1, depth = 0, host_bound = 0;
+    int cur = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (width < 2 || width > 4 || binary_count == 0) {
+        snprintf(err, err_len, "bad wide-tree request (width %d, %u nodes)", width, binary_count);
+        return hipErrorInvalidValue;
+    }
+    XR_TRY(hipMalloc(&wide, sizeof(WideNode) * cap));
+    XR_TRY(hipMalloc(&items[0], sizeof(int2) * cap));
+    XR_TRY(hipMalloc(&items[1], sizeof(int2) * cap));
+    XR_TRY(hipMalloc(&inner, sizeof(int) * cap));
+    XR_TRY(hipMalloc(&incl, sizeof(int) * cap));
+    XR_TRY(hipMalloc(&bound, sizeof(uint32_t)));
+    XR_TRY(hipMemsetAsync(bound, 0, sizeof(uint32_t), st));
+    XR_TRY(hipMemcpyAsync(items[0], &root, sizeof(int2), hipMemcpyHostToDevice, st));
+    XR_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, inner, incl, (int)cap, st));
+    XR_TRY(hipMalloc(&temp, temp_bytes));
+    while (count > 0) {
+        const uint32_t blocks = (count + 255) / 256;
+        int last = 0;
+        hipLaunchKernelGGL(k_wide_count, dim3(blocks), dim3(256), 0, st, binary, items[cur], (int)count, width, inner);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, inner, incl, (int)count, st));
+        hipLaunchKernelGGL(k_wide_emit, dim3(blocks), dim3(256), 0, st, binary, items[cur], (int)count, width,
+                           self_base, total, inner, incl, wide, items[cur ^ 1], bound);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipMemcpyAsync(&last, incl + (count - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        XR_TRY(hipStreamSynchronize(st));
+        ++depth;
+        // every wide node consumes at least one binary node of its own
+        if (last < 0 || (uint64_t)total + (uint32_t)last > cap || depth > cap) {
+            snprintf(err, err_len, "malformed binary tree (%u wide nodes at level %u)", total, depth);
+            goto fail;
+        }
+        self_base = total;
+        total += (uint32_t)last;
+        count = (uint32_t)last;
+        cur ^= 1;
+    }
+    XR_TRY(hipMalloc(&exact, sizeof(WideNode) * total));
+    XR_TRY(hipMemcpyAsync(exact, wide, sizeof(WideNode) * total, hipMemcpyDeviceToDevice, st));
+    XR_TRY(hipMemcpyAsync(&host_bound, bound, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    out->nodes = exact;
+    out->node_count = total;
+    out->depth = depth;
+    out->stack_bound = host_bound;
+    out->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    exact = nullptr;
+    rc = hipSuccess;
+    goto done;
+fail:
+    rc = hipErrorUnknown;
+done:
+    hipFree(wide);
+    hipFree(exact);
+    hipFree(items[0]);
+    hipFree(items[1]);
+    hipFree(inner);
+    hipFree(incl);
+    hipFree(bound);
+    hipFree(temp);
+    return rc;
+}
+
 }  // namespace xr

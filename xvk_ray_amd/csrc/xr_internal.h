@@ -16,7 +16,7 @@ struct SceneDev {
     const uint32_t* indices;    // binding 5
     const float4* materials;    // AMaterial as 2 float4 (binding 6)
     const uint2* offsets;       // (indexOffset, vertexOffset) per instance (binding 7)
-    const BvhNode* nodes;
+    const WideNode* nodes;      // four-child tree (xr_bvh.h)
     const BvhTri* tris;
     uint32_t material_count;
     const uint4* tex_info;    // (first texel, width, height, 0) per texture (binding 8)
@@ -65,8 +65,10 @@ XR_HD void mat_vec(const float* M, float v0, float v1, float v2, float v3, float
 // --------------------------------------------------------------------------
 // Traversal.  The stack lives in LDS, one column per lane
 // (stack[level * kBlock + lane] -> consecutive lanes hit consecutive banks);
-// levels beyond kLdsDepth spill to a private array.  The builder rejects trees
-// deeper than kLdsDepth + kSpillDepth, so the stack cannot overflow.
+// levels beyond kLdsDepth spill to a private array.  A ray never holds more
+// entries than the tree's stack bound (the largest sum of "children - 1" over
+// the nodes of a root-to-leaf path), and xr_build_accel rejects trees whose
+// bound exceeds kLdsDepth + kSpillDepth, so the stack cannot overflow.
 // --------------------------------------------------------------------------
 constexpr int kLdsDepth = 24;
 constexpr int kSpillDepth = 40;
@@ -121,6 +123,64 @@ __device__ __forceinline__ bool leaf_hits(const BvhTri* __restrict__ tris, const
     return false;
 }
 
+// Compare-exchange on (entry distance, link) pairs, nearer first.
+__device__ __forceinline__ void order2(float& ka, int& va, float& kb, int& vb) {
+    const bool s = kb < ka;
+    const float k = s ? kb : ka;
+    const int v = s ? vb : va;
+    kb = s ? ka : kb;
+    vb = s ? va : vb;
+    ka = k;
+    va = v;
+}
+
+// Four-element sorting network (5 compare-exchanges).
+__device__ __forceinline__ void order4(float& k0, int& v0, float& k1, int& v1, float& k2, int& v2, float& k3,
+                                       int& v3) {
+    order2(k0, v0, k1, v1);
+    order2(k2, v2, k3, v3);
+    order2(k0, v0, k2, v2);
+    order2(k1, v1, k3, v3);
+    order2(k1, v1, k2, v2);
+}
+
+// The four slab tests of a wide node (slots past the child count never hit).
+// On axis a the distance to grid plane g is (origin - o) / d + g * (step / d),
+// one fma from two terms formed once per node (step is a power of two, so
+// step / d is exact).  The roundings move a plane by a few ulp of the scene's
+// coordinates, far less than the padding the builders add to every box
+// (at least extent * 2^-16), so no child box that holds a hit is culled; the
+// far distance keeps the 1 + 2*gamma(3) scaling of the plain slab test.
+__device__ __forceinline__ void wide_slabs(const WideNode* __restrict__ nd, const RayPre& r, float tmin, float tb,
+                                           int4* lk, bool h[4], float e[4]) {
+    const float4 q = nd->q;
+    const uint4 p = nd->p;
+    const uint4 pz = nd->r;
+    *lk = nd->link;
+    const uint32_t meta = __float_as_uint(q.w);
+    const uint32_t cnt = meta >> 24;
+    const float ax = (q.x - r.org.x) * r.idir.x, bx = __uint_as_float((meta & 0xFFu) << 23) * r.idir.x;
+    const float ay = (q.y - r.org.y) * r.idir.y, by = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * r.idir.y;
+    const float az = (q.z - r.org.z) * r.idir.z, bz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * r.idir.z;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int sh = 8 * k;
+        const float t0x = fmaf((float)((p.x >> sh) & 0xFFu), bx, ax), t1x = fmaf((float)((p.y >> sh) & 0xFFu), bx, ax);
+        const float t0y = fmaf((float)((p.z >> sh) & 0xFFu), by, ay), t1y = fmaf((float)((p.w >> sh) & 0xFFu), by, ay);
+        const float t0z = fmaf((float)((pz.x >> sh) & 0xFFu), bz, az),
+                    t1z = fmaf((float)((pz.y >> sh) & 0xFFu), bz, az);
+        const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        tf = fminf(tf * 1.0000003576f, tb);
+        h[k] = tn <= tf && (uint32_t)k < cnt;
+        e[k] = tn;
+    }
+    // The links are needed only on some paths of the callers; without a use
+    // here the compiler sinks their load behind the slab tests, and every
+    // node visit waits for memory twice.
+    asm volatile("" ::"v"(lk->x), "v"(lk->y), "v"(lk->z), "v"(lk->w));
+}
+
 // Traversal for any-hit rays (trace_ray below picks the loop per kind of
 // ray).  kLds: stack levels kept in LDS (the rest, up to kMaxTraversalDepth,
 // in a private array); the batched kernels pick it by measurement.
@@ -132,11 +192,17 @@ __device__ __forceinline__ bool leaf_hits(const BvhTri* __restrict__ tris, const
 // nodes); then the wave intersects triangles together.  Without it a wave
 // runs the triangle loop in almost every iteration, for the few lanes that
 // sit on a leaf, while the others wait.  Leaf codes travel on the stack like
-// node indices.  The closest-hit rule is order independent and an any-hit
-// ray reports only whether something blocks it, so the results do not depend
-// on the visiting order.
+// node indices: a node's hit children are taken in slot order, the first next
+// and the others pushed (sorting them by entry distance, as the closest-hit
+// loop does, costs more than it saves on visibility rays: 1.57 against
+// 1.40 ms for the 1080p spatial pass).  The
+// closest-hit rule is order independent and an any-hit ray reports only
+// whether something blocks it, so the results do not depend on the visiting
+// order.  The tree's stack bound (the largest sum of "children - 1" along a
+// root-to-leaf path, checked when the tree is built) keeps sp within the
+// stack.
 template <int kBlock, bool kAnyHit, int kLds>
-__device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
+__device__ __forceinline__ bool trace_ray_ww(const WideNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
                                              const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
                                              Hit* hit) {
     // The bottom stack entry is a sentinel: popping it ends the ray.
@@ -144,6 +210,11 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
     int spill[kMaxTraversalDepth + 1 - kLds];
     lds_col[0] = kDone;
     int sp = 1;
+    auto push = [&](int v) {
+        if (sp < kLds) lds_col[sp * kBlock] = v;
+        else spill[sp - kLds] = v;
+        ++sp;
+    };
     int node = 0;  // >= 0: inner node, < 0: leaf code, kDone: finished
     int leaf = 0;  // parked leaf code, or 0
     float best_t = tmax;
@@ -160,23 +231,17 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
                 leaf = 0;
                 break;
             }
-            const float4 na = nodes[node].a;
-            const float4 nc = nodes[node].c;
-            const float4 nb = nodes[node].b;
-            const int4 nd = nodes[node].d;
-            const float tb = best_t * 1.0000003576f;
-            float e0, e1;
-            const bool h0 = slab_hit(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb, &e0);
-            const bool h1 = slab_hit(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb, &e1);
-            // nearer hit child next (the far one is pushed when both are hit)
-            const bool first1 = h1 && (!h0 || e1 < e0);
-            int next = first1 ? nd.y : nd.x;
-            if (h0 && h1) {
-                const int far = first1 ? nd.x : nd.y;
-                if (sp < kLds) lds_col[sp * kBlock] = far;
-                else spill[sp - kLds] = far;
-                ++sp;
-            } else if (!h0 && !h1) {
+            int4 lk;
+            bool h[4];
+            float e[4];
+            wide_slabs(nodes + node, r, tmin, best_t * 1.0000003576f, &lk, h, e);
+            int next = 0;
+            bool have = false;
+            if (h[3]) { next = lk.w; have = true; }
+            if (h[2]) { if (have) push(next); next = lk.z; have = true; }
+            if (h[1]) { if (have) push(next); next = lk.y; have = true; }
+            if (h[0]) { if (have) push(next); next = lk.x; have = true; }
+            if (!have) {
                 --sp;
                 next = stack_top<kBlock, kLds>(lds_col, spill, sp);
             }
@@ -207,18 +272,26 @@ __device__ __forceinline__ bool trace_ray_ww(const BvhNode* __restrict__ nodes, 
     hit->idx = best_idx;
     return best_idx != 0xFFFFFFFFu;
 }
-// Single loop: a lane intersects a leaf child's triangles as soon as it
-// tests the parent node (both leaf children, when both are hit, in one
-// triangle loop), then descends to the nearer hit inner child.
+// Single loop: a lane intersects the hit leaf children of a node as soon as
+// it tests the node (in slot order, in one triangle loop), then descends to
+// the nearest hit inner child and pushes the other hit inner children,
+// farthest deepest; inner children that a leaf hit has put beyond the new
+// closest t are dropped.
 template <int kBlock, bool kAnyHit, int kLds>
-__device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
+__device__ __forceinline__ bool trace_ray_inline(const WideNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
                                                  const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
                                                  Hit* hit) {
     // The bottom stack entry is a sentinel: popping it ends the ray.
     constexpr int kDone = 0x7FFFFFFF;
+    constexpr float kInf = __builtin_huge_valf();
     int spill[kMaxTraversalDepth + 1 - kLds];
     lds_col[0] = kDone;
     int sp = 1;
+    auto push = [&](int v) {
+        if (sp < kLds) lds_col[sp * kBlock] = v;
+        else spill[sp - kLds] = v;
+        ++sp;
+    };
     int node = 0;
     float best_t = tmax;
     uint32_t best_id = 0xFFFFFFFFu;
@@ -228,35 +301,37 @@ __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nod
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
     while (node != kDone && ++guard < kMaxNodeVisits) {
-        const float4 na = nodes[node].a;
-        const float4 nc = nodes[node].c;
-        const float4 nb = nodes[node].b;
-        const int4 nd = nodes[node].d;
-        const float tb = best_t * 1.0000003576f;
-        float e0, e1;
-        const bool h0 = slab_hit(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tb, &e0);
-        const bool h1 = slab_hit(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tb, &e1);
-        const bool l0 = h0 && nd.x < 0, l1 = h1 && nd.y < 0;
-        // (taking the nearer leaf first when both are hit measured slower)
-        int la = l0 ? nd.x : (l1 ? nd.y : 0);
-        int lb = (l0 && l1) ? nd.y : 0;
-        while (la < 0) {
-            if (leaf_hits<kAnyHit>(tris, r, tmin, la, &best_t, &best_id, &best_idx, &best_V, &best_W, &best_det, hit))
+        int4 lk;
+        bool h[4];
+        float e[4];
+        wide_slabs(nodes + node, r, tmin, best_t * 1.0000003576f, &lk, h, e);
+        int lm = (h[0] && lk.x < 0 ? 1 : 0) | (h[1] && lk.y < 0 ? 2 : 0) | (h[2] && lk.z < 0 ? 4 : 0) |
+                 (h[3] && lk.w < 0 ? 8 : 0);
+        while (lm) {
+            const int k = __builtin_ctz(lm);
+            lm &= lm - 1;
+            const int code = k == 0 ? lk.x : (k == 1 ? lk.y : (k == 2 ? lk.z : lk.w));
+            if (leaf_hits<kAnyHit>(tris, r, tmin, code, &best_t, &best_id, &best_idx, &best_V, &best_W, &best_det, hit))
                 return true;
-            la = lb;
-            lb = 0;
         }
-        const bool i0 = h0 && nd.x >= 0, i1 = h1 && nd.y >= 0;
-        const bool first1 = i1 && (!i0 || e1 < e0);
-        int next = first1 ? nd.y : nd.x;
-        if (i0 && i1) {
-            const int far = first1 ? nd.x : nd.y;
-            if (sp < kLds) lds_col[sp * kBlock] = far;
-            else spill[sp - kLds] = far;
-            ++sp;
-        } else if (!i0 && !i1) {
+        // The slab test against the updated closest t, as the node's own test
+        // would make it: no box holding a hit at t <= best_t is dropped.
+        const float tc = best_t * 1.0000003576f;
+        float k0 = (h[0] && lk.x >= 0 && e[0] <= tc) ? e[0] : kInf;
+        float k1 = (h[1] && lk.y >= 0 && e[1] <= tc) ? e[1] : kInf;
+        float k2 = (h[2] && lk.z >= 0 && e[2] <= tc) ? e[2] : kInf;
+        float k3 = (h[3] && lk.w >= 0 && e[3] <= tc) ? e[3] : kInf;
+        int c0 = lk.x, c1 = lk.y, c2 = lk.z, c3 = lk.w;
+        order4(k0, c0, k1, c1, k2, c2, k3, c3);
+        int next;
+        if (k0 == kInf) {
             --sp;
             next = stack_top<kBlock, kLds>(lds_col, spill, sp);
+        } else {
+            next = c0;
+            if (k3 != kInf) push(c3);
+            if (k2 != kInf) push(c2);
+            if (k1 != kInf) push(c1);
         }
         node = next;
     }
@@ -275,7 +350,7 @@ __device__ __forceinline__ bool trace_ray_inline(const BvhNode* __restrict__ nod
 // leaf's hit would have culled get visited first), so each kind of ray takes
 // the loop that suits it.
 template <int kBlock, bool kAnyHit, int kLds = kLdsDepth>
-__device__ __forceinline__ bool trace_ray(const BvhNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
+__device__ __forceinline__ bool trace_ray(const WideNode* __restrict__ nodes, const BvhTri* __restrict__ tris,
                                           const RayPre& r, float tmin, float tmax, int* __restrict__ lds_col,
                                           Hit* hit) {
     if constexpr (kAnyHit) return trace_ray_ww<kBlock, true, kLds>(nodes, tris, r, tmin, tmax, lds_col, hit);

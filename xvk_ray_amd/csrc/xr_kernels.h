@@ -98,4 +98,17 @@ hipError_t build_sah(const float* vertices, const uint32_t* indices, const uint2
                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
 
+// The traversed tree: the binary tree of a build collapsed top-down into
+// nodes of up to `width` (2..4) children.  `nodes` is a device allocation
+// owned by the caller after a successful call.
+struct WideBuildResult {
+    WideNode* nodes = nullptr;
+    uint32_t node_count = 0;
+    uint32_t depth = 0;        // wide nodes on the longest root-to-leaf path
+    uint32_t stack_bound = 0;  // largest sum of (children - 1) over such a path
+    float build_ms = 0.0f;
+};
+hipError_t build_wide(const BvhNode* binary, uint32_t binary_count, int width, hipStream_t st, WideBuildResult* out,
+                      char* err, size_t err_len);
+
 }  // namespace xr

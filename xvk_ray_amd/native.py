@@ -650,8 +650,10 @@ class Context:
         return info
 
     def read_accel(self):
-        """(nodes, triangles): nodes as (n, 16) float32 with links in columns 12-13 (int32 view),
-        triangles as (m, 12) float32 (w columns hold id / instance / primitive bits)."""
+        """(nodes, triangles): nodes as (n, 16) float32 in the layout xr_read_accel documents (grid
+        origin in columns 0-2, exponents and child count in the bits of column 3, plane bytes in
+        columns 4-9, links in columns 12-15); triangles as (m, 12) float32 (w columns hold id /
+        instance / primitive bits)."""
         info = self.accel_info()
         nodes = np.zeros((info.node_count, 16), np.float32)
         tris = np.zeros((info.triangle_count, 12), np.float32)
