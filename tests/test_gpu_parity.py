@@ -88,6 +88,67 @@ def test_trace_any_hit_matches(builder):
     assert np.array_equal(gpu["t"] >= 0, ref["t"] >= 0)
 
 
+def _mixed_scale_scene() -> N.Scene:
+    """A 2000-unit ground quad, 2000 millimetre triangles in a unit box far from
+    the origin, 200 triangles flat in x (zero extent on one axis) in the same
+    box and one large far triangle: node grids of very different steps."""
+    rng = np.random.default_rng(5)
+    c = np.array([500.0, 0.5, 500.0])
+    tris = [np.array([[-1000, 0, -1000], [1000, 0, -1000], [1000, 0, 1000]], np.float64),
+            np.array([[-1000, 0, -1000], [1000, 0, 1000], [-1000, 0, 1000]], np.float64),
+            np.array([[-900, 40, -900], [-800, 60, -900], [-900, 50, -800]], np.float64)]
+    base = c + rng.uniform(-0.5, 0.5, size=(2000, 3))
+    tris += list(base[:, None, :] + rng.uniform(-1e-3, 1e-3, size=(2000, 3, 3)))
+    flat = c + rng.uniform(-0.5, 0.5, size=(200, 3))
+    f = flat[:, None, :] + rng.uniform(-0.05, 0.05, size=(200, 3, 3))
+    f[:, :, 0] = 500.2
+    tris += list(f)
+    t = np.array(tris, np.float32)
+    v = np.zeros(3 * len(t), N.VERTEX_DTYPE)
+    v["pos"] = t.reshape(-1, 3)
+    v["normal"] = [0, 1, 0]
+    m = np.zeros(1, N.MATERIAL_DTYPE)
+    m["base_color"] = [0.5, 0.5, 0.5, 1]
+    m["base_color_texture_index"] = -1
+    return N.Scene.from_arrays(v, np.arange(len(v), dtype=np.uint32), m, np.array([0, 0], np.uint32))
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_trace_mixed_scales_axis_rays_infinite_tmax(builder):
+    """Closest hits against brute force over every triangle, and any-hit
+    flags, for rays from inside the millimetre cluster: random, axis-parallel
+    (two zero direction components), one zero component; tmin 0 or 1e-4,
+    tmax 1e4 or +inf."""
+    scene = _mixed_scale_scene()
+    ctx = N.Context(scene, 8, 8, builder=builder)
+    orc = oracle_scene(scene)
+    rng = np.random.default_rng(9)
+    n = 4000
+    rays = np.zeros(n, N.RAY_DTYPE)
+    rays["origin"] = (np.array([500.0, 0.5, 500.0]) + rng.uniform(-0.6, 0.6, size=(n, 3))).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    q = n // 4
+    axis = rng.integers(0, 3, size=q)
+    d[q:2 * q] = 0.0
+    d[np.arange(q, 2 * q), axis] = rng.choice([-1.0, 1.0], size=q)
+    d[np.arange(2 * q, 3 * q), rng.integers(0, 3, size=q)] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["direction"] = d.astype(np.float32)
+    rays["tmin"] = np.where(rng.random(n) < 0.5, 0.0, 1e-4)
+    rays["tmax"] = np.where(rng.random(n) < 0.5, np.inf, 1e4)
+    say(f"[trace-mixed] {scene.triangle_count} tris, builder {builder}")
+    gpu = ctx.trace(rays)
+    ref = orc.trace32(rays, brute=True)
+    eq = bytes_equal_rows(gpu, ref)
+    assert eq.all(), describe_mismatch("closest hit", gpu, ref, eq)
+    assert 0.2 < (gpu["t"] > 0).mean() < 0.95
+    seg = rays.copy()
+    seg["tmax"] = np.where(np.isinf(seg["tmax"]), np.inf, 0.05)
+    gany = ctx.trace(seg, any_hit=True)
+    rany = orc.trace32(seg, any_hit=True)
+    assert np.array_equal(gany["t"] >= 0, rany["t"] >= 0)
+
+
 # ---------------------------------------------------------------------------
 # ReSTIR GI frame, pass by pass
 # ---------------------------------------------------------------------------
