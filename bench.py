@@ -47,8 +47,8 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 171835.3 + 22985.5, "temporal": 236424.2, "spatial": 806323.7}
-PMC_WRITE_KIB = {"initial": 750570.4 + 24241.3, "temporal": 215781.0, "spatial": 254193.8}
+PMC_FETCH_KIB = {"initial": 171738.3 + 22955.2, "temporal": 236412.3, "spatial": 787485.4}
+PMC_WRITE_KIB = {"initial": 750515.3 + 24225.4, "temporal": 215782.1, "spatial": 253961.1}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -106,7 +106,10 @@ def mlp_study(scene, camera, args) -> dict:
     whether the cache is what makes the cheaper setting reach that error.  The
     error compared is the mean RMSE over the last 8 frames; the equal-RMSE
     time ratio is the cheapest cache setting at or below the reference's
-    error, divided by the reference's time."""
+    error, divided by the reference's time.  The comparison is repeated with
+    the visibility check as the reference ships it (always visible:
+    `reference_visibility`), plain ReSTIR GI at 5 neighbours against the cache
+    form at 5, 3 and 2."""
     import numpy as np
 
     from xvk_ray_amd import native as N
@@ -128,12 +131,12 @@ def mlp_study(scene, camera, args) -> dict:
     conv = ctx.converged_mean()
     conv_s = time.perf_counter() - t0
 
-    def run(cache: bool, spread: float, neighbours: int) -> dict:
+    def run(cache: bool, spread: float, neighbours: int, vis: int = N.XR_VIS_TRACE) -> dict:
         ctx.resize(W, H)  # empty history
         if cache:
             ctx.mlp_reset(1)
         flags = N.XR_FLAG_RADIANCE | (N.XR_FLAG_MLP_CACHE if cache else 0)
-        p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, mlp_spread=spread,
+        p = N.default_params(visibility=vis, flags=flags, mlp_spread=spread,
                              **dict(PARAMS, spatial_samples=neighbours))
         ctx.reset_counters()
         ctx.reset_pass_times()
@@ -162,7 +165,9 @@ def mlp_study(scene, camera, args) -> dict:
                         "train_samples_per_frame": train // nfr,
                         "mlp": {"gflop_per_frame": round(flop / 1e9, 3), "achieved_TFLOPs": round(tflops, 2),
                                 "frac_mfma_bf16": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)}})
-        log(f"{'cache c=' + str(spread) if cache else 'plain'} K={neighbours}: {ms:.3f} ms/frame, "
+        if vis != N.XR_VIS_TRACE:
+            out["visibility"] = "reference"
+        log(f"{'cache c=' + str(spread) if cache else 'plain'} K={neighbours} vis={vis}: {ms:.3f} ms/frame, "
             f"rmse {rmse}, last-8 mean {out['rmse_last8']:.4f}")
         return out
 
@@ -171,6 +176,12 @@ def mlp_study(scene, camera, args) -> dict:
     sweep = [run(True, c, k) for k in STUDY_NEIGHBOURS for c in SPREADS]
     ok = [r for r in sweep if r["rmse_last8"] <= plain["rmse_last8"]]
     best = min(ok, key=lambda r: r["ms_per_frame"]) if ok else None
+    # The same comparison with the visibility check as the reference ships it
+    # (always visible): its own plain ReSTIR GI against the cache form.
+    ref_plain = run(False, 0.0, PARAMS["spatial_samples"], N.XR_VIS_REFERENCE)
+    ref_cache = [run(True, SPREADS[0], k, N.XR_VIS_REFERENCE) for k in (PARAMS["spatial_samples"], 3, 2)]
+    ref_ok = [r for r in ref_cache if r["rmse_last8"] <= ref_plain["rmse_last8"]]
+    ref_best = min(ref_ok, key=lambda r: r["ms_per_frame"]) if ref_ok else None
     ctx.close()
     return {
         "camera": "orbit path frame 0, held still", "frames": frames, "converged_spp": args.conv_spp,
@@ -180,6 +191,11 @@ def mlp_study(scene, camera, args) -> dict:
         "equal_rmse_setting": ({"spread_c": best["spread_c"], "spatial_samples": best["spatial_samples"]}
                                if best else None),
         "target_ratio": 0.6,
+        "reference_visibility": {
+            "plain": ref_plain, "cache": ref_cache,
+            "time_ratio_at_equal_rmse": (round(ref_best["ms_per_frame"] / ref_plain["ms_per_frame"], 4)
+                                         if ref_best else None),
+        },
     }
 
 
