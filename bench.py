@@ -613,7 +613,8 @@ def main():
         ms = pass_ms[k] / frames
         gbs = ALGO_BYTES[name] * W * H / (ms * 1e-3) / 1e9
         per_pass[name] = {"ms": round(ms, 4), "algo_bytes_per_px": ALGO_BYTES[name], "achieved_GBs": round(gbs, 1),
-                          "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+                          "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                          "hbm_GBs_measured": round(pmc_traffic(name) / (ms * 1e-3) / 1e9, 1)}
     dom = max(names, key=lambda k: per_pass[k]["ms"])
     roofline = {
         "kernel": {"initial": "k_initial_sample", "temporal": "k_temporal", "spatial": "k_spatial"}[dom],
@@ -655,6 +656,9 @@ def main():
         "passes": per_pass,
         "temporal_plus_spatial_ms": round(per_pass["temporal"]["ms"] + per_pass["spatial"]["ms"], 4),
         "roofline": roofline,
+        # registers, scratch, LDS and resident waves of the traversal kernels
+        "kernels": {k["name"]: {f: k[f] for f in ("vgprs", "scratch_bytes", "lds_bytes", "waves_per_simd")}
+                    for k in N.kernel_info() if k["name"] in ("k_initial_sample", "k_initial_tail", "k_spatial")},
         "setup_s": round(setup_s, 2),
     }
     if not args.no_mlp_study:

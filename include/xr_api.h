@@ -324,6 +324,25 @@ int xr_get_accel_info(const xr_ctx* ctx, xr_accel_info* info);
  * records {v0.xyz, canonical id; v1.xyz, instance; v2.xyz, primitive}. */
 int xr_read_accel(xr_ctx* ctx, void* nodes, size_t node_bytes, void* triangles, size_t triangle_bytes);
 
+/* Resources of the frame kernels (no reference counterpart; measurement
+ * support): index 0..xr_kernel_count()-1 names k_initial_sample,
+ * k_initial_tail, k_temporal, k_spatial, k_plain, k_converge.  vgprs and
+ * scratch_bytes (per lane) and lds_bytes (per workgroup) as the runtime's
+ * function attributes report them; blocks_per_cu resident workgroups of
+ * `block` lanes per CU, waves_per_simd = blocks_per_cu * block / 256.
+ * Needs a device: XR_ERR_DEVICE without one. */
+typedef struct xr_kernel_info {
+    char name[32];
+    uint32_t vgprs;
+    uint32_t scratch_bytes;
+    uint32_t lds_bytes;
+    uint32_t block;
+    uint32_t blocks_per_cu;
+    uint32_t waves_per_simd;
+} xr_kernel_info;
+uint32_t xr_kernel_count(void);
+int xr_kernel_info_get(uint32_t index, xr_kernel_info* out);
+
 /* HIP stream the context records on (a hipStream_t). */
 void* xr_stream(xr_ctx* ctx);
 

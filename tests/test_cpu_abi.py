@@ -85,6 +85,20 @@ def test_create_without_device_reports_device_error():
     assert rc == N.XR_ERR_DEVICE and not h.value and msg
 
 
+def test_kernel_info_arguments():
+    L = N.lib()
+    assert L.xr_kernel_count() == 6
+    k = N.XrKernelInfo()
+    assert L.xr_kernel_info_get(99, C.byref(k)) == N.XR_ERR_INVALID_ARGUMENT
+    assert L.xr_kernel_info_get(0, None) == N.XR_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_kernel_info_without_device():
+    k = N.XrKernelInfo()
+    assert N.lib().xr_kernel_info_get(0, C.byref(k)) == N.XR_ERR_DEVICE
+
+
 def test_null_context_calls_are_safe():
     L = N.lib()
     L.xr_destroy(None)

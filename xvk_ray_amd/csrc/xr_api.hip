@@ -382,6 +382,32 @@ const char* xr_last_error(const xr_ctx* ctx) { return ctx ? ctx->err.c_str() : g
 
 void* xr_stream(xr_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+uint32_t xr_kernel_count(void) { return (uint32_t)kKernelInfoCount; }
+
+int xr_kernel_info_get(uint32_t index, xr_kernel_info* out) {
+    if (!out) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (index >= (uint32_t)kKernelInfoCount)
+        return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "kernel index %u out of range (%d kernels)", index,
+                       kKernelInfoCount);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(nullptr, XR_ERR_DEVICE, "no HIP device");
+    const char* name = nullptr;
+    hipFuncAttributes a;
+    int blocks = 0, block = 0;
+    const hipError_t e = kernel_resources(index, &name, &a, &blocks, &block);
+    if (e != hipSuccess) return set_err(nullptr, XR_ERR_DEVICE, "kernel attributes: %s", hipGetErrorString(e));
+    memset(out, 0, sizeof(*out));
+    snprintf(out->name, sizeof(out->name), "%s", name);
+    out->vgprs = (uint32_t)a.numRegs;
+    out->scratch_bytes = (uint32_t)a.localSizeBytes;
+    out->lds_bytes = (uint32_t)a.sharedSizeBytes;
+    out->block = (uint32_t)block;
+    out->blocks_per_cu = (uint32_t)blocks;
+    out->waves_per_simd = (uint32_t)(blocks * block / 256);
+    return XR_OK;
+}
+
 void xr_destroy(xr_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);

@@ -36,6 +36,11 @@ hipError_t launch_sq_error(const float4* img, const double* acc, double inv_spp,
                            hipStream_t st);
 hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, float4* hits, bool any_hit,
                         hipStream_t st);
+// Resources of frame kernel `index` (0 .. kKernelInfoCount-1): name, the
+// runtime's attributes, resident workgroups per CU and the workgroup size.
+constexpr int kKernelInfoCount = 6;
+hipError_t kernel_resources(uint32_t index, const char** name, hipFuncAttributes* attr, int* blocks_per_cu,
+                            int* block);
 
 // MLP radiance cache state: one device allocation of mlp_bytes() bytes carved
 // by mlp_bind into 256-byte-aligned regions.

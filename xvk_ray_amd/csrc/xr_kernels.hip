@@ -1129,4 +1129,31 @@ hipError_t launch_trace(const SceneDev& sc, const float4* rays, uint32_t n, floa
     return hipGetLastError();
 }
 
+
+// Resources of the frame kernels as the runtime reports them (registers,
+// scratch and LDS per workgroup, resident workgroups per CU), for the
+// measurement output.
+hipError_t kernel_resources(uint32_t index, const char** name, hipFuncAttributes* attr, int* blocks_per_cu,
+                            int* block) {
+    struct Entry {
+        const char* name;
+        const void* fn;
+        int block;
+    };
+    static const Entry table[kKernelInfoCount] = {
+        {"k_initial_sample", reinterpret_cast<const void*>(&k_initial_sample), kBlock},
+        {"k_initial_tail", reinterpret_cast<const void*>(&k_initial_tail), kBlock},
+        {"k_temporal", reinterpret_cast<const void*>(&k_temporal), 256},
+        {"k_spatial", reinterpret_cast<const void*>(&k_spatial), kBlock},
+        {"k_plain", reinterpret_cast<const void*>(&k_plain), kBlock},
+        {"k_converge", reinterpret_cast<const void*>(&k_converge), kBlock},
+    };
+    if (index >= (uint32_t)kKernelInfoCount) return hipErrorInvalidValue;
+    *name = table[index].name;
+    *block = table[index].block;
+    hipError_t e = hipFuncGetAttributes(attr, table[index].fn);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, table[index].fn, table[index].block, 0);
+}
+
 }  // namespace xr

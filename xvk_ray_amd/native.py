@@ -100,6 +100,12 @@ class XrAccelInfo(C.Structure):
                 ("triangle_bytes", C.c_uint64)]
 
 
+class XrKernelInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("vgprs", C.c_uint32), ("scratch_bytes", C.c_uint32),
+                ("lds_bytes", C.c_uint32), ("block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("waves_per_simd", C.c_uint32)]
+
+
 class XrhSettings(C.Structure):
     _fields_ = [("rendering_mode", C.c_uint32), ("spp", C.c_uint32), ("number_of_bounces", C.c_uint32),
                 ("max_spp", C.c_uint32), ("fov", C.c_float), ("enable_ray_accumulation", C.c_uint32),
@@ -127,7 +133,7 @@ XR_API_SYMBOLS = [
     "xr_default_params", "xr_create", "xr_build_accel", "xr_resize", "xr_render", "xr_sync", "xr_destroy",
     "xr_read_buffer", "xr_write_buffer", "xr_read_image", "xr_buffer_bytes", "xr_trace_rays",
     "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
-    "xr_reset_counters", "xr_get_accel_info",
+    "xr_reset_counters", "xr_get_accel_info", "xr_kernel_count", "xr_kernel_info_get",
     "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
     "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
     "xr_mlp_fp8_weights", "xr_mlp_params",
@@ -189,6 +195,9 @@ def lib() -> C.CDLL:
         L.xr_counters.argtypes = [P, C.POINTER(C.c_uint64), U32]
         L.xr_reset_counters.argtypes = [P]
         L.xr_get_accel_info.argtypes = [P, C.POINTER(XrAccelInfo)]
+        L.xr_kernel_count.argtypes = []
+        L.xr_kernel_count.restype = U32
+        L.xr_kernel_info_get.argtypes = [U32, C.POINTER(XrKernelInfo)]
         L.xr_read_accel.argtypes = [P, P, C.c_size_t, P, C.c_size_t]
         L.xr_stream.argtypes = [P]
         L.xr_stream.restype = P
@@ -276,6 +285,21 @@ class XrError(RuntimeError):
 
 def _fptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def kernel_info() -> list[dict]:
+    """Resources of the frame kernels (xr_kernel_info_get); needs a device."""
+    L = lib()
+    out = []
+    for i in range(L.xr_kernel_count()):
+        k = XrKernelInfo()
+        rc = L.xr_kernel_info_get(i, C.byref(k))
+        if rc:
+            raise XrError(rc, L.xr_global_last_error().decode())
+        out.append({"name": k.name.decode(), "vgprs": k.vgprs, "scratch_bytes": k.scratch_bytes,
+                    "lds_bytes": k.lds_bytes, "block": k.block, "blocks_per_cu": k.blocks_per_cu,
+                    "waves_per_simd": k.waves_per_simd})
+    return out
 
 
 def default_params(**overrides) -> XrRestirParams:
