@@ -598,9 +598,14 @@ int xr_build_accel(xr_ctx* c, int builder) {
         w.nodes = nullptr;
     }
     hipFree(r.nodes);
-    if (e != hipSuccess || !w.nodes) {
+    if (e != hipSuccess) {
         dev_free(&c->d_tris);
         return set_err(c, XR_ERR_DEVICE, "wide BVH build failed: %s", msg);
+    }
+    if (!w.nodes) {
+        dev_free(&c->d_tris);
+        return set_err(c, XR_ERR_UNSUPPORTED, "BVH stack bound exceeds the traversal stack (%d) at every width",
+                       kMaxTraversalDepth);
     }
     c->d_nodes = w.nodes;
     c->accel.triangle_count = r.tri_count;
