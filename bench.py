@@ -658,7 +658,7 @@ def main():
         "roofline": roofline,
         # registers, scratch, LDS and resident waves of the traversal kernels
         "kernels": {k["name"]: {f: k[f] for f in ("vgprs", "scratch_bytes", "lds_bytes", "waves_per_simd")}
-                    for k in N.kernel_info() if k["name"] in ("k_initial_sample", "k_initial_tail", "k_spatial")},
+                    for k in N.kernel_info() if k["name"] in ("k_initial_sample", "k_tail_bounce", "k_spatial")},
         "setup_s": round(setup_s, 2),
     }
     if not args.no_mlp_study:

@@ -326,7 +326,7 @@ int xr_read_accel(xr_ctx* ctx, void* nodes, size_t node_bytes, void* triangles, 
 
 /* Resources of the frame kernels (no reference counterpart; measurement
  * support): index 0..xr_kernel_count()-1 names k_initial_sample,
- * k_initial_tail, k_temporal, k_spatial, k_plain, k_converge.  vgprs and
+ * k_tail_bounce, k_temporal, k_spatial, k_plain, k_converge.  vgprs and
  * scratch_bytes (per lane) and lds_bytes (per workgroup) as the runtime's
  * function attributes report them; blocks_per_cu resident workgroups of
  * `block` lanes per CU, waves_per_simd = blocks_per_cu * block / 256.

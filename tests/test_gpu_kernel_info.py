@@ -13,13 +13,13 @@ pytestmark = pytest.mark.gpu
 def test_kernel_resources():
     info = {k["name"]: k for k in N.kernel_info()}
     say(f"[kernels] {info}")
-    assert set(info) == {"k_initial_sample", "k_initial_tail", "k_temporal", "k_spatial", "k_plain", "k_converge"}
+    assert set(info) == {"k_initial_sample", "k_tail_bounce", "k_temporal", "k_spatial", "k_plain", "k_converge"}
     for k in info.values():
         assert 0 < k["vgprs"] <= 512 and k["blocks_per_cu"] >= 1 and k["block"] in (256,)
     # the spatial pass: 20 stack levels of 256 lanes in LDS, 7 waves per SIMD
     assert info["k_spatial"]["lds_bytes"] >= 20 * 256 * 4
     assert info["k_spatial"]["waves_per_simd"] == 7
     # the other traversal kernels: 24 levels, 6 waves
-    for name in ("k_initial_sample", "k_initial_tail", "k_plain"):
+    for name in ("k_initial_sample", "k_tail_bounce", "k_plain"):
         assert info[name]["lds_bytes"] >= 24 * 256 * 4
         assert info[name]["waves_per_simd"] == 6

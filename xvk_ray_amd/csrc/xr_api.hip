@@ -82,7 +82,7 @@ struct xr_ctx {
     uint32_t* d_output = nullptr;
     float4* d_radiance = nullptr;
     uint32_t* d_cache_list = nullptr;  // radiance-cache work lists (2 + 2n words)
-    float4* d_tail = nullptr;          // initial-pass paths past the head bounces (4 float4 each)
+    float4* d_tail = nullptr;          // initial-pass paths past the head bounces (2 queues of n x 4 float4)
     uint32_t* d_tail_count = nullptr;
     unsigned long long* d_counters = nullptr;
 
@@ -200,8 +200,8 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_output, n * 4));
     XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
-    XR_HIP(c, dev_alloc(&c->d_tail, 4 * n * f4));
-    XR_HIP(c, dev_alloc(&c->d_tail_count, sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_tail, 2 * 4 * n * f4));  // two path queues, one per bounce parity
+    XR_HIP(c, dev_alloc(&c->d_tail_count, 2 * sizeof(uint32_t)));
     XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));

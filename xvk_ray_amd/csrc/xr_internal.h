@@ -40,7 +40,8 @@ struct FrameDev {
     // count, [2, 2+n) query pixels, [2+n, 2+2n) training pixels.
     uint32_t* cache_list;
     // Paths alive after the head bounces of the initial-sample pass (4 float4
-    // each, n capacity) and their count.
+    // each, n capacity) and their count; the single-view pass keeps a second
+    // queue of n paths at tail + 4n, count at tail_count[1].
     float4* tail;
     uint32_t* tail_count;
 };
