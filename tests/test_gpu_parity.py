@@ -356,7 +356,12 @@ def test_missing_texture_refuses_the_frame():
 # Plain path tracer
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("max_depth", [3, 0])
-def test_plain_accumulation(max_depth):
+@pytest.mark.parametrize("spp", [1, 2])
+def test_plain_accumulation(max_depth, spp):
+    """Plain path tracing over three accumulated frames: one sample per pixel
+    at unbounded depth runs as a head kernel plus one launch per bounce (a
+    closed box: most paths survive, so the launches hand paths on); two
+    samples, or depth 3, in one kernel."""
     scene = cornell_scene()
     W, H = 80, 48
     ctx = N.Context(scene, W, H)
@@ -366,8 +371,8 @@ def test_plain_accumulation(max_depth):
     out = np.zeros(W * H, np.uint32)
     params = N.default_params(plain_max_depth=max_depth)
     for frame in range(1, 4):
-        ubo = N.make_ubo(view, view, 60.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
-        say(f"[plain] frame {frame} depth={max_depth}")
+        ubo = N.make_ubo(view, view, 60.0, W, H, spp=spp, total=frame * spp, bounces=10, sky=0, reuse=0, frame=frame)
+        say(f"[plain] frame {frame} depth={max_depth} spp={spp}")
         ctx.render(ubo, N.XR_MODE_PLAIN, params)
         ctx.sync()
         orc.plain(N.ubo_to_bytes(ubo), W, H, max_depth, accum, out)

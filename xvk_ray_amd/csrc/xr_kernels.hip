@@ -3,7 +3,7 @@
 //   k_initial_sample  ReSTIRGIInitialSample.rgen.hlsl:35-214
 //   k_temporal        ReSTIRGITemporalReuse.rgen.hlsl:42-151
 //   k_spatial         ReSTIRGISpatialReuse.rgen.hlsl:64-183
-//   k_plain           rgen.hlsl:31-151 (+ rhit.hlsl, sampling.hlsl)
+//   k_plain, k_plain_head, k_plain_bounce   rgen.hlsl:31-151 (+ rhit.hlsl, sampling.hlsl)
 //   k_trace           TraceRay on a caller-supplied ray batch
 //
 // Image kernels run 256-lane workgroups on 32x8 pixel tiles; each wave64
@@ -1023,6 +1023,79 @@ hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const Fram
 // The plain closest-hit program (rhit.hlsl:29-59) returns the same position,
 // normal, UV and material as the ReSTIR one; it does not compute the
 // geometric normal, which the plain loop does not use.
+// State of one plain path between bounces.
+struct PlainPath {
+    v3 origin, dir, thr, color;
+    uint32_t rng;
+};
+
+// Bounces b0, b0+1, ... of a plain path (the loop of rgen.hlsl:59-136);
+// returns true when the path is still alive at bounce b1.
+__device__ __forceinline__ bool plain_bounces(const SceneDev& sc, const UniformBufferObject& ubo, uint32_t b0,
+                                              uint32_t b1, uint32_t max_depth, PlainPath* pp, int* col,
+                                              uint32_t* rays) {
+    for (uint32_t bounce = b0;; ++bounce) {
+        if (max_depth != 0u && bounce == max_depth) return false;
+        if (bounce == b1) return true;
+        const RayPre rp = make_ray_pre(pp->origin, pp->dir);
+        Hit hit;
+        ++*rays;
+        Payload pl;
+        bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+        if (found) {
+            closest_hit(sc, hit, &pl);
+            found = pl.material != -1;
+        }
+        if (!found) {
+            const float t = 0.5f * (pp->dir.y + 1.0f);
+            // lerp(a, b, t) = a + t*(b - a)
+            const v3 sky = mk3(1.0f + t * (0.5f - 1.0f), 1.0f + t * (0.7f - 1.0f), 1.0f + t * (1.0f - 1.0f));
+            if (ubo.hasSkyBox) pp->color = add(pp->color, mul(pp->thr, sky));
+            return false;
+        }
+        const float4 m0 = sc.materials[2 * pl.material];
+        const float4 m1 = sc.materials[2 * pl.material + 1];
+        if (__float_as_uint(m1.w) == kDiffuseLight) {
+            pp->color = add(pp->color, muls(mul(pp->thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
+            return false;
+        }
+        v3 wi;
+        float pdf, cos_t;
+        const v3 f = sample_bsdf(sc, pl.normal, m0, m1, pl.u, pl.v, &wi, &pdf, &cos_t, &pp->rng);
+        if (pdf <= 1e-6f) return false;
+        pp->thr = mul(pp->thr, divs(muls(f, cos_t), pdf));
+        if (bounce > 3) {
+            const float p = maxf(pp->thr.x, maxf(pp->thr.y, pp->thr.z));
+            if (rand01(&pp->rng) > p) return false;
+            pp->thr = muls(pp->thr, 1.0f / p);
+        }
+        const v3 hp = add(pp->origin, muls(pp->dir, pl.t));
+        const v3 od = dot(wi, pl.normal) > 0.0f ? pl.normal : neg(pl.normal);
+        pp->origin = add(hp, muls(od, 0.001f));
+        pp->dir = wi;
+    }
+}
+
+// Accumulation and output of one pixel (rgen.hlsl:139-150).
+__device__ __forceinline__ void plain_finish(const FrameDev& fr, const UniformBufferObject& ubo, uint32_t pix,
+                                             v3 pixel) {
+    const bool accumulate = ubo.spp != ubo.totalNumberOfSamples;
+    v3 acc = mk3(0.f, 0.f, 0.f);
+    if (accumulate) acc = xyz(fr.accum[pix]);
+    acc = add(acc, pixel);
+    const v3 fin = divs(acc, (float)ubo.totalNumberOfSamples);
+    fr.accum[pix] = f4(acc, 0.0f);
+    fr.output[pix] = resolve_bgra8(fin, 0.0f);
+}
+
+__device__ __forceinline__ void plain_camera(const UniformBufferObject& ubo, uint32_t x, uint32_t y,
+                                             const FrameDev& fr, PlainPath* pp) {
+    camera_ray(ubo, x, y, fr.width, fr.height, &pp->rng, &pp->origin, &pp->dir);
+    pp->color = mk3(0.f, 0.f, 0.f);
+    pp->thr = mk3(1.f, 1.f, 1.f);
+}
+
+// Every sample of every pixel in one kernel (any spp).
 __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain(SceneDev sc, FrameDev fr, UniformBufferObject ubo,
                                                   uint32_t max_depth) {
     __shared__ int stack[kLdsDepth * kBlock];
@@ -1031,64 +1104,92 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain(SceneDe
     uint32_t rays = 0;
     if (tc.in_image) {
         const uint32_t x = tc.x, y = tc.y;
-        const uint32_t pix = y * fr.width + x;
-        uint32_t rng = init_seed(x, y, fr.width, ubo.currentFrame);
+        PlainPath pp;
+        pp.rng = init_seed(x, y, fr.width, ubo.currentFrame);
         v3 pixel = mk3(0.f, 0.f, 0.f);
-        int* col = stack + threadIdx.x;
         for (uint32_t i = 0; i < ubo.spp; ++i) {
-            v3 origin, dir;
-            camera_ray(ubo, x, y, fr.width, fr.height, &rng, &origin, &dir);
-            v3 color = mk3(0.f, 0.f, 0.f);
-            v3 thr = mk3(1.f, 1.f, 1.f);
-            for (uint32_t bounce = 0;; ++bounce) {
-                if (max_depth != 0u && bounce == max_depth) break;
-                const RayPre rp = make_ray_pre(origin, dir);
-                Hit hit;
-                ++rays;
-                Payload pl;
-                bool found = trace_ray<kBlock, false>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
-                if (found) {
-                    closest_hit(sc, hit, &pl);
-                    found = pl.material != -1;
-                }
-                if (!found) {
-                    const float t = 0.5f * (dir.y + 1.0f);
-                    // lerp(a, b, t) = a + t*(b - a)
-                    const v3 sky = mk3(1.0f + t * (0.5f - 1.0f), 1.0f + t * (0.7f - 1.0f), 1.0f + t * (1.0f - 1.0f));
-                    if (ubo.hasSkyBox) color = add(color, mul(thr, sky));
-                    break;
-                }
-                const float4 m0 = sc.materials[2 * pl.material];
-                const float4 m1 = sc.materials[2 * pl.material + 1];
-                if (__float_as_uint(m1.w) == kDiffuseLight) {
-                    color = add(color, muls(mul(thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
-                    break;
-                }
-                v3 wi;
-                float pdf, cos_t;
-                const v3 f = sample_bsdf(sc, pl.normal, m0, m1, pl.u, pl.v, &wi, &pdf, &cos_t, &rng);
-                if (pdf <= 1e-6f) break;
-                thr = mul(thr, divs(muls(f, cos_t), pdf));
-                if (bounce > 3) {
-                    const float p = maxf(thr.x, maxf(thr.y, thr.z));
-                    if (rand01(&rng) > p) break;
-                    thr = muls(thr, 1.0f / p);
-                }
-                const v3 hp = add(origin, muls(dir, pl.t));
-                const v3 od = dot(wi, pl.normal) > 0.0f ? pl.normal : neg(pl.normal);
-                origin = add(hp, muls(od, 0.001f));
-                dir = wi;
-            }
-            pixel = add(pixel, color);
+            plain_camera(ubo, x, y, fr, &pp);
+            plain_bounces(sc, ubo, 0u, 0xFFFFFFFFu, max_depth, &pp, stack + threadIdx.x, &rays);
+            pixel = add(pixel, pp.color);
         }
-        const bool accumulate = ubo.spp != ubo.totalNumberOfSamples;
-        v3 acc = mk3(0.f, 0.f, 0.f);
-        if (accumulate) acc = xyz(fr.accum[pix]);
-        acc = add(acc, pixel);
-        const v3 fin = divs(acc, (float)ubo.totalNumberOfSamples);
-        fr.accum[pix] = f4(acc, 0.0f);
-        fr.output[pix] = resolve_bgra8(fin, 0.0f);
+        plain_finish(fr, ubo, y * fr.width + x, pixel);
     }
+    count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// One sample per pixel as a head kernel (bounces 0-1) and one launch per
+// further bounce, the paths handed on through the two path queues as in the
+// initial-sample pass (a sample's RNG continues from the previous sample's,
+// so only spp = 1 splits).  Record: [0] origin, pixel [1] direction, rng
+// [2] throughput [3] colour so far.
+__device__ __forceinline__ void plain_store(float4* rec, uint32_t pix, const PlainPath& pp) {
+    rec[0] = f4(pp.origin, __uint_as_float(pix));
+    rec[1] = f4(pp.dir, __uint_as_float(pp.rng));
+    rec[2] = f4(pp.thr, 0.0f);
+    rec[3] = f4(pp.color, 0.0f);
+}
+
+__device__ __forceinline__ void plain_hand_on(bool alive, float4* queue, uint32_t* count, uint32_t pix,
+                                              const PlainPath& pp, uint32_t* lds_count, uint32_t* lds_base) {
+    const uint32_t q = block_slot(alive, count, lds_count, lds_base);
+    if (alive) plain_store(queue + 4 * (size_t)q, pix, pp);
+}
+
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain_head(SceneDev sc, FrameDev fr,
+                                                                               UniformBufferObject ubo,
+                                                                               uint32_t max_depth) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    __shared__ uint32_t lds_base;
+    const TileCoord tc = tile_pixel(fr.width, fr.height);
+    uint32_t rays = 0;
+    bool alive = false;
+    PlainPath pp;
+    const uint32_t pix = tc.y * fr.width + tc.x;
+    if (tc.in_image) {
+        pp.rng = init_seed(tc.x, tc.y, fr.width, ubo.currentFrame);
+        plain_camera(ubo, tc.x, tc.y, fr, &pp);
+        alive = plain_bounces(sc, ubo, 0u, kHeadBounces, max_depth, &pp, stack + threadIdx.x, &rays);
+        if (!alive) plain_finish(fr, ubo, pix, add(mk3(0.f, 0.f, 0.f), pp.color));
+    }
+    plain_hand_on(alive, fr.tail, fr.tail_count, pix, pp, &lds_count, &lds_base);
+    count_rays(&fr.counters[0], rays, &lds_count);
+}
+
+// Bounce `bounce` of the handed-on plain paths, or (few paths left, or the
+// last launch) all their remaining bounces; as k_tail_bounce.
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain_bounce(SceneDev sc, FrameDev fr,
+                                                                                 UniformBufferObject ubo,
+                                                                                 uint32_t max_depth, uint32_t bounce,
+                                                                                 uint32_t last) {
+    __shared__ int stack[kLdsDepth * kBlock];
+    __shared__ uint32_t lds_count;
+    __shared__ uint32_t lds_base;
+    const uint32_t side = (bounce - kHeadBounces) & 1u;
+    const float4* in = fr.tail + (size_t)side * 4 * fr.n;
+    float4* out = fr.tail + (size_t)(side ^ 1u) * 4 * fr.n;
+    const uint32_t count = fr.tail_count[side];
+    if (blockIdx.x * kBlock >= count) return;  // uniform over the workgroup
+    const bool one = !last && (uint64_t)count * 4u > (uint64_t)fr.n;  // uniform over the launch
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t rays = 0;
+    bool alive = false;
+    PlainPath pp;
+    uint32_t pix = 0;
+    if (q < count) {
+        const float4* rec = in + 4 * (size_t)q;
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+        pix = __float_as_uint(r0.w);
+        pp.origin = xyz(r0);
+        pp.dir = xyz(r1);
+        pp.rng = __float_as_uint(r1.w);
+        pp.thr = xyz(r2);
+        pp.color = xyz(r3);
+        alive = plain_bounces(sc, ubo, bounce, one ? bounce + 1u : 0xFFFFFFFFu, max_depth, &pp,
+                              stack + threadIdx.x, &rays);
+        if (!alive) plain_finish(fr, ubo, pix, add(mk3(0.f, 0.f, 0.f), pp.color));
+    }
+    if (one) plain_hand_on(alive, out, fr.tail_count + (side ^ 1u), pix, pp, &lds_count, &lds_base);
     count_rays(&fr.counters[0], rays, &lds_count);
 }
 
@@ -1149,8 +1250,27 @@ hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame
 }
 hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,
                         hipStream_t st) {
-    hipLaunchKernelGGL(k_plain, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
+    // One kernel for several samples per pixel, and where at most one bounce
+    // would follow the head (the extra launches cost more than they save:
+    // 0.136 against 0.159 ms for the 640x360 Cornell frame at depth 3).
+    if (ubo.spp != 1u || (max_depth != 0u && max_depth <= kHeadBounces + 1u)) {
+        hipLaunchKernelGGL(k_plain, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
+                           max_depth);
+        return hipGetLastError();
+    }
+    // Launches past the head: one per bounce up to the depth cap, at most
+    // kPlainLaunches (unbounded paths: the last launch finishes every path).
+    constexpr uint32_t kPlainLaunches = 12;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, 1u);
+    hipLaunchKernelGGL(k_plain_head, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        max_depth);
+    const uint32_t end = max_depth != 0u ? max_depth : kHeadBounces + kPlainLaunches;
+    for (uint32_t b = kHeadBounces; b < end; ++b) {
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count + ((b - kHeadBounces + 1u) & 1u),
+                           1u);
+        hipLaunchKernelGGL(k_plain_bounce, dim3((fr.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, sc, fr, ubo,
+                           max_depth, b, b + 1u == end ? 1u : 0u);
+    }
     return hipGetLastError();
 }
 hipError_t launch_converge(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
