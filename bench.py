@@ -43,12 +43,13 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # HBM bytes per launch of each pass's kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1
 # --no-cpu-baseline --no-mlp-study` (KiB per dispatch, averaged over the
-# dispatches), each scaled by what the counter reads for a plain 1 GiB
+# frames; the initial pass adds its head kernel and its per-bounce tail
+# launches of one frame), each scaled by what the counter reads for a plain 1 GiB
 # device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 171738.3 + 22955.2, "temporal": 236412.3, "spatial": 787485.4}
-PMC_WRITE_KIB = {"initial": 750515.3 + 24225.4, "temporal": 215782.1, "spatial": 253961.1}
+PMC_FETCH_KIB = {"initial": 171701.1 + 20709.8, "temporal": 236400.8, "spatial": 787109.2}
+PMC_WRITE_KIB = {"initial": 750600.7 + 20043.4, "temporal": 215782.0, "spatial": 253955.2}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
