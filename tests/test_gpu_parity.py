@@ -355,7 +355,7 @@ def test_missing_texture_refuses_the_frame():
 # ---------------------------------------------------------------------------
 # Plain path tracer
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("max_depth", [3, 0])
+@pytest.mark.parametrize("max_depth", [3, 5, 0])
 @pytest.mark.parametrize("spp", [1, 2])
 def test_plain_accumulation(max_depth, spp):
     """Plain path tracing over three accumulated frames: one sample per pixel

@@ -147,19 +147,43 @@ static_assert(offsetof(InferLds, act) % 16 == 0 && sizeof(InferLds) <= 160 * 102
 // FP8 weight format: the E4M3 codes are read (1 byte per weight) and
 // dequantised into the BF16 LDS copy; E4M3 times a power of two is exact in
 // BF16, so the MFMA sees exactly the quantised weights.
-__device__ __forceinline__ uint16_t weight_bf16(const MlpDev& m, int l, int i) {
-    if (m.wt_f8) return bf16_bits(e4m3_float(m.wt_f8[i]) * m.f8_scale[l]);
-    return m.wt_bf[i];
-}
-
+// Weights into LDS 8 at a time (16 bytes of BF16, or 8 FP8 codes): a row of
+// W^T is 96 (layer 0) or 64 elements, so no 8-element chunk straddles a row.
+// Element-wise 2-byte loads made this the launch's floor (about 40 us, paid
+// even when the frame has few queries).
 __device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
-    for (int i = threadIdx.x; i < 64 * kMlpIn; i += blockDim.x) {
-        const int r = i / kMlpIn, c = i % kMlpIn;
-        s.wt[r * kStrideIn + c] = weight_bf16(m, 0, i);
-    }
-    for (int i = threadIdx.x; i < 4 * 64 * 64; i += blockDim.x) {
-        const int l = i / 4096, r = (i % 4096) / 64, c = i % 64;
-        s.wt[64 * kStrideIn + l * 64 * kStrideHid + r * kStrideHid + c] = weight_bf16(m, 1 + l, 64 * 96 + i);
+    constexpr int kChunks0 = 64 * kMlpIn / 8;
+    constexpr int kChunks = kChunks0 + 4 * 64 * 64 / 8;
+    for (int c = threadIdx.x; c < kChunks; c += blockDim.x) {
+        int l, src, dst;
+        if (c < kChunks0) {
+            const int r = c / (kMlpIn / 8), col = (c % (kMlpIn / 8)) * 8;
+            l = 0;
+            src = r * kMlpIn + col;
+            dst = r * kStrideIn + col;
+        } else {
+            const int d = c - kChunks0, e = d % 512, r = e / 8, col = (e % 8) * 8;
+            l = 1 + d / 512;
+            src = 64 * kMlpIn + (l - 1) * 4096 + r * 64 + col;
+            dst = 64 * kStrideIn + (l - 1) * 64 * kStrideHid + r * kStrideHid + col;
+        }
+        u32x4 v;
+        if (m.wt_f8) {
+            const uint2 q = *(const uint2*)(m.wt_f8 + src);
+            const float sc = m.f8_scale[l];
+            uint32_t h[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t word = j < 2 ? q.x : q.y;
+                const int sh = 16 * (j & 1);
+                h[j] = (uint32_t)bf16_bits(e4m3_float((uint8_t)(word >> sh)) * sc) |
+                       ((uint32_t)bf16_bits(e4m3_float((uint8_t)(word >> (sh + 8))) * sc) << 16);
+            }
+            v = u32x4{h[0], h[1], h[2], h[3]};
+        } else {
+            v = *(const u32x4*)(m.wt_bf + src);
+        }
+        *(u32x4*)(s.wt + dst) = v;
     }
     for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) s.bias[i] = m.w[mlp_boff(i / 64) + i % 64];
     for (int i = threadIdx.x; i < 64 * 3 + 3; i += blockDim.x) s.w5[i] = m.w[mlp_woff(5) + i];
