@@ -45,6 +45,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// FP32 -> BF16 bits on the device: one v_cvt_pk_bf16_f32, round to nearest
+// even like bf16_bits (the two agree on every finite value; the network's
+// activations, gradients and weights are finite).
+__device__ __forceinline__ uint16_t bf16_dev(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
 __device__ __forceinline__ bf16x8 frag16(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const u32x4*)p); }
 
 __device__ __forceinline__ bf16x8 frag_gather(const uint16_t* base, int stride) {
@@ -78,7 +83,7 @@ __device__ __forceinline__ void fwd_layer(const uint16_t* in, int sin, const uin
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                out[row * sout + col] = bf16_bits(fmaxf(acc[j] + bj, 0.0f));
+                out[row * sout + col] = bf16_dev(fmaxf(acc[j] + bj, 0.0f));
             }
         }
     }
@@ -125,7 +130,7 @@ __device__ __forceinline__ void pixel_features(const MlpDev& m, const float4* __
 
 __device__ __forceinline__ void store_row(uint16_t* row, const float* f, int nf) {
     for (int i = 0; i < nf; i += 2) {
-        const uint32_t pk = (uint32_t)bf16_bits(f[i]) | ((uint32_t)bf16_bits(f[i + 1]) << 16);
+        const uint32_t pk = (uint32_t)bf16_dev(f[i]) | ((uint32_t)bf16_dev(f[i + 1]) << 16);
         *(uint32_t*)(row + i) = pk;
     }
 }
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
             for (int k = 0; k < 64; ++k) {
                 const float dh = g0 * w5[3 * k] + g1 * w5[3 * k + 1] + g2 * w5[3 * k + 2];
                 const bool on = bf16_float(s.h[4][lane * kStrideHid + k]) > 0.0f;
-                s.dz[lane * kStrideHid + k] = bf16_bits(on ? dh : 0.0f);
+                s.dz[lane * kStrideHid + k] = bf16_dev(on ? dh : 0.0f);
             }
         }
         wave_sync();
@@ -417,7 +422,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
                     for (int j = 0; j < 16; ++j) {
                         const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
                         const bool on = bf16_float(in[row * sin + col]) > 0.0f;
-                        s.dz[row * kStrideHid + col] = bf16_bits(on ? dh[tr][tc][j] : 0.0f);
+                        s.dz[row * kStrideHid + col] = bf16_dev(on ? dh[tr][tc][j] : 0.0f);
                     }
                 }
             }
