@@ -12,7 +12,7 @@
 // layer is 192 FMAs per row on the VALU.
 //
 //   k_mlp_infer   cache queries of the initial-sample pass -> L_out (persistent,
-//                 weights in LDS, 4 waves per workgroup)
+//                 weights in LDS, 8 waves per workgroup)
 //   k_mlp_train   forward + backward on the frame's training samples, FP32
 //                 gradients accumulated with atomics into 32 partial copies
 //   k_mlp_adam    sums the partials, removes the loss scale, Adam step on the
@@ -89,6 +89,41 @@ __device__ __forceinline__ void fwd_layer(const uint16_t* in, int sin, const uin
     }
 }
 
+// The same layer computed in place in a [64][stride] buffer: a row block's
+// two 32-column output tiles are both accumulated before either is stored,
+// and no later output tile reads those rows, so one buffer per wave serves
+// every layer.
+template <int K>
+__device__ __forceinline__ void fwd_layer_inplace(uint16_t* buf, int stride, const uint16_t* wt, int wstride,
+                                                  const float* bias, int lane) {
+    const int lr = lane & 31, lk = 8 * (lane >> 5);
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+        f32x16 acc[2];
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) {
+            acc[tc] = f32x16{};
+#pragma unroll
+            for (int ks = 0; ks < K / 16; ++ks) {
+                const bf16x8 a = frag16(buf + (tr * 32 + lr) * stride + ks * 16 + lk);
+                const bf16x8 b = frag16(wt + (tc * 32 + lr) * wstride + ks * 16 + lk);
+                acc[tc] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tc], 0, 0, 0);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) {
+            const int col = tc * 32 + lr;
+            const float bj = bias[col];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                buf[row * stride + col] = bf16_dev(fmaxf(acc[tc][j] + bj, 0.0f));
+            }
+        }
+    }
+}
+
 // Output layer (64 -> 3) for the row owned by this lane.
 __device__ __forceinline__ void out_layer(const uint16_t* h, const float* w5, const float* b5, float* pred) {
     float s0 = b5[0], s1 = b5[1], s2 = b5[2];
@@ -139,13 +174,15 @@ __device__ __forceinline__ void store_row(uint16_t* row, const float* f, int nf)
 // Inference (cache queries) and evaluation
 // ---------------------------------------------------------------------------
 // LDS: padded W^T of the five hidden layers, their biases and the output
-// layer, plus two activation buffers per wave.
+// layer, plus one activation buffer per wave (layers computed in place), for
+// kInferWaves waves: two per SIMD with one workgroup per CU.
 // Every array starts on a 16-byte boundary (the fragment loads are 16 bytes).
+constexpr int kInferWaves = 8;
 struct InferLds {
     alignas(16) uint16_t wt[64 * kStrideIn + 4 * 64 * kStrideHid];
     alignas(16) float bias[5 * 64];
     alignas(16) float w5[64 * 3 + 3];
-    alignas(16) uint16_t act[4][64 * kStrideIn + 64 * kStrideHid];
+    alignas(16) uint16_t act[kInferWaves][64 * kStrideIn];
 };
 static_assert(offsetof(InferLds, act) % 16 == 0 && sizeof(InferLds) <= 160 * 1024, "InferLds layout");
 
@@ -195,27 +232,26 @@ __device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
 }
 
 // Runs the five hidden layers and the output layer on the tile in act[w]
-// (inputs already stored in its first buffer with stride kStrideIn).
+// (inputs already stored there with stride kStrideIn).
 __device__ __forceinline__ void forward_tile(InferLds& s, int w, int lane, float* pred) {
     uint16_t* A = s.act[w];
-    uint16_t* B = s.act[w] + 64 * kStrideIn;
-    wave_sync();
-    fwd_layer<kMlpIn>(A, kStrideIn, s.wt, kStrideIn, s.bias, B, kStrideHid, lane);
-    wave_sync();
     const uint16_t* wt1 = s.wt + 64 * kStrideIn;
-    fwd_layer<64>(B, kStrideHid, wt1 + 0 * 64 * kStrideHid, kStrideHid, s.bias + 64, A, kStrideHid, lane);
     wave_sync();
-    fwd_layer<64>(A, kStrideHid, wt1 + 1 * 64 * kStrideHid, kStrideHid, s.bias + 128, B, kStrideHid, lane);
+    fwd_layer_inplace<kMlpIn>(A, kStrideIn, s.wt, kStrideIn, s.bias, lane);
     wave_sync();
-    fwd_layer<64>(B, kStrideHid, wt1 + 2 * 64 * kStrideHid, kStrideHid, s.bias + 192, A, kStrideHid, lane);
+    fwd_layer_inplace<64>(A, kStrideIn, wt1 + 0 * 64 * kStrideHid, kStrideHid, s.bias + 64, lane);
     wave_sync();
-    fwd_layer<64>(A, kStrideHid, wt1 + 3 * 64 * kStrideHid, kStrideHid, s.bias + 256, B, kStrideHid, lane);
+    fwd_layer_inplace<64>(A, kStrideIn, wt1 + 1 * 64 * kStrideHid, kStrideHid, s.bias + 128, lane);
     wave_sync();
-    out_layer(B + lane * kStrideHid, s.w5, s.w5 + 192, pred);
+    fwd_layer_inplace<64>(A, kStrideIn, wt1 + 2 * 64 * kStrideHid, kStrideHid, s.bias + 192, lane);
+    wave_sync();
+    fwd_layer_inplace<64>(A, kStrideIn, wt1 + 3 * 64 * kStrideHid, kStrideHid, s.bias + 256, lane);
+    wave_sync();
+    out_layer(A + lane * kStrideIn, s.w5, s.w5 + 192, pred);
     wave_sync();
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const float4* __restrict__ materials) {
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpDev m, FrameDev fr, const float4* __restrict__ materials) {
     __shared__ InferLds s;
     load_weights_lds(m, s);
     __syncthreads();
@@ -228,7 +264,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const 
     const uint32_t nq = fr.cache_list[0], nt = fr.cache_list[1];
     const uint32_t count = nq + nt;
     const uint32_t ntiles = (count + 63) / 64;
-    for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+    for (uint32_t tile = blockIdx.x * kInferWaves + w; tile < ntiles; tile += gridDim.x * kInferWaves) {
         const uint32_t idx = tile * 64 + lane;
         const uint32_t kind = idx < nq ? 1u : 2u;
         const uint32_t pix = idx < nq ? fr.cache_list[2 + idx] : (idx < count ? fr.cache_list[2 + n + (idx - nq)] : 0u);
@@ -249,14 +285,14 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev m, FrameDev fr, const 
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_eval(MlpDev m, const float* __restrict__ feats, uint32_t rows,
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_eval(MlpDev m, const float* __restrict__ feats, uint32_t rows,
                                                   float* __restrict__ out) {
     __shared__ InferLds s;
     load_weights_lds(m, s);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t ntiles = (rows + 63) / 64;
-    for (uint32_t tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+    for (uint32_t tile = blockIdx.x * kInferWaves + w; tile < ntiles; tile += gridDim.x * kInferWaves) {
         const uint32_t r = tile * 64 + lane;
         float f[kMlpIn];
         for (int i = 0; i < kMlpIn; ++i) f[i] = r < rows ? feats[(size_t)r * kMlpIn + i] : 0.0f;
@@ -563,13 +599,14 @@ hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool s
 }
 
 hipError_t launch_mlp_infer(const MlpState& st, const FrameDev& fr, const float4* materials, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(256), 0, s, mlp_dev(st), fr, materials);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(64 * kInferWaves), 0, s, mlp_dev(st), fr, materials);
     return hipGetLastError();
 }
 
 hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows, float* out, hipStream_t s) {
-    const uint32_t blocks = (rows + 255) / 256 < 256 ? (rows + 255) / 256 : 256;
-    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks ? blocks : 1), dim3(256), 0, s, mlp_dev(st), feats, rows, out);
+    const uint32_t per = 64u * kInferWaves;
+    const uint32_t blocks = (rows + per - 1) / per < 256 ? (rows + per - 1) / per : 256;
+    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks ? blocks : 1), dim3(per), 0, s, mlp_dev(st), feats, rows, out);
     return hipGetLastError();
 }
 
