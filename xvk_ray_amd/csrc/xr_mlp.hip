@@ -13,8 +13,9 @@
 //
 //   k_mlp_infer   cache queries of the initial-sample pass -> L_out (persistent,
 //                 weights in LDS, 8 waves per workgroup)
-//   k_mlp_train   forward + backward on the frame's training samples, FP32
-//                 gradients accumulated with atomics into 32 partial copies
+//   k_mlp_train   forward + backward on the frame's training samples (4 waves per
+//                 64-sample tile), FP32 gradients accumulated with atomics into
+//                 32 partial copies
 //   k_mlp_adam    sums the partials, removes the loss scale, Adam step on the
 //                 FP32 master weights, repacks the BF16 copies
 //   k_mlp_eval    forward on caller-given inputs (testing)
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_eval(MlpDev m, const f
 }
 
 // ---------------------------------------------------------------------------
-// Training: forward + backward of 64-sample tiles, one wave per workgroup
+// Training: forward + backward of 64-sample tiles, four waves per workgroup
 // ---------------------------------------------------------------------------
 struct TrainLds {
     alignas(16) uint16_t x[64 * kStrideIn];
@@ -314,21 +315,52 @@ struct TrainLds {
     alignas(16) float g[64 * 4];
 };
 
+// One 32x32 output tile (row block tr, column block tc) of a hidden layer:
+// the arithmetic of fwd_layer for that tile.
+template <int K>
+__device__ __forceinline__ void fwd_subtile(const uint16_t* in, int sin, const uint16_t* wt, int wstride,
+                                            const float* bias, uint16_t* out, int sout, int lane, int tr, int tc) {
+    const int lr = lane & 31, lk = 8 * (lane >> 5);
+    f32x16 acc = {};
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+        const bf16x8 a = frag16(in + (tr * 32 + lr) * sin + ks * 16 + lk);
+        const bf16x8 b = frag16(wt + (tc * 32 + lr) * wstride + ks * 16 + lk);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int col = tc * 32 + lr;
+    const float bj = bias[col];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        out[row * sout + col] = bf16_dev(fmaxf(acc[j] + bj, 0.0f));
+    }
+}
+
+// Four waves per 64-sample tile: wave 0 owns the rows for the features, the
+// output layer and the loss (lane = sample); every layer's four 32x32 output
+// tiles, the weight-gradient tiles and the row loops are shared out between
+// the waves.  With one wave per tile the tile's VALU work ran serially on one
+// SIMD (68 us per frame); the frame has only ~131 tiles.
+constexpr int kTrainWaves = 4;
+
 // Loss: relative L2, sum over samples and channels of (p - y)^2 / (sg(p)^2 + 0.01),
 // times inv_count.  g = loss_scale * dLoss/dp.
 // kFromFrame: the samples are the frame's training pixels listed by the
 // initial pass (*count of them in list); otherwise nslots rows of
 // caller-given features and targets.  inv_count = 1 / (3 * sample count).
 template <bool kFromFrame>
-__global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __restrict__ samples, uint32_t n,
-                                                  const uint32_t* __restrict__ count,
-                                                  const uint32_t* __restrict__ list, uint32_t nslots,
-                                                  const float4* __restrict__ materials,
-                                                  const float* __restrict__ feats, const float* __restrict__ targets,
-                                                  float loss_scale, float inv_count) {
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const float4* __restrict__ samples,
+                                                                uint32_t n, const uint32_t* __restrict__ count,
+                                                                const uint32_t* __restrict__ list, uint32_t nslots,
+                                                                const float4* __restrict__ materials,
+                                                                const float* __restrict__ feats,
+                                                                const float* __restrict__ targets, float loss_scale,
+                                                                float inv_count) {
     __shared__ TrainLds s;
-    const int lane = threadIdx.x;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 31, lk = 8 * (lane >> 5);
+    const int tr = wv >> 1, tc = wv & 1;
     float* grad = m.grad + (size_t)(blockIdx.x % kGradCopies) * kMlpParams;
     if (kFromFrame) {
         nslots = *count;
@@ -336,48 +368,53 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
     }
     const uint32_t ntiles = (nslots + 63) / 64;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint32_t slot = tile * 64 + lane;
-        float f[kMlpIn];
         float y[3] = {0.f, 0.f, 0.f};
         bool valid = false;
-        if (kFromFrame) {
-            const uint32_t pix = slot < nslots ? list[slot] : n;
-            if (pix < n) {
-                const float4 s4 = samples[4 * n + pix];
-                if (__float_as_uint(s4.w) == 2u) {
-                    valid = true;
-                    y[0] = s4.x; y[1] = s4.y; y[2] = s4.z;
-                    pixel_features(m, samples, n, pix, materials, f);
+        if (wv == 0) {
+            const uint32_t slot = tile * 64 + lane;
+            float f[kMlpIn];
+            if (kFromFrame) {
+                const uint32_t pix = slot < nslots ? list[slot] : n;
+                if (pix < n) {
+                    const float4 s4 = samples[4 * n + pix];
+                    if (__float_as_uint(s4.w) == 2u) {
+                        valid = true;
+                        y[0] = s4.x; y[1] = s4.y; y[2] = s4.z;
+                        pixel_features(m, samples, n, pix, materials, f);
+                    }
                 }
+            } else if (slot < nslots) {
+                valid = true;
+                for (int i = 0; i < kMlpIn; ++i) f[i] = feats[(size_t)slot * kMlpIn + i];
+                for (int c = 0; c < 3; ++c) y[c] = targets[(size_t)slot * 3 + c];
             }
-        } else if (slot < nslots) {
-            valid = true;
-            for (int i = 0; i < kMlpIn; ++i) f[i] = feats[(size_t)slot * kMlpIn + i];
-            for (int c = 0; c < 3; ++c) y[c] = targets[(size_t)slot * 3 + c];
+            if (!valid)
+                for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
+            store_row(s.x + lane * kStrideIn, f, kMlpIn);
         }
-        if (!valid)
-            for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
-        store_row(s.x + lane * kStrideIn, f, kMlpIn);
-        wave_sync();
-        // forward, keeping every hidden activation
-        fwd_layer<kMlpIn>(s.x, kStrideIn, m.wt_bf + wt_off(0), kMlpIn, m.w + mlp_boff(0), s.h[0], kStrideHid, lane);
+        __syncthreads();
+        // forward, keeping every hidden activation: wave wv computes tile (tr, tc)
+        fwd_subtile<kMlpIn>(s.x, kStrideIn, m.wt_bf + wt_off(0), kMlpIn, m.w + mlp_boff(0), s.h[0], kStrideHid, lane,
+                            tr, tc);
         for (int l = 1; l < 5; ++l) {
-            wave_sync();
-            fwd_layer<64>(s.h[l - 1], kStrideHid, m.wt_bf + wt_off(l), 64, m.w + mlp_boff(l), s.h[l], kStrideHid,
-                          lane);
+            __syncthreads();
+            fwd_subtile<64>(s.h[l - 1], kStrideHid, m.wt_bf + wt_off(l), 64, m.w + mlp_boff(l), s.h[l], kStrideHid,
+                            lane, tr, tc);
         }
-        wave_sync();
-        float pred[3];
-        out_layer(s.h[4] + lane * kStrideHid, m.w + mlp_woff(5), m.w + mlp_boff(5), pred);
-        for (int c = 0; c < 3; ++c) {
-            const float d = pred[c] - y[c];
-            const float gc = valid ? loss_scale * inv_count * 2.0f * d / (pred[c] * pred[c] + 0.01f) : 0.0f;
-            s.g[lane * 4 + c] = gc;
+        __syncthreads();
+        const float* w5 = m.w + mlp_woff(5);
+        if (wv == 0) {
+            float pred[3];
+            out_layer(s.h[4] + lane * kStrideHid, w5, m.w + mlp_boff(5), pred);
+            for (int c = 0; c < 3; ++c) {
+                const float d = pred[c] - y[c];
+                const float gc = valid ? loss_scale * inv_count * 2.0f * d / (pred[c] * pred[c] + 0.01f) : 0.0f;
+                s.g[lane * 4 + c] = gc;
+            }
         }
-        wave_sync();
-        // output layer: dW5[k][c] = sum_r h4[r][k] g[r][c]; db5[c] = sum_r g[r][c]
-        {
-            const float* w5 = m.w + mlp_woff(5);
+        __syncthreads();
+        // output layer: dW5[k][c] = sum_r h4[r][k] g[r][c]; db5[c] = sum_r g[r][c]  (wave 0, lane = k)
+        if (wv == 0) {
             float a0 = 0.f, a1 = 0.f, a2 = 0.f;
             for (int r = 0; r < 64; ++r) {
                 const float x = bf16_float(s.h[4][r * kStrideHid + lane]);
@@ -393,78 +430,66 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, const float4* __rest
                 for (int r = 0; r < 64; ++r) b += s.g[r * 4 + lane];
                 atomicAdd(&grad[mlp_boff(5) + lane], b);
             }
-            // dZ4 = (g . W5^T) masked by h4 > 0, for this lane's row
+        }
+        // dZ4 = (g . W5^T) masked by h4 > 0: lane = row, wave wv takes columns 16 wv .. 16 wv + 15
+        {
             const float g0 = s.g[lane * 4 + 0], g1 = s.g[lane * 4 + 1], g2 = s.g[lane * 4 + 2];
-            for (int k = 0; k < 64; ++k) {
+            for (int k = 16 * wv; k < 16 * wv + 16; ++k) {
                 const float dh = g0 * w5[3 * k] + g1 * w5[3 * k + 1] + g2 * w5[3 * k + 2];
                 const bool on = bf16_float(s.h[4][lane * kStrideHid + k]) > 0.0f;
                 s.dz[lane * kStrideHid + k] = bf16_dev(on ? dh : 0.0f);
             }
         }
-        wave_sync();
+        __syncthreads();
         // hidden layers 4..0: dW_l = in^T . dz, db_l = colsum(dz), dz_prev = (dz . W_l^T) * (in > 0)
         for (int l = 4; l >= 0; --l) {
             const uint16_t* in = l == 0 ? s.x : s.h[l - 1];
             const int sin = l == 0 ? kStrideIn : kStrideHid;
             const int K = l == 0 ? kMlpIn : 64;
             const int wo = mlp_woff(l);
-            for (int tm = 0; tm < K / 32; ++tm) {
-                for (int tn = 0; tn < 2; ++tn) {
-                    f32x16 acc = {};
+            const int nsub = (K / 32) * 2;
+            for (int sub = wv; sub < nsub; sub += kTrainWaves) {
+                const int tm = sub >> 1, tn = sub & 1;
+                f32x16 acc = {};
 #pragma unroll
-                    for (int ks = 0; ks < 4; ++ks) {
-                        const bf16x8 a = frag_gather(in + (ks * 16 + lk) * sin + tm * 32 + lr, sin);
-                        const bf16x8 b = frag_gather(s.dz + (ks * 16 + lk) * kStrideHid + tn * 32 + lr, kStrideHid);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-                    }
-                    const int col = tn * 32 + lr;
+                for (int ks = 0; ks < 4; ++ks) {
+                    const bf16x8 a = frag_gather(in + (ks * 16 + lk) * sin + tm * 32 + lr, sin);
+                    const bf16x8 b = frag_gather(s.dz + (ks * 16 + lk) * kStrideHid + tn * 32 + lr, kStrideHid);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                }
+                const int col = tn * 32 + lr;
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int row = tm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                        atomicAdd(&grad[wo + row * 64 + col], acc[j]);
-                    }
+                for (int j = 0; j < 16; ++j) {
+                    const int row = tm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                    atomicAdd(&grad[wo + row * 64 + col], acc[j]);
                 }
             }
-            {
+            {  // column = lane, rows 16 wv .. 16 wv + 15
                 float b = 0.f;
-                for (int r = 0; r < 64; ++r) b += bf16_float(s.dz[r * kStrideHid + lane]);
+                for (int r = 16 * wv; r < 16 * wv + 16; ++r) b += bf16_float(s.dz[r * kStrideHid + lane]);
                 atomicAdd(&grad[mlp_boff(l) + lane], b);
             }
             if (l == 0) break;
-            // dH_in = dz . W_l^T  (W_l row-major [64][64] from the BF16 copy)
+            // dH_in = dz . W_l^T  (W_l row-major [64][64] from the BF16 copy): wave wv computes tile (tr, tc)
             const uint16_t* wl = m.w_bf + (l - 1) * 4096;
-            f32x16 dh[2][2];
+            f32x16 acc = {};
 #pragma unroll
-            for (int tr = 0; tr < 2; ++tr) {
-#pragma unroll
-                for (int tc = 0; tc < 2; ++tc) {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ++ks) {
-                        const bf16x8 a = frag16(s.dz + (tr * 32 + lr) * kStrideHid + ks * 16 + lk);
-                        const bf16x8 b = frag16(wl + (tc * 32 + lr) * 64 + ks * 16 + lk);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-                    }
-                    dh[tr][tc] = acc;
-                }
+            for (int ks = 0; ks < 4; ++ks) {
+                const bf16x8 a = frag16(s.dz + (tr * 32 + lr) * kStrideHid + ks * 16 + lk);
+                const bf16x8 b = frag16(wl + (tc * 32 + lr) * 64 + ks * 16 + lk);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
-            wave_sync();
+            __syncthreads();  // every wave is done reading dz
+            const int col = tc * 32 + lr;
 #pragma unroll
-            for (int tr = 0; tr < 2; ++tr) {
-#pragma unroll
-                for (int tc = 0; tc < 2; ++tc) {
-                    const int col = tc * 32 + lr;
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                        const bool on = bf16_float(in[row * sin + col]) > 0.0f;
-                        s.dz[row * kStrideHid + col] = bf16_dev(on ? dh[tr][tc][j] : 0.0f);
-                    }
-                }
+            for (int j = 0; j < 16; ++j) {
+                const int row = tr * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                const bool on = bf16_float(in[row * sin + col]) > 0.0f;
+                s.dz[row * kStrideHid + col] = bf16_dev(on ? acc[j] : 0.0f);
             }
-            wave_sync();
+            __syncthreads();
         }
-        wave_sync();
+        __syncthreads();
     }
 }
 
@@ -614,7 +639,7 @@ hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const 
                                   hipStream_t s) {
     // The sample count is on the device (cache_list[1]); 256 workgroups
     // stride over its 64-sample tiles.
-    hipLaunchKernelGGL(k_mlp_train<true>, dim3(256), dim3(64), 0, s, mlp_dev(st), fr.samples, fr.n,
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(256), dim3(64 * kTrainWaves), 0, s, mlp_dev(st), fr.samples, fr.n,
                        (const uint32_t*)(fr.cache_list + 1), (const uint32_t*)(fr.cache_list + 2 + fr.n), 0u,
                        materials, (const float*)nullptr, (const float*)nullptr, loss_scale, 0.0f);
     return hipGetLastError();
@@ -624,7 +649,7 @@ hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const f
                                  float loss_scale, hipStream_t s) {
     const uint32_t tiles = (rows + 63) / 64;
     const uint32_t blocks = tiles < 512 ? tiles : 512;
-    hipLaunchKernelGGL(k_mlp_train<false>, dim3(blocks ? blocks : 1), dim3(64), 0, s, mlp_dev(st),
+    hipLaunchKernelGGL(k_mlp_train<false>, dim3(blocks ? blocks : 1), dim3(64 * kTrainWaves), 0, s, mlp_dev(st),
                        (const float4*)nullptr, 0u, (const uint32_t*)nullptr, (const uint32_t*)nullptr, rows,
                        (const float4*)nullptr, feats,
                        targets, loss_scale, 1.0f / (3.0f * (float)(rows ? rows : 1)));
