@@ -96,7 +96,9 @@ def test_forward_encoded_features():
     assert np.abs(gpu - ref).max() <= 1e-2 * scale
 
 
-@pytest.mark.parametrize("rows", [64, 1000, 8192])
+# 40,001 rows: 626 tiles of 64 for at most 512 workgroups, so some workgroups
+# loop to a second tile, and the last tile holds one row.
+@pytest.mark.parametrize("rows", [64, 1000, 8192, 40001])
 def test_gradient_matches_oracle(rows):
     ctx = _ctx()
     w = _positive_params(12)
