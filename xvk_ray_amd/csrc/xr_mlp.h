@@ -82,6 +82,28 @@ XR_HD float e4m3_layer_scale(float maxabs) {
     return ldexpf(1.0f, m == 0.5f ? k - 1 : k);
 }
 
+// The 24 frequency features of one normalised coordinate u: out[2k] =
+// sin(2^k pi u), out[2k+1] = cos(2^k pi u).
+XR_HD void mlp_encode_axis(float u, float* out) {
+    float scale = XR_PI;
+    for (int k = 0; k < kMlpBands; ++k) {
+        float s, c;
+        sincos_xr(scale * u, &s, &c);
+        out[2 * k] = s;
+        out[2 * k + 1] = c;
+        scale = scale * 2.0f;
+    }
+}
+
+// Inputs [72, 96): w, n, a, roughness 0, zero padding.
+XR_HD void mlp_encode_tail(v3 w, v3 n, v3 a, float* out) {
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+    out[3] = n.x; out[4] = n.y; out[5] = n.z;
+    out[6] = a.x; out[7] = a.y; out[8] = a.z;
+    out[9] = 0.0f;
+    for (int i = 10; i < kMlpIn - 6 * kMlpBands; ++i) out[i] = 0.0f;
+}
+
 // The 96 network inputs of a vertex at position p (in world units, scaled to
 // [0,1]^3 by the scene box lo / inv_ext), outgoing direction w (unit,
 // pointing away from the vertex), shading normal n, albedo a:
@@ -91,21 +113,8 @@ XR_HD float e4m3_layer_scale(float maxabs) {
 //   [82, 96) zero
 XR_HD void mlp_encode(v3 p, v3 w, v3 n, v3 a, v3 lo, v3 inv_ext, float* out) {
     const float u[3] = {(p.x - lo.x) * inv_ext.x, (p.y - lo.y) * inv_ext.y, (p.z - lo.z) * inv_ext.z};
-    for (int ax = 0; ax < 3; ++ax) {
-        float scale = XR_PI;
-        for (int k = 0; k < kMlpBands; ++k) {
-            float s, c;
-            sincos_xr(scale * u[ax], &s, &c);
-            out[ax * 2 * kMlpBands + 2 * k] = s;
-            out[ax * 2 * kMlpBands + 2 * k + 1] = c;
-            scale = scale * 2.0f;
-        }
-    }
-    out[72] = w.x; out[73] = w.y; out[74] = w.z;
-    out[75] = n.x; out[76] = n.y; out[77] = n.z;
-    out[78] = a.x; out[79] = a.y; out[80] = a.z;
-    out[81] = 0.0f;
-    for (int i = 82; i < kMlpIn; ++i) out[i] = 0.0f;
+    for (int ax = 0; ax < 3; ++ax) mlp_encode_axis(u[ax], out + ax * 2 * kMlpBands);
+    mlp_encode_tail(w, n, a, out + 6 * kMlpBands);
 }
 
 // Per-frame selection of training pixels: pixel p trains when

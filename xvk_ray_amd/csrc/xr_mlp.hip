@@ -337,10 +337,10 @@ __device__ __forceinline__ void fwd_subtile(const uint16_t* in, int sin, const u
     }
 }
 
-// Four waves per 64-sample tile: wave 0 owns the rows for the features, the
-// output layer and the loss (lane = sample); every layer's four 32x32 output
-// tiles, the weight-gradient tiles and the row loops are shared out between
-// the waves.  With one wave per tile the tile's VALU work ran serially on one
+// Four waves per 64-sample tile: each wave encodes a quarter of the input
+// columns, wave 0 owns the rows for the output layer and the loss (lane =
+// sample), and every layer's four 32x32 output tiles, the weight-gradient
+// tiles and the row loops are shared out between the waves.  With one wave per tile the tile's VALU work ran serially on one
 // SIMD (68 us per frame); the frame has only ~131 tiles.
 constexpr int kTrainWaves = 4;
 
@@ -368,30 +368,40 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
     }
     const uint32_t ntiles = (nslots + 63) / 64;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // inputs: wave w < 3 writes the 24 frequency features of axis w, wave 3
+        // columns 72..95 (direction, normal, albedo, padding)
+        const uint32_t slot = tile * 64 + lane;
+        const int c0 = 24 * wv;
         float y[3] = {0.f, 0.f, 0.f};
+        float f[24];
         bool valid = false;
-        if (wv == 0) {
-            const uint32_t slot = tile * 64 + lane;
-            float f[kMlpIn];
-            if (kFromFrame) {
-                const uint32_t pix = slot < nslots ? list[slot] : n;
-                if (pix < n) {
-                    const float4 s4 = samples[4 * n + pix];
-                    if (__float_as_uint(s4.w) == 2u) {
-                        valid = true;
-                        y[0] = s4.x; y[1] = s4.y; y[2] = s4.z;
-                        pixel_features(m, samples, n, pix, materials, f);
+        if (kFromFrame) {
+            const uint32_t pix = slot < nslots ? list[slot] : n;
+            if (pix < n) {
+                const float4 s4 = samples[4 * n + pix];
+                if (__float_as_uint(s4.w) == 2u) {
+                    valid = true;
+                    y[0] = s4.x; y[1] = s4.y; y[2] = s4.z;
+                    const float4 s2 = samples[2 * n + pix];
+                    if (wv < 3) {
+                        const float pa = wv == 0 ? s2.x : (wv == 1 ? s2.y : s2.z);
+                        mlp_encode_axis((pa - m.lo[wv]) * m.inv_ext[wv], f);
+                    } else {
+                        const float4 s0 = samples[pix], s3 = samples[3 * n + pix];
+                        const v3 w = normalize(sub(mk3(s0.x, s0.y, s0.z), mk3(s2.x, s2.y, s2.z)));
+                        const float4 mc = materials[2 * __float_as_uint(s3.w)];
+                        mlp_encode_tail(w, mk3(s3.x, s3.y, s3.z), mk3(mc.x, mc.y, mc.z), f);
                     }
                 }
-            } else if (slot < nslots) {
-                valid = true;
-                for (int i = 0; i < kMlpIn; ++i) f[i] = feats[(size_t)slot * kMlpIn + i];
-                for (int c = 0; c < 3; ++c) y[c] = targets[(size_t)slot * 3 + c];
             }
-            if (!valid)
-                for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
-            store_row(s.x + lane * kStrideIn, f, kMlpIn);
+        } else if (slot < nslots) {
+            valid = true;
+            for (int i = 0; i < 24; ++i) f[i] = feats[(size_t)slot * kMlpIn + c0 + i];
+            for (int c = 0; c < 3; ++c) y[c] = targets[(size_t)slot * 3 + c];
         }
+        if (!valid)
+            for (int i = 0; i < 24; ++i) f[i] = 0.0f;
+        store_row(s.x + lane * kStrideIn + c0, f, 24);
         __syncthreads();
         // forward, keeping every hidden activation: wave wv computes tile (tr, tc)
         fwd_subtile<kMlpIn>(s.x, kStrideIn, m.wt_bf + wt_off(0), kMlpIn, m.w + mlp_boff(0), s.h[0], kStrideHid, lane,
