@@ -311,7 +311,8 @@ float3 TextureSample(const TextureSet& ts, int t, float2 uv) {
         r[c] = ((w00 * tex(x0, y0, c) + w10 * tex(x1, y0, c)) + w01 * tex(x0, y1, c)) + w11 * tex(x1, y1, c);
     return F3(r[0], r[1], r[2]);
 }
-// (mat.baseColorTextureIndex >= 0) ? texColor.rgb : mat.baseColor.rgb (BSDF.hlsl:22-23, 29-30)
+// (mat.baseColorTextureIndex >= 0) ? texColor.rgb : mat.baseColor.rgb (BSDF.hlsl:22-23), the
+// Lambertian albedo; emitters never reach it (the bounce loops break before SampleBSDF)
 float3 MaterialColor(const TextureSet& ts, const BSDFMaterial& mat, float2 uv) {
     if (mat.baseColorTextureIndex >= 0) return TextureSample(ts, mat.baseColorTextureIndex, uv);
     return F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]);
@@ -691,8 +692,10 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
             break;
         }
         const BSDFMaterial& mat = s.materials[pl.MaterialIndex];
-        float3 bc = MaterialColor(s.textures, mat, pl.UV);
+        // Emitters shade with the untextured mat.baseColor.rgb * 4.0
+        // (InitialSample.rgen:117-120): the loop never calls SampleBSDF for them.
         if (mat.materialModel == BSDFDiffuseLight) {
+            const float3 bc = F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]);
             L_out = L_out + throughput * bc * 4.0f;
             if (bounce == 1) { x_sample = pl.Position; n_sample = pl.Normal; }
             if (bounce == 0) { x_view = pl.Position; n_view = pl.Normal; }
@@ -960,8 +963,9 @@ uint32_t plain_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32_t y, uin
                 break;
             }
             const BSDFMaterial& mat = s.materials[pl.MaterialIndex];
+            // untextured mat.baseColor.rgb * 4.0 (rgen.hlsl:96-99)
             if (mat.materialModel == BSDFDiffuseLight) {
-                rayColor = rayColor + throughput * MaterialColor(s.textures, mat, pl.UV) * 4.0f;
+                rayColor = rayColor + throughput * F3(mat.baseColor[0], mat.baseColor[1], mat.baseColor[2]) * 4.0f;
                 break;
             }
             float pdf = 0, cos_theta = 0;

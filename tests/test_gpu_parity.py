@@ -292,8 +292,10 @@ def test_plain_cb2_gltf(tmp_path):
 
 # Textured materials: the Cornell scene with planar texture coordinates that
 # run past [0, 1] (clamp to edge), alternate diffuse materials on a 37x23 or
-# a 5x9 noise texture and the emitter on a 1x1 texture (emission = texel * 4).
-def textured_cornell() -> N.Scene:
+# a 5x9 noise texture and the emitter naming a 1x1 texture, which the bounce
+# loops never sample: emission stays baseColor * 4 (InitialSample.rgen:117-120,
+# rgen.hlsl:96-99).
+def textured_cornell(emitter_texture: bool = True) -> N.Scene:
     v, i, m, o = cornell_scene().arrays()
     v, m = v.copy(), m.copy()
     v["tex_coord"][:, 0] = v["pos"][:, 0] * 1.7 - 0.2
@@ -305,7 +307,7 @@ def textured_cornell() -> N.Scene:
     lam = np.flatnonzero(m["material_model"] == N.MATERIAL_LAMBERTIAN)
     m["base_color_texture_index"][lam[::2]] = 0
     m["base_color_texture_index"][lam[1::2]] = 1
-    m["base_color_texture_index"][m["material_model"] == N.MATERIAL_DIFFUSE_LIGHT] = 2
+    m["base_color_texture_index"][m["material_model"] == N.MATERIAL_DIFFUSE_LIGHT] = 2 if emitter_texture else -1
     return N.Scene.from_arrays(v, i, m, o, tex)
 
 
@@ -334,6 +336,13 @@ def test_plain_textured():
         eq = bytes_equal_rows(ga, accum)
         assert eq.all(), describe_mismatch("accumulation", ga, accum, eq)
     assert accum[:, :3].max() > 0
+    # the emitter's texture is never sampled: the same frames without it are identical
+    ctx2 = N.Context(textured_cornell(emitter_texture=False), W, H)
+    for frame in range(1, 3):
+        ubo = N.make_ubo(view, view, 60.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
+        ctx2.render(ubo, N.XR_MODE_PLAIN, N.default_params(plain_max_depth=0))
+        ctx2.sync()
+    assert ctx2.read(N.XR_BUF_ACCUMULATION).tobytes() == ctx.read(N.XR_BUF_ACCUMULATION).tobytes()
 
 
 def test_missing_texture_refuses_the_frame():

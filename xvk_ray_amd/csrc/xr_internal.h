@@ -447,7 +447,8 @@ __device__ __forceinline__ v3 tex_bilinear(const SceneDev& sc, int t, float u, f
 }
 
 // Base colour of a material at texture coordinates (u, v): its texture when
-// it names one, else baseColor.rgb (BSDF.hlsl:22-23, 29-30).
+// it names one, else baseColor.rgb (BSDF.hlsl:22-23); used for the Lambertian
+// albedo only, since the bounce loops shade emitters untextured.
 __device__ __forceinline__ v3 material_color(const SceneDev& sc, const float4 m0, const float4 m1, float u,
                                              float v) {
     const int t = __float_as_int(m1.x);

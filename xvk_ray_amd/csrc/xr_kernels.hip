@@ -193,8 +193,9 @@ __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uin
         }
         const float4 m0 = sc.materials[2 * pl.material];
         const float4 m1 = sc.materials[2 * pl.material + 1];
+        // Emitters shade with the untextured baseColor.rgb * 4 (InitialSample.rgen:117-120).
         if (__float_as_uint(m1.w) == kDiffuseLight) {
-            o->L = add(o->L, muls(mul(ps->thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
+            o->L = add(o->L, muls(mul(ps->thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
             if (bounce == 1) { o->x_s = pl.position; o->n_s = pl.normal; }
             if (bounce == 0) { o->x_view = pl.position; o->n_view = pl.normal; }
             return false;
@@ -1055,8 +1056,9 @@ __device__ __forceinline__ bool plain_bounces(const SceneDev& sc, const UniformB
         }
         const float4 m0 = sc.materials[2 * pl.material];
         const float4 m1 = sc.materials[2 * pl.material + 1];
+        // untextured baseColor.rgb * 4 (rgen.hlsl:96-99)
         if (__float_as_uint(m1.w) == kDiffuseLight) {
-            pp->color = add(pp->color, muls(mul(pp->thr, material_color(sc, m0, m1, pl.u, pl.v)), 4.0f));
+            pp->color = add(pp->color, muls(mul(pp->thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
             return false;
         }
         v3 wi;
