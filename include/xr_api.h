@@ -380,6 +380,21 @@ int xr_mlp_eval(xr_ctx* ctx, const float* inputs, uint32_t rows, float* outputs)
 int xr_mlp_gradient(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float* grad_out);
 /* One Adam step (lr; loss scale as in the frame) on the given data. */
 int xr_mlp_train_rows(xr_ctx* ctx, const float* inputs, const float* targets, uint32_t rows, float lr);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The gradient
+ * of xr_mlp_gradient for rows x 96 inputs and rows x 3 targets already in
+ * device memory, written to d_grad_out (XR_MLP_PARAMS floats, device memory).
+ * Calls no hipMalloc / hipFree: the 32 partial gradient copies live in a
+ * static device array of the library (2,949,504 bytes per device, held until
+ * the process ends), cleared by a kernel at the start of every call.  The
+ * network must exist (xr_mlp_reset) and rows <= XR_MLP_NOALLOC_MAX_ROWS
+ * (3 * rows exact in FP32, the loss normalisation), else an error.
+ * Asynchronous on the context's stream: make the inputs ready before the
+ * call and xr_sync before reading d_grad_out.  Calls that overlap on one
+ * device race on the shared array. */
+#define XR_MLP_NOALLOC_MAX_ROWS 4194304u
+int xr_mlp_gradient_noalloc(xr_ctx* ctx, const float* d_inputs, const float* d_targets, uint32_t rows,
+                            float* d_grad_out);
 #define XR_MLP_WEIGHTS_BF16 0
 #define XR_MLP_WEIGHTS_FP8 1
 #define XR_MLP_FP8_BYTES 22528 /* E4M3 W^T of the hidden layers: [64][96] then 4 x [64][64] */

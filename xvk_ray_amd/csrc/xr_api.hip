@@ -48,6 +48,7 @@ constexpr uint32_t kCounters = 5;
 struct xr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    float* noalloc_scratch = nullptr;  // the device's static scratch array (xr_mlp_gradient_noalloc)
     std::string err;
 
     // scene
@@ -501,6 +502,7 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
         if (e2 != hipSuccess) return fail(set_err(c, hip_code(e2), "%s: %s", #expr, hipGetErrorString(e2))); \
     } while (0)
     XR_CREATE_HIP(hipSetDevice(0));
+    XR_CREATE_HIP(mlp_noalloc_scratch(&c->noalloc_scratch));
     XR_CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& set : c->ring)
         for (auto& ev : set.ev) XR_CREATE_HIP(hipEventCreate(&ev));
@@ -1096,6 +1098,21 @@ int xr_mlp_gradient(xr_ctx* c, const float* inputs, const float* targets, uint32
         for (int p = 0; p < 32; ++p) g += parts[(size_t)p * kMlpParams + i];
         grad_out[i] = (float)g;
     }
+    return XR_OK;
+}
+
+int xr_mlp_gradient_noalloc(xr_ctx* c, const float* d_inputs, const float* d_targets, uint32_t rows,
+                            float* d_grad_out) {
+    if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_inputs || !d_targets || !d_grad_out || rows == 0)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "NULL device pointer or no rows");
+    if (rows > XR_MLP_NOALLOC_MAX_ROWS)
+        return set_err(c, XR_ERR_INVALID_ARGUMENT, "xr_mlp_gradient_noalloc: %u rows exceed the maximum of %u", rows,
+                       (unsigned)XR_MLP_NOALLOC_MAX_ROWS);
+    if (!c->d_mlp) return set_err(c, XR_ERR_STATE, "xr_mlp_gradient_noalloc: no network yet (call xr_mlp_reset)");
+    if (!c->noalloc_scratch) return set_err(c, XR_ERR_STATE, "xr_mlp_gradient_noalloc: scratch address not resolved");
+    XR_HIP(c, launch_mlp_gradient_noalloc(c->mlp, c->noalloc_scratch, d_inputs, d_targets, rows, d_grad_out,
+                                          c->stream));
     return XR_OK;
 }
 

@@ -68,6 +68,13 @@ hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const 
                                   hipStream_t s);
 hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const float* targets, uint32_t rows,
                                  float loss_scale, hipStream_t s);
+// Gradient of caller-given device rows with the partial copies in the
+// library's static scratch array (no allocation).  mlp_noalloc_scratch
+// resolves that array's address on the current device (once per device).
+constexpr int kMaxScratchDevices = 64;
+hipError_t mlp_noalloc_scratch(float** out);
+hipError_t launch_mlp_gradient_noalloc(const MlpState& st, float* scratch_ptr, const float* feats,
+                                       const float* targets, uint32_t rows, float* grad_out, hipStream_t s);
 
 // Cache termination of the initial-sample pass (MLP mode).
 struct CacheCfg {

@@ -24,6 +24,7 @@
 #include "xr_mlp.h"
 
 #include <cstddef>
+#include <mutex>
 
 namespace xr {
 
@@ -652,6 +653,61 @@ hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const 
     hipLaunchKernelGGL(k_mlp_train<true>, dim3(256), dim3(64 * kTrainWaves), 0, s, mlp_dev(st), fr.samples, fr.n,
                        (const uint32_t*)(fr.cache_list + 1), (const uint32_t*)(fr.cache_list + 2 + fr.n), 0u,
                        materials, (const float*)nullptr, (const float*)nullptr, loss_scale, 0.0f);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gradient without allocation: the 32 partial gradient copies live in a
+// file-scope device array instead of the network's state, so a caller that
+// cannot allocate (graph capture, serving loops) gets the gradient of
+// caller-given device rows.  One array per device, held until the process
+// ends; every call on a device shares it (overlapping calls race).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchFloats = (size_t)kGradCopies * kMlpParams;  // 2,949,504 bytes
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t mlp_noalloc_scratch(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (float*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+__global__ __launch_bounds__(256) void k_clear_f32(float* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0f;
+}
+
+// grad_out[i] = sum over the copies in copy order, in FP64, rounded once.
+__global__ __launch_bounds__(256) void k_mlp_grad_sum(const float* __restrict__ parts, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kMlpParams) return;
+    double g = 0.0;
+    for (int p = 0; p < kGradCopies; ++p) g += (double)parts[(size_t)p * kMlpParams + i];
+    out[i] = (float)g;
+}
+
+hipError_t launch_mlp_gradient_noalloc(const MlpState& st, float* scratch_ptr, const float* feats,
+                                       const float* targets, uint32_t rows, float* grad_out, hipStream_t s) {
+    MlpState g = st;
+    g.grad = scratch_ptr;
+    hipLaunchKernelGGL(k_clear_f32, dim3(1024), dim3(256), 0, s, scratch_ptr, kNoAllocScratchFloats);
+    hipError_t e = launch_mlp_train_rows(g, feats, targets, rows, 1.0f, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_grad_sum, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, scratch_ptr, grad_out);
     return hipGetLastError();
 }
 

@@ -135,7 +135,7 @@ XR_API_SYMBOLS = [
     "xr_trace_rays_device", "xr_pass_times", "xr_pass_time_totals", "xr_reset_pass_times", "xr_counters",
     "xr_reset_counters", "xr_get_accel_info", "xr_kernel_count", "xr_kernel_info_get",
     "xr_read_accel", "xr_stream", "xr_converge", "xr_converge_reset", "xr_converged_spp", "xr_rmse",
-    "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
+    "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_gradient_noalloc", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
     "xr_mlp_fp8_weights", "xr_mlp_params",
     "xr_batch_create", "xr_batch_render", "xr_batch_frame_time", "xr_batch_set_graph", "xr_batch_config_count",
     "xr_batch_config_info", "xr_batch_set_config", "xr_batch_config", "xr_batch_autotune", "xr_batch_reset",
@@ -204,6 +204,7 @@ def lib() -> C.CDLL:
         L.xr_mlp_reset.argtypes = [P, C.c_uint64]
         L.xr_mlp_eval.argtypes = [P, P, U32, P]
         L.xr_mlp_gradient.argtypes = [P, P, P, U32, P]
+        L.xr_mlp_gradient_noalloc.argtypes = [P, P, P, U32, P]
         L.xr_mlp_train_rows.argtypes = [P, P, P, U32, C.c_float]
         L.xr_mlp_params.argtypes = [P, P, I]
         L.xr_mlp_set_weight_format.argtypes = [P, I]
