@@ -427,7 +427,10 @@ int xr_batch_create(xr_ctx* ctx, uint32_t view_count, const uint32_t* widths, co
 int xr_batch_render(xr_batch* batch, const xr_ubo* ubos, const xr_restir_params* params);
 /* Milliseconds of the last xr_batch_render (events around it).  Waits. */
 int xr_batch_frame_time(xr_batch* batch, float* ms);
-/* enable != 0 (default): replay a captured HIP graph; 0: launch eagerly. */
+/* enable != 0 (default): replay a captured HIP graph; 0: launch eagerly.
+ * The graph is captured again on the next xr_batch_render whenever the
+ * parameters or the kernel configuration change, or xr_set_textures /
+ * xr_build_accel replaced the scene's device memory since the capture. */
 int xr_batch_set_graph(xr_batch* batch, int enable);
 /* Kernel configurations (tile shape and LDS traversal-stack depth). */
 uint32_t xr_batch_config_count(void);

@@ -292,7 +292,8 @@ def test_texture_index_without_textures_is_kept(tmp_path):
     assert scene.arrays()[2]["base_color_texture_index"][1] == 2
 
 
-@pytest.mark.parametrize("case", ["missing", "json", "float_indices", "no_indices", "short_buffer", "past_buffer"])
+@pytest.mark.parametrize("case", ["missing", "json", "float_indices", "no_indices", "short_buffer", "past_buffer",
+                                  "count_wraps", "count_huge", "index_count_wraps", "count_fraction", "material_huge"])
 def test_errors(tmp_path, case):
     if case == "missing":
         path = tmp_path / "nope.gltf"
@@ -308,6 +309,16 @@ def test_errors(tmp_path, case):
             del prim["indices"]
         if case == "past_buffer":
             b.accessors[prim["attributes"]["POSITION"]]["count"] = 1000
+        if case == "count_wraps":  # count * 12 wraps to 0 in 64 bits
+            b.accessors[prim["attributes"]["POSITION"]]["count"] = 2 ** 62
+        if case == "count_huge":
+            b.accessors[prim["attributes"]["POSITION"]]["count"] = 1e300
+        if case == "index_count_wraps":  # count * 4 wraps to 0 in 64 bits
+            b.accessors[prim["indices"]]["count"] = 2 ** 62
+        if case == "count_fraction":
+            b.accessors[prim["attributes"]["POSITION"]]["count"] = 3.5
+        if case == "material_huge":
+            prim["material"] = 1e12
         doc = {"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}], "meshes": [{"primitives": [prim]}],
                "bufferViews": b.views, "accessors": b.accessors}
         path = _write(tmp_path, case, doc, [bytes(b.data)])

@@ -131,3 +131,38 @@ def test_batch_rejects_bad_input(scene):
         batch.set_config(99)
     with pytest.raises(N.XrError):
         batch.read(7, N.XR_BUF_OUTPUT)
+
+
+def test_graph_recaptured_after_new_textures_and_rebuilt_tree():
+    """A captured graph holds the scene's device pointers by value: new
+    textures (reallocated) or a rebuilt tree must make the next frame capture
+    again.  The graph batch must keep equal to an eager batch on the same
+    context across both changes."""
+    from tests.helpers import CORNELL_TARGET
+    from tests.test_gpu_parity import textured_cornell
+    scene = textured_cornell()
+    ctx = N.Context(scene, 8, 8)
+    sizes = [(48, 40), (32, 24)]
+    eyes = [(0.5, 0.5, -1.2), (0.45, 0.55, -1.1)]
+    g = N.Batch(ctx, sizes)
+    e = N.Batch(ctx, sizes)
+    e.set_graph(False)
+    p = N.default_params(**PARAMS)
+    rng = np.random.default_rng(8)
+    for f in range(1, 7):
+        if f == 3:  # new texture contents and sizes: the texture arrays are reallocated
+            say("[batch] new textures")
+            ctx.set_textures([(7, 5, rng.integers(0, 256, 7 * 5 * 4, dtype=np.uint8)),
+                              (3, 3, rng.integers(0, 256, 3 * 3 * 4, dtype=np.uint8)),
+                              (1, 1, np.array([10, 20, 30, 255], np.uint8))])
+        if f == 5:  # a rebuilt tree: nodes and triangles are freed and allocated again
+            say("[batch] rebuilt tree")
+            ctx.build_accel(N.XR_BUILDER_PLOC)
+        row = [N.make_ubo(look_view(eye, CORNELL_TARGET), look_view(eye, CORNELL_TARGET), 60.0, w, h, bounces=4,
+                          frame=f) for eye, (w, h) in zip(eyes, sizes)]
+        g.render(row, p)
+        e.render(row, p)
+        ctx.sync()
+        for v in range(len(sizes)):
+            for b in BUFFERS:
+                assert g.read(v, b).tobytes() == e.read(v, b).tobytes(), (f, v, b)

@@ -315,6 +315,47 @@ def test_cache_frame_matches_oracle(fp8):
     assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())
 
 
+def test_cache_frame_training_step_matches_oracle_gradient():
+    """The frame-mode training input (features encoded per wave from the
+    sample planes, targets = traced suffix radiance of the listed training
+    pixels): the first Adam step after a reset moves every weight by
+    -lr * sign(g), so its signs must be the signs of the oracle's gradient of
+    the same loss on the oracle's features and traced targets of the training
+    pixels.  A wrong column, axis, albedo or target mapping scrambles them."""
+    scene = N.Scene.cornell(1)
+    W, H = 64, 48
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    verts, _, mats, _ = scene.arrays()
+    lo = verts["pos"].min(0).astype(np.float32)
+    inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
+    stride, lr = 4, 1e-3
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, mlp_train_pixels=W * H // stride, mlp_spread=0.01,
+                              mlp_lr=lr)
+    ctx.mlp_reset(5)
+    w0 = ctx.mlp_params()
+    w0[B.mlp_layer_offsets(5)[1]:] = 1.0  # output biases away from 0: no dead outputs on the first step
+    ctx.mlp_params(w0)
+    say(f"[cache-train] first cache frame {W}x{H}, stride {stride}")
+    ubo = _cache_frame(ctx, W, H, 1, params)
+    w1 = ctx.mlp_params().astype(np.float64)
+    offset = ((1 * 7919) & 0xFFFFFFFF) % stride
+    ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H, cache=(stride, offset, 0.01))["samples"]
+    train = np.nonzero(ref["pad2"] == 2)[0]
+    assert len(train) > 200
+    x = _features(ref, mats, lo, inv, train)
+    y = ref["l_out"][train].astype(np.float32)
+    _, g = B.mlp_loss_grad(w0, x, y, mode=1)
+    step = w1 - w0.astype(np.float64)
+    big = np.abs(g) > 1e-2 * np.abs(g).max()
+    huge = np.abs(g) > 1e-1 * np.abs(g).max()
+    assert big.sum() > 500
+    agree = np.sign(step[big]) == -np.sign(g[big])
+    assert agree.mean() >= 0.995, agree.mean()
+    assert np.all(np.sign(step[huge]) == -np.sign(g[huge]))
+    assert np.allclose(np.abs(step[big]), lr, rtol=2e-2)
+
+
 def test_cache_gain_over_untrained_and_ray_savings():
     """What training buys: with the same path termination, the trained cache's
     per-frame radiance is closer to the converged image than the untrained

@@ -49,6 +49,9 @@ struct xr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     float* noalloc_scratch = nullptr;  // the device's static scratch array (xr_mlp_gradient_noalloc)
+    // Bumped whenever device memory that a SceneDev points at is freed or
+    // replaced (textures, BVH): captured batch graphs hold a SceneDev by value.
+    uint64_t scene_gen = 0;
     std::string err;
 
     // scene
@@ -554,6 +557,7 @@ int xr_set_textures(xr_ctx* c, const xr_texture* t, uint32_t count) {
         if (total >= (1ull << 32)) return set_err(c, XR_ERR_UNSUPPORTED, "more than 2^32 texels");
     }
     XR_HIP(c, hipStreamSynchronize(c->stream));  // frames in flight may still read the old textures
+    ++c->scene_gen;
     dev_free(&c->d_tex_info);
     dev_free(&c->d_texels);
     c->texture_count = 0;
@@ -576,6 +580,7 @@ int xr_build_accel(xr_ctx* c, int builder) {
     dev_free(&c->d_nodes);
     dev_free(&c->d_tris);
     c->built = false;
+    ++c->scene_gen;
     BvhBuildResult r;
     char msg[256] = {0};
     hipError_t e = (builder == XR_BUILDER_PLOC ? build_ploc : builder == XR_BUILDER_SAH ? build_sah : build_lbvh)(
@@ -1193,7 +1198,7 @@ struct xr_batch {
     hipGraphExec_t exec = nullptr;
     PassParams graph_pp{};
     uint32_t graph_cfg = 0;
-    const void* graph_nodes = nullptr;  // BVH the graph was captured with
+    uint64_t graph_scene_gen = ~0ull;  // the context's scene generation the graph was captured with
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -1268,7 +1273,8 @@ PassParams pass_params(const xr_restir_params& p) {
 bool same_pp(const PassParams& a, const PassParams& b) { return memcmp(&a, &b, sizeof(a)) == 0; }
 
 // One frame of every view: eager launches, or the captured graph (captured
-// again when the parameters, the kernel configuration or the BVH changed).
+// again when the parameters, the kernel configuration or the scene's device
+// memory -- textures, BVH -- changed since the capture).
 int batch_frame(xr_batch* b, const PassParams& pp) {
     xr_ctx* c = b->ctx;
     const SceneDev sc = scene_dev(c);
@@ -1277,7 +1283,7 @@ int batch_frame(xr_batch* b, const PassParams& pp) {
         XR_HIP(c, launch_batch_frame(sc, bd, b->frames.data(), b->cfg, pp, c->stream));
         return XR_OK;
     }
-    if (!b->exec || !same_pp(pp, b->graph_pp) || b->graph_cfg != b->cfg || b->graph_nodes != (const void*)c->d_nodes) {
+    if (!b->exec || !same_pp(pp, b->graph_pp) || b->graph_cfg != b->cfg || b->graph_scene_gen != c->scene_gen) {
         if (b->exec) hipGraphExecDestroy(b->exec);
         if (b->graph) hipGraphDestroy(b->graph);
         b->exec = nullptr;
@@ -1292,7 +1298,7 @@ int batch_frame(xr_batch* b, const PassParams& pp) {
         XR_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graph_pp = pp;
         b->graph_cfg = b->cfg;
-        b->graph_nodes = c->d_nodes;
+        b->graph_scene_gen = c->scene_gen;
     }
     XR_HIP(c, hipGraphLaunch(b->exec, c->stream));
     return XR_OK;

@@ -33,6 +33,26 @@
 // Where the reference would read out of bounds or crash (no indices, an index
 // type it does not handle, an accessor past its buffer, a non-float
 // attribute), this loader returns an error instead.
+//
+// The matrix routines (inverse, mat * vec order) restate OpenGL Mathematics
+// (GLM) 0.9.9.8 so the vertex arrays match the reference's bit for bit.  GLM
+// is used under its MIT licence:
+//   Copyright (c) 2005 - G-Truc Creation
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions: The above copyright notice and this permission
+//   notice shall be included in all copies or substantial portions of the
+//   Software.  THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND,
+//   EXPRESS OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT.  IN
+//   NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM,
+//   DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR
+//   OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
 #include <ctype.h>
 #include <errno.h>
 #include <math.h>
@@ -448,6 +468,21 @@ size_t index_of(const Json* v, size_t bound, const std::string& what) {
     if (!(d >= 0.0) || d != floor(d) || d >= (double)bound) fail(what + " out of range");
     return (size_t)d;
 }
+// accessor.count as an element count whose byte size (count * elem_bytes)
+// cannot overflow: a non-negative integer below 2^50 / elem_bytes (any larger
+// count runs past every buffer, which data_of then reports).
+size_t count_of(const Json& acc, size_t elem_bytes) {
+    const double d = num(acc.get("count"), "accessor.count");
+    if (!(d >= 0.0) || d != floor(d) || d > 1125899906842624.0 / (double)elem_bytes)
+        fail("accessor.count out of range");
+    return (size_t)d;
+}
+// A signed 32-bit integer field (material or texture index; -1 allowed).
+int32_t int32_of(const Json* v, const std::string& what) {
+    const double d = num(v, what);
+    if (!(d >= -2147483648.0 && d <= 2147483647.0) || d != floor(d)) fail(what + " out of range");
+    return (int32_t)d;
+}
 const Json& array_of(const Json& root, const char* key) {
     static const Json empty = [] {
         Json j;
@@ -546,7 +581,7 @@ private:
                 const Json* t = pbr->get("baseColorTexture");
                 if (t) {
                     const Json* idx = t->get("index");
-                    mat.base_color_texture_index = idx ? (int32_t)num(idx, "baseColorTexture.index") : -1;
+                    mat.base_color_texture_index = idx ? int32_of(idx, "baseColorTexture.index") : -1;
                 }
             }
             if (untextured_) mat.base_color_texture_index = -1;
@@ -612,7 +647,7 @@ private:
         const Json& acc = array_of(root_, "accessors").items[acc_index];
         if (num(acc.get("componentType"), "accessor.componentType") != 5126.0)
             fail(path_ + ": vertex attribute is not float32");
-        const size_t n = (size_t)num(acc.get("count"), "accessor.count");
+        const size_t n = count_of(acc, comps * sizeof(float));
         const uint8_t* base = data_of(acc, n * comps * sizeof(float));
         *count = n;
         return reinterpret_cast<const float*>(base);
@@ -679,7 +714,7 @@ private:
                 if (nt < count) fail(path_ + ": TEXCOORD_0 shorter than POSITION");
             }
             const Json* pm = prim.get("material");
-            const int32_t material = pm ? (int32_t)num(pm, "primitive.material") : -1;
+            const int32_t material = pm ? int32_of(pm, "primitive.material") : -1;
             for (size_t v = 0; v < count; ++v) {
                 xr_vertex out;
                 float p[3], n3[3] = {0.0f, 0.0f, 0.0f};
@@ -702,7 +737,7 @@ private:
             if (!pi) fail(path_ + ": primitive without indices");
             const Json& acc = array_of(root_, "accessors").items[index_of(pi, n_acc, "primitive.indices")];
             const double type = num(acc.get("componentType"), "accessor.componentType");
-            const size_t n = (size_t)num(acc.get("count"), "accessor.count");
+            const size_t n = count_of(acc, 4);
             if (type == 5125.0) {
                 const uint8_t* b = data_of(acc, n * 4);
                 for (size_t k = 0; k < n; ++k) {

@@ -6,8 +6,10 @@
 // has alpha or a tRNS chunk; 16-bit samples keep their high byte and samples
 // below 8 bits are scaled by 255 / (2^depth - 1), as stb_image does.
 //
-// Inflate (RFC 1951) with canonical Huffman decoding one bit at a time,
-// inside the zlib wrapper (RFC 1950, Adler-32 checked); PNG filters 0-4 with
+// Inflate (RFC 1951) with canonical Huffman decoding one bit at a time, in
+// the shape of Mark Adler's puff.c (zlib's contrib/puff, a reference inflater
+// written for clarity: counts and symbols per code length, decoded bit by
+// bit), inside the zlib wrapper (RFC 1950, Adler-32 checked); PNG filters 0-4 with
 // the Paeth predictor (PNG 1.2, section 6); chunk CRCs checked.  Interlaced
 // (Adam7) images are refused.
 #include <stdint.h>
