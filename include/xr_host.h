@@ -88,6 +88,9 @@ int xrh_camera_static(const float eye[3], const float target[3], xrh_camera_path
  * looking at a second seeded path
  * near the ground), sampled at `frames` evenly spaced parameters. */
 int xrh_camera_orbit(uint64_t seed, uint32_t n_ctrl, uint32_t frames, xrh_camera_path** out);
+/* Explicit per-frame eye and target positions (3 floats each per frame):
+ * stands in for the reference's interactive camera input. */
+int xrh_camera_keyframes(const float* eyes, const float* targets, uint32_t frames, xrh_camera_path** out);
 /* View matrix (column-major) of path frame k (k taken modulo the frame count). */
 int xrh_camera_view(const xrh_camera_path* p, uint32_t k, float view[16]);
 void xrh_camera_destroy(xrh_camera_path* p);

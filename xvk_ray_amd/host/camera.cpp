@@ -107,6 +107,16 @@ int xrh_camera_static(const float eye[3], const float target[3], xrh_camera_path
     return XR_OK;
 }
 
+int xrh_camera_keyframes(const float* eyes, const float* targets, uint32_t frames, xrh_camera_path** out) {
+    if (!eyes || !targets || !out || frames == 0) return XR_ERR_INVALID_ARGUMENT;
+    xrh_camera_path* p = new (std::nothrow) xrh_camera_path();
+    if (!p) return XR_ERR_OUT_OF_MEMORY;
+    p->eye.assign(eyes, eyes + 3 * (size_t)frames);
+    p->target.assign(targets, targets + 3 * (size_t)frames);
+    *out = p;
+    return XR_OK;
+}
+
 int xrh_camera_orbit(uint64_t seed, uint32_t n_ctrl, uint32_t frames, xrh_camera_path** out) {
     if (!out || n_ctrl < 4 || frames == 0) return XR_ERR_INVALID_ARGUMENT;
     xrh_camera_path* p = new (std::nothrow) xrh_camera_path();

@@ -74,8 +74,9 @@ int xrh_app_draw_frame(xrh_app* a, const xr_restir_params* params) {
         a->total_samples = 0;
         a->rebuild_rays = false;
     }
-    const uint32_t left = a->settings.max_spp > a->total_samples ? a->settings.max_spp - a->total_samples : 0u;
-    a->spp = std::min(left, a->settings.spp);
+    // glm::clamp(maxSpp - totalSamples, 0u, spp) in uint32 arithmetic (the
+    // difference wraps if maxSpp is below totalSamples, as in the reference)
+    a->spp = std::min(a->settings.max_spp - a->total_samples, a->settings.spp);
     a->total_samples += a->spp;
     a->current_frame_total = (a->current_frame_total + 1) % 10000;
     // PathTracer::Render: camera update (pathTracer.cpp:88); a moved camera

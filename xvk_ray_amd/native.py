@@ -146,7 +146,7 @@ XR_API_SYMBOLS = [
 XR_HOST_SYMBOLS = [
     "xrh_scene_cornell", "xrh_scene_terrain", "xrh_scene_gltf", "xrh_scene_desc", "xrh_scene_textures",
     "xrh_scene_triangle_count", "xrh_scene_destroy", "xrh_png_decode",
-    "xrh_camera_static", "xrh_camera_orbit", "xrh_camera_view", "xrh_camera_destroy", "xrh_perspective",
+    "xrh_camera_static", "xrh_camera_keyframes", "xrh_camera_orbit", "xrh_camera_view", "xrh_camera_destroy", "xrh_perspective",
     "xrh_inverse", "xrh_app_create", "xrh_app_draw_frame", "xrh_app_last_ubo", "xrh_app_ctx", "xrh_app_last_error",
     "xrh_app_destroy", "xrh_make_ubo", "xrh_png_encode", "xrh_write_png",
 ]
@@ -256,6 +256,7 @@ def host() -> C.CDLL:
         H.xrh_scene_destroy.restype = None
         H.xrh_camera_static.argtypes = [C.POINTER(F), C.POINTER(F), C.POINTER(P)]
         H.xrh_camera_orbit.argtypes = [U64, U32, U32, C.POINTER(P)]
+        H.xrh_camera_keyframes.argtypes = [C.POINTER(F), C.POINTER(F), U32, C.POINTER(P)]
         H.xrh_camera_view.argtypes = [P, U32, C.POINTER(F)]
         H.xrh_camera_destroy.argtypes = [P]
         H.xrh_camera_destroy.restype = None
@@ -443,6 +444,17 @@ class Camera:
         rc = host().xrh_camera_static(_fptr(e), _fptr(t), C.byref(h))
         if rc:
             raise XrError(rc, "xrh_camera_static")
+        return cls(h)
+
+    @classmethod
+    def keyframes(cls, eyes, targets) -> "Camera":
+        e = np.ascontiguousarray(eyes, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        assert e.shape == t.shape
+        h = C.c_void_p()
+        rc = host().xrh_camera_keyframes(_fptr(e), _fptr(t), len(e), C.byref(h))
+        if rc:
+            raise XrError(rc, "xrh_camera_keyframes")
         return cls(h)
 
     @classmethod
