@@ -200,6 +200,34 @@ def mlp_study(scene, camera, args) -> dict:
     }
 
 
+def host_info() -> dict:
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may run on, the CPU model, the OpenMP thread setting, and whether the
+    tools for the lavapipe baseline (dxc, a lavapipe Vulkan ICD) exist."""
+    import glob
+    import shutil
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    icds = glob.glob("/usr/share/vulkan/icd.d/*lvp*") + glob.glob("/etc/vulkan/icd.d/*lvp*")
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "dxc": shutil.which("dxc") is not None,
+            "lavapipe_icd": bool(icds), "vulkan_loader": bool(glob.glob("/usr/lib/x86_64-linux-gnu/libvulkan.so*"))}
+
+
+def baseline_threads() -> int:
+    """Every CPU this process may run on, unless OMP_NUM_THREADS asks for fewer
+    (the GPU hosts give one GPU's process a share of their cores that way)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
 def cpu_baseline(scene, app, ctx, target_s: float) -> dict:
     """The oracle (C++ restatement of the shaders, OpenMP over the host cores)
     on a bounded pixel sample of the same frame, scaled to a full frame."""
@@ -209,7 +237,7 @@ def cpu_baseline(scene, app, ctx, target_s: float) -> dict:
     from xvk_ray_amd import native as N
 
     L = Oracle.lib()
-    threads = min(L.orc_num_threads(), 16)
+    threads = baseline_threads()
     L.orc_set_threads(threads)
     v, i, m, o = scene.arrays()
     orc = OracleScene(v, i, m, o)
@@ -258,6 +286,7 @@ def cpu_baseline(scene, app, ctx, target_s: float) -> dict:
         "unit": "Mrays/s",
         "ms_per_frame": round(ms_frame, 1),
         "cores": threads,
+        "host": host_info(),
         "kind": "port",
         "sample": f"{what} (all three passes, traced visibility), {total:.1f} s; "
                   f"C++ restatement of the shaders, not lavapipe",
@@ -325,7 +354,7 @@ def run_c1(args) -> dict:
 
         from oracle.binding import Oracle, OracleScene
         L = Oracle.lib()
-        threads = min(L.orc_num_threads(), 16)
+        threads = baseline_threads()
         L.orc_set_threads(threads)
         orc = OracleScene(*scene.arrays())
         ub = N.ubo_to_bytes(app.last_ubo())
@@ -338,7 +367,7 @@ def run_c1(args) -> dict:
             total += time.perf_counter() - t
             reps += 1
         res["cpu_baseline"] = {"value": round(rays / total / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                               "kind": "port", "ms_per_frame": round(total * 1e3 / reps, 2),
+                               "host": host_info(), "kind": "port", "ms_per_frame": round(total * 1e3 / reps, 2),
                                "sample": f"{reps} full 640x360 frames, {total:.1f} s; C++ restatement of the "
                                          f"shaders, not lavapipe"}
     return res
