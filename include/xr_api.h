@@ -145,7 +145,7 @@ typedef struct xr_restir_params {
     uint32_t flags;            /* XR_FLAG_* */
     float mlp_spread;          /* cache: stop at x_s when its spread > mlp_spread * primary footprint (0.01) */
     float mlp_lr;              /* cache: Adam learning rate (1e-3) */
-    uint32_t mlp_train_pixels; /* cache: training samples per frame (65,536) */
+    uint32_t mlp_train_pixels; /* cache: training samples per frame (65,536) of the paths reaching a diffuse x_s */
     float mlp_loss_scale;      /* cache: loss scale of the BF16 backward pass (128) */
     uint32_t reserved[4];      /* must be zero */
 } xr_restir_params;
@@ -156,11 +156,12 @@ typedef struct xr_restir_params {
 #define XR_FLAG_RADIANCE 0x1u
 /* flags: MLP radiance cache (no reference counterpart).  In mode 2 the
  * initial-sample pass ends most paths at their first diffuse vertex x_s and
- * takes L_out from a 96-64x5-3 MLP evaluated there; every
- * (pixels / mlp_train_pixels)-th pixel keeps tracing and its path-suffix
- * radiance trains the network online (one Adam step per frame, BF16 MFMA,
- * FP32 accumulation).  Paths whose x_s is too close to their primary footprint
- * fall back to plain tracing. */
+ * takes L_out from a 96-64x5-3 MLP evaluated there.  Of the frame's paths
+ * that reach a diffuse x_s, exactly mlp_train_pixels (or all, if fewer),
+ * spread evenly over them in pixel order and rotating with the frame, keep
+ * tracing and their path-suffix radiance trains the network online (one Adam
+ * step per frame, BF16 MFMA, FP32 accumulation).  Other paths whose x_s is
+ * too close to their primary footprint fall back to plain tracing. */
 #define XR_FLAG_MLP_CACHE 0x2u
 /* flags: with XR_FLAG_MLP_CACHE, the cache's hidden-layer weights are FP8
  * E4M3 with one power-of-two scale per layer (dequantised exactly to BF16 for

@@ -70,7 +70,7 @@ class Oracle:
             L.orc_trace64.restype = None
             L.orc_initial_sample.argtypes = [P, P, U32, U32, P, U32, P, P, P, P, P]
             L.orc_initial_sample.restype = U64
-            L.orc_initial_sample_cache.argtypes = [P, P, U32, U32, P, U32, P, P, P, P, P, U32, U32, F]
+            L.orc_initial_sample_cache.argtypes = [P, P, U32, U32, P, U32, P, P, P, P, P, U32, F, P]
             L.orc_initial_sample_cache.restype = U64
             L.orc_mlp_param_count.restype = I
             L.orc_mlp_layer_offset.argtypes = [I, I]
@@ -165,7 +165,9 @@ class OracleScene:
         return ids, ts
 
     def initial_sample(self, ubo_bytes: bytes, W: int, H: int, pixels=None, cache=None):
-        """cache: None, or (stride, offset, spread_c) of the radiance-cache path termination."""
+        """cache: None, or (training samples per frame, spread_c) of the radiance-cache
+        path termination; the result then also holds `train`, the per-pixel
+        training choice (uint8)."""
         from xvk_ray_amd.native import SAMPLE_DTYPE
         n = W * H
         samples = np.zeros(n, SAMPLE_DTYPE)
@@ -175,11 +177,13 @@ class OracleScene:
         motion = np.zeros(n, np.uint32)
         ub = np.frombuffer(ubo_bytes, np.uint8).copy()
         px = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)
-        stride, offset, spread = cache if cache is not None else (0, 0, 0.0)
+        target, spread = cache if cache is not None else (0, 0.0)
+        train = np.zeros(n, np.uint8)
         rays = self.L.orc_initial_sample_cache(self.h, ub.ctypes.data, W, H, _ptr(px), 0 if px is None else len(px),
                                                samples.ctypes.data, gpos.ctypes.data, gnrm.ctypes.data,
-                                               gmatuv.ctypes.data, motion.ctypes.data, stride, offset, spread)
-        return dict(samples=samples, gpos=gpos, gnrm=gnrm, gmatuv=gmatuv, motion=motion, rays=rays)
+                                               gmatuv.ctypes.data, motion.ctypes.data, target, spread,
+                                               train.ctypes.data)
+        return dict(samples=samples, gpos=gpos, gnrm=gnrm, gmatuv=gmatuv, motion=motion, rays=rays, train=train)
 
     def spatial(self, W, H, frame, reuse, num_samples, radius, m_max, visibility, samples, temporal, spatial, output,
                 pixels=None) -> int:

@@ -653,17 +653,39 @@ struct GOut {
     uint32_t motion;
 };
 // Radiance-cache path termination (no reference counterpart; Mueller et al.
-// 2021, "Real-time Neural Radiance Caching", section 4): at the first
-// secondary vertex x_s the path stops into the cache when the spread
-// t1^2 / (p_q |cos theta_s|) exceeds c times the primary footprint
-// t0^2 / (4 pi |cos theta_v|); pixel p keeps tracing as a training sample
-// when p % stride == offset.  Recorded in the spare words: pad1 = material
-// at x_s, pad2 = 1 (query) or 2 (training).
+// 2021, "Real-time Neural Radiance Caching", section 4).  A pixel whose
+// path reaches a diffuse first secondary vertex x_s is a candidate.  Of the
+// frame's C candidates exactly min(T, C) keep tracing as training samples:
+// with r the candidate's rank in pixel order and r' = (r + o) mod C,
+// o = (frame * 7919 mod 2^32) mod C, it trains when C <= T or
+// floor((r' + 1) T / C) > floor(r' T / C).  Every other candidate stops into
+// the cache when the spread t1^2 / (p_q |cos theta_s|) exceeds c times the
+// primary footprint t0^2 / (4 pi |cos theta_v|), else traces on as a plain
+// path.  Recorded in the spare words: pad1 = material at x_s, pad2 = 1
+// (query) or 2 (training).
 struct CacheOpt {
     bool enabled = false;
-    uint32_t stride = 1, offset = 0;
+    bool probe = false;              // stop candidates at x_s with pad2 = 3 (the selection's pre-pass)
+    const uint8_t* train = nullptr;  // per pixel: 1 = training sample
     float spread_c = 0.0f;
 };
+
+// The training choice above from the per-pixel candidate flags.
+std::vector<uint8_t> select_training(const std::vector<uint8_t>& candidate, uint32_t frame, uint32_t target) {
+    std::vector<uint8_t> train(candidate.size(), 0);
+    uint64_t count = 0;
+    for (uint8_t c : candidate) count += c;
+    if (count == 0) return train;
+    const uint64_t o = (uint64_t)(uint32_t)(frame * 7919u) % count;
+    uint64_t r = 0;
+    for (size_t p = 0; p < candidate.size(); ++p) {
+        if (!candidate[p]) continue;
+        const uint64_t rr = (r + o) % count;
+        train[p] = count <= target || (rr + 1) * target / count > rr * target / count;
+        ++r;
+    }
+    return train;
+}
 uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32_t y, uint32_t W, uint32_t H,
                               ReSTIRGISample& out, GOut& g, const CacheOpt& cache = CacheOpt()) {
     uint32_t cache_kind = 0, xs_material = 0;
@@ -710,7 +732,11 @@ uint32_t initial_sample_pixel(const Scene& s, const UBO& ubo, uint32_t x, uint32
                 footprint0 = pl.HitT * pl.HitT / (4.0f * PI * cos_hit);
             } else if (bounce == 1) {
                 xs_material = (uint32_t)pl.MaterialIndex;
-                if ((y * W + x) % cache.stride == cache.offset) {
+                if (cache.probe) {
+                    cache_kind = 3;
+                    break;
+                }
+                if (cache.train[y * W + x]) {
                     cache_kind = 2;
                 } else if (pl.HitT * pl.HitT / (p_q * cos_hit) > cache.spread_c * footprint0) {
                     cache_kind = 1;
@@ -1196,16 +1222,32 @@ void orc_trace64(const void* h, const double* rays, uint32_t n, uint32_t* ids, d
 
 // ---- passes over whole images (reference layouts) ----
 // pixels: list of pixel indices to compute (NULL = all W*H).
+// train_target = 0: no cache.  Otherwise the training set is chosen from a
+// pre-pass over every pixel (bounces 0 and 1), whatever `pixels` lists.
 uint64_t orc_initial_sample_cache(const void* h, const void* ubo, uint32_t W, uint32_t H, const uint32_t* pixels,
                                   uint32_t npix, void* samples, float* gpos, float* gnrm, float* gmatuv,
-                                  uint32_t* motion, uint32_t cache_stride, uint32_t cache_offset, float spread_c) {
-    CacheOpt cache;
-    cache.enabled = cache_stride != 0;
-    cache.stride = cache_stride ? cache_stride : 1;
-    cache.offset = cache_offset;
-    cache.spread_c = spread_c;
+                                  uint32_t* motion, uint32_t train_target, float spread_c, uint8_t* train_out) {
     const Scene& s = *(const Scene*)h;
     const UBO& u = *(const UBO*)ubo;
+    CacheOpt cache;
+    cache.enabled = train_target != 0;
+    cache.spread_c = spread_c;
+    std::vector<uint8_t> train;
+    if (cache.enabled) {
+        std::vector<uint8_t> candidate((size_t)W * H, 0);
+        CacheOpt probe = cache;
+        probe.probe = true;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+            ReSTIRGISample tmp;
+            GOut g;
+            initial_sample_pixel(s, u, (uint32_t)(p % W), (uint32_t)(p / W), W, H, tmp, g, probe);
+            candidate[p] = tmp.pad2 == 3 ? 1 : 0;
+        }
+        train = select_training(candidate, u.currentFrame, train_target);
+        cache.train = train.data();
+        if (train_out) memcpy(train_out, train.data(), train.size());
+    }
     ReSTIRGISample* out = (ReSTIRGISample*)samples;
     const int64_t count = pixels ? (int64_t)npix : (int64_t)W * H;
     uint64_t rays = 0;
@@ -1226,7 +1268,8 @@ uint64_t orc_initial_sample_cache(const void* h, const void* ubo, uint32_t W, ui
 uint64_t orc_initial_sample(const void* h, const void* ubo, uint32_t W, uint32_t H, const uint32_t* pixels,
                             uint32_t npix, void* samples, float* gpos, float* gnrm, float* gmatuv,
                             uint32_t* motion) {
-    return orc_initial_sample_cache(h, ubo, W, H, pixels, npix, samples, gpos, gnrm, gmatuv, motion, 0, 0, 0.0f);
+    return orc_initial_sample_cache(h, ubo, W, H, pixels, npix, samples, gpos, gnrm, gmatuv, motion, 0, 0.0f,
+                                    nullptr);
 }
 
 void orc_temporal(uint32_t W, uint32_t H, uint32_t frame, uint32_t m_max, const uint32_t* pixels, uint32_t npix,

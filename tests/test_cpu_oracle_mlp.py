@@ -264,20 +264,21 @@ def test_cache_termination_invariants():
     orc = oracle_scene(cornell_scene())
     ub = N.ubo_to_bytes(_cornell_ubo(W, H))
     plain = orc.initial_sample(ub, W, H)
-    stride, offset = 7, 3
-    always = orc.initial_sample(ub, W, H, cache=(stride, offset, 0.0))
-    never = orc.initial_sample(ub, W, H, cache=(stride, offset, 1e30))
+    T = 97
+    always = orc.initial_sample(ub, W, H, cache=(T, 0.0))
+    never = orc.initial_sample(ub, W, H, cache=(T, 1e30))
     s_p, s_a, s_n = plain["samples"], always["samples"], never["samples"]
-    pix = np.arange(W * H)
-    train = pix % stride == offset
+    train = always["train"].astype(bool)
+    assert np.array_equal(train, never["train"].astype(bool))  # the choice does not depend on the threshold
     kind_a, kind_n = s_a["pad2"], s_n["pad2"]
-    # training pixels: flagged when x_s is a diffuse hit, path identical to the plain one
+    # training pixels: flagged, path identical to the plain one
     fields = ["x_view", "n_view", "x_sample", "n_sample", "l_out", "f", "p_q", "material_id", "rand_seed"]
     for k in fields:
         assert np.array_equal(s_a[k][train], s_p[k][train]), k
         assert np.array_equal(s_n[k], s_p[k]), k  # infinite threshold: no path is cut
-    assert set(np.unique(kind_n)) <= {0, 2} and np.all(kind_n[~train] == 0)
-    # threshold 0: every non-training pixel whose path reached a diffuse x_s is a query with L = 0
+    assert np.array_equal(kind_a == 2, train) and np.array_equal(kind_n == 2, train)
+    assert set(np.unique(kind_n)) <= {0, 2}
+    # threshold 0: every other pixel whose path reached a diffuse x_s is a query with L = 0
     query = kind_a == 1
     assert query.sum() > 0.3 * W * H
     assert np.all(~train[query])
@@ -291,3 +292,32 @@ def test_cache_termination_invariants():
     rest = (kind_a == 0) & ~train
     for k in fields:
         assert np.array_equal(s_a[k][rest], s_p[k][rest]), k
+
+
+def _gaps(chosen_ranks, C):
+    r = np.sort(chosen_ranks)
+    return np.diff(np.concatenate([r, [r[0] + C]]))
+
+
+@pytest.mark.parametrize("T", [1, 64, 97, 500, 100000])
+def test_cache_training_choice_exact_count_even_spacing(T):
+    """Of C candidates (diffuse x_s), exactly min(T, C) train, spread evenly
+    over the candidates in pixel order (cyclic gaps floor(C/T) or ceil(C/T)),
+    and the choice moves with the frame."""
+    W, H = 48, 40
+    orc = oracle_scene(cornell_scene())
+    sets = []
+    for frame in (3, 4):
+        out = orc.initial_sample(N.ubo_to_bytes(_cornell_ubo(W, H, frame)), W, H, cache=(T, 0.0))
+        kind, train = out["samples"]["pad2"], out["train"].astype(bool)
+        cand = (kind == 1) | (kind == 2)
+        C = int(cand.sum())
+        assert C > 0.3 * W * H and np.all(cand[train])
+        assert train.sum() == min(T, C)
+        if T < C:
+            ranks = np.cumsum(cand)[train] - 1
+            g = _gaps(ranks, C)
+            assert set(np.unique(g)) <= {C // T, -(-C // T)}, np.unique(g)
+        sets.append(train)
+    if T < 500:
+        assert not np.array_equal(sets[0], sets[1])

@@ -4,14 +4,21 @@ pixel subsets (needs an MI355X).
   * 12 batched views of uneven size (4x 256x256, 4x 960x540, 2x 1920x1080,
     2x 1280x720) of the 2.75 M-triangle terrain, each on its own seeded
     camera path, one launch per pass for all views, replayed as a HIP graph
-    with the autotuner's kernel configuration.
+    with the autotuner's kernel configuration;
+  * 1920x1080 ReSTIR GI with the BF16 radiance cache (65,536 training
+    samples per frame) on the same terrain;
+  * 3840x2160 ReSTIR GI at depth 6 with FP8 cache weights on the
+    13.6 M-triangle terrain.
 
 After every frame each view's buffers are read back and the oracle restates,
 for a random subset of that view's pixels, the initial sample (from the
 UBO), the temporal pass (from the frame's samples, motion, G-buffer and the
 previous frame's reservoirs, which the frame leaves in place) and the spatial
 pass (from the frame's samples and temporal reservoirs); the results must
-agree bit for bit.
+agree bit for bit.  In cache frames the cached radiance of query and
+training pixels is compared with the oracle's network on the weights after
+the frame's training step instead (1e-2 of the output scale), and the
+training choice (which pixels, how many) bit for bit.
 """
 from __future__ import annotations
 
@@ -94,3 +101,119 @@ def test_c3_batched_views_match_oracle(terrain):
             assert valid > 0.3
     batch.close()
     ctx.close()
+
+
+def _features(samples, mats, lo, inv, idx):
+    from tests.test_gpu_mlp import _features as f
+    return f(samples, mats, lo, inv, idx)
+
+
+def check_cache_frame(ctx, orc, scene, W, H, ubo, frame, px, params, fp8):
+    """One radiance-cache frame on the pixels px: every traced field and the
+    cache kind bit-exact; query and training pixels carry the updated
+    network's prediction (within 1e-2 of the output scale); the exact number
+    of training samples; temporal and spatial passes bit-exact."""
+    from oracle import binding as B
+    samples = ctx.read(N.XR_BUF_SAMPLES).copy()
+    ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H, pixels=px, cache=(params.mlp_train_pixels,
+                                                                           params.mlp_spread))
+    rs = ref["samples"]
+    train_all = ref["train"].astype(bool)
+    g, r = samples[px].copy(), rs[px].copy()
+    kind = r["pad2"]
+    assert np.array_equal(g["pad2"], kind)
+    cached = (kind == 1) | (kind == 2)
+    g["l_out"][cached] = 0
+    r["l_out"][cached] = 0
+    _eq("cache-frame samples", g, r)
+    counts = ctx.counters_all()
+    assert counts[4] == train_all.sum()  # the frame's whole training set, as the oracle chose it
+    # the network's radiance at query / training pixels (a bounded sample of them)
+    w_after = ctx.mlp_params()
+    verts, _, mats, _ = scene.arrays()
+    lo = verts["pos"].min(0).astype(np.float32)
+    inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
+    idx = px[cached][:3000]
+    pred = np.maximum(B.mlp_forward(w_after, _features(samples, mats, lo, inv, idx), mode=2 if fp8 else 1), 0.0)
+    got = samples["l_out"][idx].astype(np.float64)
+    scale = max(1.0, np.abs(pred).max())
+    assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())
+    return int(counts[4]), int(cached.sum())
+
+
+def _passes_after(ctx, orc, W, H, frame, px, params):
+    read = ctx.read
+    from oracle.binding import temporal as orc_temporal
+    n = W * H
+    samples = read(N.XR_BUF_SAMPLES).copy()
+    # the samples' spare words carry the cache kind and x_s material; a
+    # reservoir has no room for them (its planes' spare words hold w_sum, W, M)
+    samples["pad1"] = 0
+    samples["pad2"] = 0
+    motion = read(N.XR_BUF_MOTION).copy()
+    gpos = np.ascontiguousarray(read(N.XR_BUF_GBUF_POS))
+    t_gpu = read(N.XR_BUF_TEMPORAL).copy()
+    s_gpu = read(N.XR_BUF_SPATIAL).copy()
+    cur, wr = frame % 2, 1 - frame % 2
+    t_ref, s_ref = t_gpu.copy(), s_gpu.copy()
+    orc_temporal(W, H, frame, params.temporal_m_cap, samples, motion, gpos, t_ref, s_ref, np.zeros(n, np.uint32),
+                 pixels=px)
+    _eq("temporal reservoirs", t_gpu[px + cur * n], t_ref[px + cur * n])
+    s_ref = s_gpu.copy()
+    orc.spatial(W, H, frame, 1, params.spatial_samples, params.spatial_radius, params.spatial_m_cap,
+                params.visibility, samples, t_gpu, s_ref, np.zeros(n, np.uint32), pixels=px)
+    _eq("spatial reservoirs", s_gpu[px + wr * n], s_ref[px + wr * n])
+
+
+def test_c4_cache_1080p_matches_oracle(terrain):
+    """1920x1080 ReSTIR GI with the BF16 radiance cache on the 2.75 M-triangle
+    terrain, 65,536 training samples per frame."""
+    scene, orc = terrain
+    W, H = 1920, 1080
+    ctx = N.Context(scene, W, H)
+    cam = N.Camera.orbit(2, 8, 128)
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, **PARAMS)
+    assert params.mlp_train_pixels == 65536
+    rng = np.random.default_rng(19)
+    px = np.sort(rng.choice(W * H, size=20000, replace=False)).astype(np.uint32)
+    prev = cam.view(0)
+    for k in range(4):
+        view = cam.view(k)
+        ubo = N.make_ubo(view, prev, 45.0, W, H, bounces=4, frame=k + 1)
+        prev = view
+        ctx.reset_counters()
+        ctx.render(ubo, N.XR_MODE_RESTIR, params)
+        ctx.sync()
+        if k >= 2:
+            say(f"[configs] C4 frame {k + 1}")
+            nt, nc = check_cache_frame(ctx, orc, scene, W, H, ubo, k + 1, px, params, fp8=False)
+            say(f"[configs] C4 frame {k + 1}: {nt} training samples, {nc} cached pixels in the subset")
+            assert nt == 65536
+            _passes_after(ctx, orc, W, H, k + 1, px, params)
+
+
+def test_c5_4k_fp8_depth6_matches_oracle():
+    """3840x2160 ReSTIR GI, max depth 6, FP8 radiance cache, on the
+    13.6 M-triangle terrain (2048^2 heightfield + 4,096 icospheres + 256 lights)."""
+    say("[configs] C5 scene")
+    scene = N.Scene.terrain(1, 2048, 4096, 256)
+    orc = oracle_scene(scene)
+    W, H = 3840, 2160
+    ctx = N.Context(scene, W, H)
+    cam = N.Camera.orbit(2, 8, 128)
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE | N.XR_FLAG_MLP_FP8, **PARAMS)
+    rng = np.random.default_rng(23)
+    px = np.sort(rng.choice(W * H, size=20000, replace=False)).astype(np.uint32)
+    prev = cam.view(0)
+    for k in range(2):
+        view = cam.view(k)
+        ubo = N.make_ubo(view, prev, 45.0, W, H, bounces=6, frame=k + 1)
+        prev = view
+        ctx.reset_counters()
+        ctx.render(ubo, N.XR_MODE_RESTIR, params)
+        ctx.sync()
+        say(f"[configs] C5 frame {k + 1}")
+        nt, nc = check_cache_frame(ctx, orc, scene, W, H, ubo, k + 1, px, params, fp8=True)
+        say(f"[configs] C5 frame {k + 1}: {nt} training samples, {nc} cached pixels in the subset")
+        assert nt == 65536
+        _passes_after(ctx, orc, W, H, k + 1, px, params)

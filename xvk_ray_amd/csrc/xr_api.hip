@@ -86,6 +86,7 @@ struct xr_ctx {
     uint32_t* d_output = nullptr;
     float4* d_radiance = nullptr;
     uint32_t* d_cache_list = nullptr;  // radiance-cache work lists (2 + 2n words)
+    uint32_t* d_cache_work = nullptr;  // training choice: n pending slots + block scan (nblocks + 1)
     float4* d_tail = nullptr;          // initial-pass paths past the head bounces (2 queues of n x 4 float4)
     uint32_t* d_tail_count = nullptr;
     unsigned long long* d_counters = nullptr;
@@ -182,6 +183,7 @@ void free_frame(xr_ctx* c) {
     dev_free(&c->d_output);
     dev_free(&c->d_radiance);
     dev_free(&c->d_cache_list);
+    dev_free(&c->d_cache_work);
     dev_free(&c->d_tail);
     dev_free(&c->d_tail_count);
     dev_free(&c->d_conv);
@@ -205,7 +207,8 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
     XR_HIP(c, dev_alloc(&c->d_tail, 2 * 4 * n * f4));  // two path queues, one per bounce parity
-    XR_HIP(c, dev_alloc(&c->d_tail_count, 2 * sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_tail_count, 3 * sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_cache_work, (n + (n + kCacheSelBlock - 1) / kCacheSelBlock + 1) * sizeof(uint32_t)));
     XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_samples, 0, kPlanes * n * f4, c->stream));
@@ -254,6 +257,8 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.radiance = nullptr;
     f.counters = c->d_counters;
     f.cache_list = c->d_cache_list;
+    f.cache_slot = c->d_cache_work;
+    f.cache_blocks = c->d_cache_work ? c->d_cache_work + f.n : nullptr;
     f.tail = c->d_tail;
     f.tail_count = c->d_tail_count;
     return f;
@@ -690,7 +695,7 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
     } else {
         const uint32_t passes = p.passes ? p.passes : 7u;
         const bool cache_on = (p.flags & XR_FLAG_MLP_CACHE) != 0;
-        CacheCfg cache = {0u, 1u, 0u, 0.0f};
+        CacheCfg cache = {0u, 0u, 0u, 0.0f};
         if (cache_on) {
             rc = ensure_mlp(c);
             if (rc) return rc;
@@ -699,10 +704,9 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
                 c->mlp.fp8 = fp8;
                 XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
             }
-            const uint32_t n = fr.n;
             cache.enabled = 1u;
-            cache.stride = p.mlp_train_pixels >= n ? 1u : n / p.mlp_train_pixels;
-            cache.offset = mlp_train_offset(u.currentFrame, cache.stride);
+            cache.train_target = p.mlp_train_pixels;
+            cache.frame = u.currentFrame;
             cache.spread_c = p.mlp_spread;
         }
         XR_HIP(c, hipEventRecord(ev[0], c->stream));
@@ -1378,6 +1382,8 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
         f.radiance = nullptr;
         f.counters = b->d_counters;
         f.cache_list = nullptr;
+        f.cache_slot = nullptr;
+        f.cache_blocks = nullptr;
         f.tail = b->d_tail;
         f.tail_count = b->d_tail_count;
         b->frames[v] = f;

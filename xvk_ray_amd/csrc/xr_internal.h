@@ -39,9 +39,15 @@ struct FrameDev {
     // Radiance-cache work lists of the frame: [0] query count, [1] training
     // count, [2, 2+n) query pixels, [2+n, 2+2n) training pixels.
     uint32_t* cache_list;
+    // Training choice of the frame (cache mode): per pixel 1 + the index of its
+    // pending path record (0: not a candidate), and per 4096-pixel block the
+    // candidates before it (exclusive scan; entry nblocks = the total).
+    uint32_t* cache_slot;
+    uint32_t* cache_blocks;
     // Paths alive after the head bounces of the initial-sample pass (4 float4
     // each, n capacity) and their count; the single-view pass keeps a second
-    // queue of n paths at tail + 4n, count at tail_count[1].
+    // queue of n paths at tail + 4n, count at tail_count[1], which in cache
+    // mode first holds the candidates' pending records (count tail_count[2]).
     float4* tail;
     uint32_t* tail_count;
 };

@@ -79,10 +79,12 @@ hipError_t launch_mlp_gradient_noalloc(const MlpState& st, float* scratch_ptr, c
 // Cache termination of the initial-sample pass (MLP mode).
 struct CacheCfg {
     uint32_t enabled;
-    uint32_t stride;   // training pixels: pix % stride == offset
-    uint32_t offset;
-    float spread_c;    // terminate at x_s when its spread exceeds spread_c * primary footprint
+    uint32_t train_target;  // training samples per frame (of the candidates with a diffuse x_s)
+    uint32_t frame;         // rotates the training choice
+    float spread_c;         // terminate at x_s when its spread exceeds spread_c * primary footprint
 };
+// Pixels per workgroup of the training-choice kernels (256 threads x 16).
+constexpr uint32_t kCacheSelBlock = 4096;
 
 // Output of the GPU builder.  All arrays are device allocations owned by the
 // caller after a successful build (free with hipFree).

@@ -129,19 +129,24 @@ struct PathOut {
     v3 x_view, n_view, x_s, n_s, L, f0;
     float p_q, uv_u, uv_v;
     uint32_t mat_idx, rand_seed;
-    uint32_t cache;   // 0: plain path, 1: ends in a cache query at x_s, 2: training path
+    uint32_t cache;   // 0: plain path, 1: ends in a cache query at x_s, 2: training path, 3 / 4: pending
     uint32_t xs_mat;  // material at x_s (cache modes)
     bool valid;       // primitiveHitValid
 };
 
 // Camera ray + bounce loop of the initial sample pass (InitialSample.rgen:41-181),
 // split so that a kernel can run a range of bounces and hand the path on.
-// With the radiance cache enabled (no reference counterpart) the path stops
-// at its first diffuse vertex x_s when that vertex's spread
-// t1^2 / (p_q |cos theta_s|) exceeds spread_c times the primary footprint
-// t0^2 / (4 pi |cos theta_v|) (Mueller et al. 2021, "Real-time Neural
-// Radiance Caching"), leaving L_out to the cache; every stride-th pixel keeps
-// tracing to full depth and becomes a training sample.
+// With the radiance cache enabled (no reference counterpart) a path that
+// reaches a diffuse first secondary vertex x_s is a candidate: exactly
+// min(T, candidates) of the frame's candidates, spread evenly over them in
+// pixel order, keep tracing to full depth as training samples; every other
+// candidate stops at x_s when that vertex's spread t1^2 / (p_q |cos theta_s|)
+// exceeds spread_c times the primary footprint t0^2 / (4 pi |cos theta_v|)
+// (Mueller et al. 2021, "Real-time Neural Radiance Caching"), leaving L_out
+// to the cache, and traces on as a plain path otherwise.  The head kernel
+// runs a candidate through bounce 1 and parks it as pending (kind 3, or 4
+// when it would stop into the cache); k_cache_select decides once every
+// pixel's candidacy is known.
 struct PathState {
     v3 origin, dir, thr;
     uint32_t rng;
@@ -162,6 +167,10 @@ __device__ __forceinline__ void path_begin(const UniformBufferObject& ubo, uint3
     o->cache = 0u; o->xs_mat = 0u;
     o->valid = false;
 }
+
+constexpr uint32_t kCachePending = 3u;       // candidate that traces on unless chosen for training
+constexpr uint32_t kCachePendingQuery = 4u;  // candidate that stops into the cache unless chosen
+constexpr uint32_t kRecordAlive = 0x100u;    // pending record: the path is alive after the head bounces
 
 // Bounces b0 .. b1-1 of the loop.  Returns true when the path is still alive
 // at bounce b1 (< bounces), false when it ended (miss, emitter, cache query,
@@ -206,15 +215,8 @@ __device__ __forceinline__ bool path_bounces(const SceneDev& sc, uint32_t x, uin
                 ps->a0 = pl.t * pl.t / (4.0f * XR_PI * fabsf(dot(pl.normal, ps->dir)));
             } else if (bounce == 1) {
                 o->xs_mat = (uint32_t)pl.material;
-                if ((y * w + x) % cache.stride == cache.offset) {
-                    o->cache = 2u;
-                } else {
-                    const float a1 = pl.t * pl.t / (o->p_q * fabsf(dot(pl.normal, ps->dir)));
-                    if (a1 > cache.spread_c * ps->a0) {
-                        o->cache = 1u;
-                        return false;
-                    }
-                }
+                const float a1 = pl.t * pl.t / (o->p_q * fabsf(dot(pl.normal, ps->dir)));
+                o->cache = a1 > cache.spread_c * ps->a0 ? kCachePendingQuery : kCachePending;
             }
         }
         const uint32_t seed_here = ps->rng;
@@ -312,7 +314,7 @@ __device__ __forceinline__ bool initial_head_pixel(const SceneDev& sc, const Fra
     S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
     S[2 * n + pix] = f4(o.x_s, o.p_q);
     S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
-    if (!alive) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
+    if (!alive && o.cache < kCachePending) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
     S[5 * n + pix] = f4(o.f0, 0.0f);
     if (o.valid) {
         fr.gpos[pix] = f4(o.x_view, 1.0f);
@@ -361,16 +363,131 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
         kind = o.cache;
         valid = o.valid ? 1u : 0u;
     }
-    hand_on(alive, fr.tail, fr.tail_count, tc.y * fr.width + tc.x, 0u, ps, o, &lds_count, &lds_base);
+    const uint32_t pix = tc.y * fr.width + tc.x;
+    const bool pending = kind >= kCachePending;
+    hand_on(alive && !pending, fr.tail, fr.tail_count, pix, 0u, ps, o, &lds_count, &lds_base);
+    if (cache.enabled) {  // uniform over the launch
+        // candidates park their record (alive or not) in the second queue
+        const uint32_t q = block_slot(pending, fr.tail_count + 2, &lds_count, &lds_base);
+        if (pending) {
+            o.cache = kind | (alive ? kRecordAlive : 0u);
+            store_path(fr.tail + 4 * (size_t)fr.n + 4 * (size_t)q, pix, 0u, ps, o);
+        }
+        if (tc.in_image) fr.cache_slot[pix] = pending ? q + 1u : 0u;
+    }
     count_rays(&fr.counters[0], rays, &lds_count);
     count_rays(&fr.counters[2], valid, &lds_count);
-    if (cache.enabled) {  // uniform over the launch
-        const uint32_t pix = tc.y * fr.width + tc.x;
-        wave_append(&fr.cache_list[0], fr.cache_list + 2, kind == 1u, pix);
-        wave_append(&fr.cache_list[1], fr.cache_list + 2 + fr.n, kind == 2u, pix);
-        count_rays(&fr.counters[3], kind == 1u ? 1u : 0u, &lds_count);
-        count_rays(&fr.counters[4], kind == 2u ? 1u : 0u, &lds_count);
+}
+
+// ---------------------------------------------------------------------------
+// Training choice of the radiance cache: candidates ranked in pixel order by
+// an exclusive scan of per-pixel flags over 4096-pixel blocks; the chosen ones
+// and the plain candidates join the tail queue, the others end as queries.
+// ---------------------------------------------------------------------------
+// Candidates among the 16 pixels of round r of this thread in its block
+// (pixel block*4096 + r*256 + thread: rounds read coalesced rows).
+__global__ __launch_bounds__(256) void k_cache_count(FrameDev fr) {
+    __shared__ uint32_t part[4];
+    const uint32_t base = blockIdx.x * kCacheSelBlock;
+    uint32_t c = 0;
+    for (uint32_t r = 0; r < kCacheSelBlock / 256; ++r) {
+        const uint32_t p = base + r * 256 + threadIdx.x;
+        if (p < fr.n && fr.cache_slot[p] != 0u) ++c;
     }
+    for (int off = 32; off > 0; off >>= 1) c += (uint32_t)__shfl_down((int)c, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) fr.cache_blocks[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of the nblocks block counts in place (one workgroup of
+// 1024 threads, 1024 counts per step); the total goes to entry nblocks.
+__global__ __launch_bounds__(1024) void k_cache_scan(FrameDev fr, uint32_t nblocks) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < nblocks; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nblocks ? fr.cache_blocks[i] : 0u;
+        uint32_t x = v;  // inclusive scan within the wave
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        uint32_t before = carry;
+        for (uint32_t k = 0; k < wv; ++k) before += wsum[k];
+        if (i < nblocks) fr.cache_blocks[i] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int k = 0; k < 16; ++k) t += wsum[k];
+            carry += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) fr.cache_blocks[nblocks] = carry;
+}
+
+// Decides every pending candidate: rank r in pixel order, r' = (r + o) mod C
+// with o = (frame * 7919) mod C, training when C <= T or
+// floor((r' + 1) T / C) > floor(r' T / C) (exactly min(T, C) of them).
+// Training and plain candidates still alive join the tail queue (tail_count[0]);
+// queries and finished paths write their L_out and kind.  Appends the
+// frame's query / training pixel lists and counts them.
+__global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cache) {
+    __shared__ uint32_t wcount[4];
+    __shared__ uint32_t lds_count;
+    __shared__ uint32_t lds_base;
+    const uint32_t nblocks = gridDim.x;
+    const uint32_t C = fr.cache_blocks[nblocks];
+    const uint64_t T = cache.train_target;
+    const uint32_t o = C ? (cache.frame * 7919u) % C : 0u;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t running = fr.cache_blocks[blockIdx.x];
+    uint32_t nq = 0, nt = 0;
+    const float4* pend = fr.tail + 4 * (size_t)fr.n;
+    for (uint32_t r = 0; r < kCacheSelBlock / 256; ++r) {
+        const uint32_t p = blockIdx.x * kCacheSelBlock + r * 256 + threadIdx.x;
+        const uint32_t slot = p < fr.n ? fr.cache_slot[p] : 0u;
+        const bool cand = slot != 0u;
+        const unsigned long long m = __ballot(cand);
+        if (lane == 0) wcount[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t rank = running + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        for (uint32_t k = 0; k < wv; ++k) rank += wcount[k];
+        const uint32_t round_total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        uint32_t kind = 0u;
+        bool go_on = false;
+        float4 rec[4];
+        if (cand) {
+            for (int k = 0; k < 4; ++k) rec[k] = pend[4 * (size_t)(slot - 1u) + k];
+            const uint32_t w = __float_as_uint(rec[2].w);
+            uint32_t rr = rank + o;
+            if (rr >= C) rr -= C;
+            const bool train = (uint64_t)C <= T || ((uint64_t)rr + 1u) * T / C > (uint64_t)rr * T / C;
+            kind = train ? 2u : ((w & 0xFFu) == kCachePendingQuery ? 1u : 0u);
+            go_on = kind != 1u && (w & kRecordAlive) != 0u;
+            if (!go_on) fr.samples[4 * (size_t)fr.n + p] = make_float4(rec[3].x, rec[3].y, rec[3].z, __uint_as_float(kind));
+            // the material at x_s is kept for cache kinds only (spare word of plane 3)
+            if (kind == 0u) reinterpret_cast<uint32_t*>(fr.samples + 3 * (size_t)fr.n + p)[3] = 0u;
+            rec[2].w = __uint_as_float(kind);
+            nq += kind == 1u;
+            nt += kind == 2u;
+        }
+        const uint32_t q = block_slot(go_on, fr.tail_count, &lds_count, &lds_base);
+        if (go_on)
+            for (int k = 0; k < 4; ++k) fr.tail[4 * (size_t)q + k] = rec[k];
+        wave_append(&fr.cache_list[0], fr.cache_list + 2, cand && kind == 1u, p);
+        wave_append(&fr.cache_list[1], fr.cache_list + 2 + fr.n, cand && kind == 2u, p);
+        running += round_total;
+        __syncthreads();  // wcount is rewritten by the next round
+    }
+    count_rays(&fr.counters[3], nq, &lds_count);
+    count_rays(&fr.counters[4], nt, &lds_count);
 }
 
 // Remaining bounces of one handed-on path; writes its L_out.
@@ -1227,9 +1344,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __r
 // ---------------------------------------------------------------------------
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
                                  const CacheCfg& cache, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, 1u);
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, cache.enabled ? 3u : 1u);
     hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        cache);
+    if (cache.enabled) {
+        const uint32_t nblocks = (fr.n + kCacheSelBlock - 1) / kCacheSelBlock;
+        hipLaunchKernelGGL(k_cache_count, dim3(nblocks), dim3(256), 0, st, fr);
+        hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(1024), 0, st, fr, nblocks);
+        hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(256), 0, st, fr, cache);
+    }
     const uint32_t nb = ubo.numberOfBounces;
     for (uint32_t b = kHeadBounces; b < nb; ++b) {
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count + ((b - kHeadBounces + 1u) & 1u),

@@ -117,9 +117,4 @@ XR_HD void mlp_encode(v3 p, v3 w, v3 n, v3 a, v3 lo, v3 inv_ext, float* out) {
     mlp_encode_tail(w, n, a, out + 6 * kMlpBands);
 }
 
-// Per-frame selection of training pixels: pixel p trains when
-// p % stride == (frame * 7919) % stride, so training slot k is pixel
-// k * stride + offset.
-XR_HD uint32_t mlp_train_offset(uint32_t frame, uint32_t stride) { return (frame * 7919u) % stride; }
-
 }  // namespace xr
