@@ -437,21 +437,29 @@ __global__ __launch_bounds__(1024) void k_cache_scan(FrameDev fr, uint32_t nbloc
 // floor((r' + 1) T / C) > floor(r' T / C) (exactly min(T, C) of them).
 // Training and plain candidates still alive join the tail queue (tail_count[0]);
 // queries and finished paths write their L_out and kind.  Appends the
-// frame's query / training pixel lists and counts them.
+// frame's query / training pixel lists and counts them.  A workgroup decides
+// its 4096 pixels first (slots from LDS counters), then takes one global slot
+// range per list and writes: one global atomic per list and workgroup (a
+// wave-level append hammered two counters 65 k times per frame: 0.58 ms).
 __global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cache) {
+    constexpr uint32_t kRounds = kCacheSelBlock / 256;
     __shared__ uint32_t wcount[4];
-    __shared__ uint32_t lds_count;
-    __shared__ uint32_t lds_base;
+    __shared__ uint32_t cnt[3];   // go on, query list, training list
+    __shared__ uint32_t base[3];
+    __shared__ uint8_t dec[kCacheSelBlock];      // bit 0: go on, bits 1-2: kind (1 query, 2 training), 0xFF: none
+    __shared__ uint16_t idx_go[kCacheSelBlock];  // slot within the workgroup's tail range
+    __shared__ uint16_t idx_ls[kCacheSelBlock];  // slot within the workgroup's query / training range
     const uint32_t nblocks = gridDim.x;
     const uint32_t C = fr.cache_blocks[nblocks];
     const uint64_t T = cache.train_target;
     const uint32_t o = C ? (cache.frame * 7919u) % C : 0u;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0u;
     uint32_t running = fr.cache_blocks[blockIdx.x];
-    uint32_t nq = 0, nt = 0;
     const float4* pend = fr.tail + 4 * (size_t)fr.n;
-    for (uint32_t r = 0; r < kCacheSelBlock / 256; ++r) {
-        const uint32_t p = blockIdx.x * kCacheSelBlock + r * 256 + threadIdx.x;
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        const uint32_t j = r * 256 + threadIdx.x;
+        const uint32_t p = blockIdx.x * kCacheSelBlock + j;
         const uint32_t slot = p < fr.n ? fr.cache_slot[p] : 0u;
         const bool cand = slot != 0u;
         const unsigned long long m = __ballot(cand);
@@ -459,35 +467,54 @@ __global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cach
         __syncthreads();
         uint32_t rank = running + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         for (uint32_t k = 0; k < wv; ++k) rank += wcount[k];
-        const uint32_t round_total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-        uint32_t kind = 0u;
-        bool go_on = false;
-        float4 rec[4];
+        running += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        uint8_t d = 0xFFu;
         if (cand) {
-            for (int k = 0; k < 4; ++k) rec[k] = pend[4 * (size_t)(slot - 1u) + k];
-            const uint32_t w = __float_as_uint(rec[2].w);
+            const float4 r2 = pend[4 * (size_t)(slot - 1u) + 2];
+            const uint32_t w = __float_as_uint(r2.w);
             uint32_t rr = rank + o;
             if (rr >= C) rr -= C;
             const bool train = (uint64_t)C <= T || ((uint64_t)rr + 1u) * T / C > (uint64_t)rr * T / C;
-            kind = train ? 2u : ((w & 0xFFu) == kCachePendingQuery ? 1u : 0u);
-            go_on = kind != 1u && (w & kRecordAlive) != 0u;
-            if (!go_on) fr.samples[4 * (size_t)fr.n + p] = make_float4(rec[3].x, rec[3].y, rec[3].z, __uint_as_float(kind));
-            // the material at x_s is kept for cache kinds only (spare word of plane 3)
-            if (kind == 0u) reinterpret_cast<uint32_t*>(fr.samples + 3 * (size_t)fr.n + p)[3] = 0u;
-            rec[2].w = __uint_as_float(kind);
-            nq += kind == 1u;
-            nt += kind == 2u;
+            const uint32_t kind = train ? 2u : ((w & 0xFFu) == kCachePendingQuery ? 1u : 0u);
+            const bool go_on = kind != 1u && (w & kRecordAlive) != 0u;
+            d = (uint8_t)((go_on ? 1u : 0u) | (kind << 1));
+            if (go_on) idx_go[j] = (uint16_t)atomicAdd(&cnt[0], 1u);
+            if (kind != 0u) idx_ls[j] = (uint16_t)atomicAdd(&cnt[kind], 1u);
         }
-        const uint32_t q = block_slot(go_on, fr.tail_count, &lds_count, &lds_base);
-        if (go_on)
-            for (int k = 0; k < 4; ++k) fr.tail[4 * (size_t)q + k] = rec[k];
-        wave_append(&fr.cache_list[0], fr.cache_list + 2, cand && kind == 1u, p);
-        wave_append(&fr.cache_list[1], fr.cache_list + 2 + fr.n, cand && kind == 2u, p);
-        running += round_total;
+        dec[j] = d;
         __syncthreads();  // wcount is rewritten by the next round
     }
-    count_rays(&fr.counters[3], nq, &lds_count);
-    count_rays(&fr.counters[4], nt, &lds_count);
+    if (threadIdx.x < 3) {
+        uint32_t* ctr = threadIdx.x == 0 ? fr.tail_count : fr.cache_list + (threadIdx.x - 1u);
+        base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(ctr, cnt[threadIdx.x]) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        const uint32_t j = r * 256 + threadIdx.x;
+        const uint8_t d = dec[j];
+        if (d == 0xFFu) continue;
+        const uint32_t p = blockIdx.x * kCacheSelBlock + j;
+        const uint32_t slot = fr.cache_slot[p];
+        const uint32_t kind = d >> 1;
+        const float4* src = pend + 4 * (size_t)(slot - 1u);
+        if (d & 1u) {
+            float4* dst = fr.tail + 4 * (size_t)(base[0] + idx_go[j]);
+            dst[0] = src[0];
+            dst[1] = src[1];
+            const float4 r2 = src[2];
+            dst[2] = make_float4(r2.x, r2.y, r2.z, __uint_as_float(kind));
+            dst[3] = src[3];
+        } else {
+            const float4 r3 = src[3];
+            fr.samples[4 * (size_t)fr.n + p] = make_float4(r3.x, r3.y, r3.z, __uint_as_float(kind));
+        }
+        // the material at x_s is kept for cache kinds only (spare word of plane 3)
+        if (kind == 0u) reinterpret_cast<uint32_t*>(fr.samples + 3 * (size_t)fr.n + p)[3] = 0u;
+        else fr.cache_list[2 + (kind == 2u ? fr.n : 0u) + base[kind] + idx_ls[j]] = p;
+    }
+    __shared__ uint32_t lds_count;
+    count_rays(&fr.counters[3], threadIdx.x == 0 ? cnt[1] : 0u, &lds_count);
+    count_rays(&fr.counters[4], threadIdx.x == 0 ? cnt[2] : 0u, &lds_count);
 }
 
 // Remaining bounces of one handed-on path; writes its L_out.
