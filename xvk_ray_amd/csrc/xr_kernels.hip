@@ -293,7 +293,13 @@ __device__ __forceinline__ void store_path(float4* rec, uint32_t pix, uint32_t v
 
 // Head of the initial-sample pass for pixel (x, y): bounces 0 and 1 and every
 // write of the pass except, for a path still alive, L_out (written by the
-// tail).  Returns whether the path is alive.
+// tail).  Returns whether the path is alive.  The loop body is that of
+// path_bounces for bounces 0 and 1, in one loop with one traversal; what
+// bounce 0 settles (visible point, normal, seed, material, f, G-buffer,
+// motion) is written before the bounce-1 ray is traced, so it is not live
+// across that traversal (the head kernel's registers spilled to scratch:
+// 288 bytes per lane, 620 MB of writes per 1080p frame against 282 MB
+// compulsory).
 template <int kLds>
 __device__ __forceinline__ bool initial_head_pixel(const SceneDev& sc, const FrameDev& fr,
                                                    const UniformBufferObject& ubo, uint32_t x, uint32_t y,
@@ -301,40 +307,105 @@ __device__ __forceinline__ bool initial_head_pixel(const SceneDev& sc, const Fra
                                                    uint32_t* rays) {
     PathOut& o = *po;
     const uint32_t pix = y * fr.width + x;
-    path_begin(ubo, x, y, fr.width, fr.height, ubo.currentFrame, ps, &o);
-    const uint32_t nb = ubo.numberOfBounces;
-    const bool alive = path_bounces<kLds>(sc, x, y, fr.width, nb, 0u, nb < kHeadBounces ? nb : kHeadBounces,
-                                          ubo.hasSkyBox != 0u, ps, col, &o, cache, rays);
-    if (!alive) path_end(&o);
     const uint32_t n = fr.n;
     float4* S = fr.samples;
+    path_begin(ubo, x, y, fr.width, fr.height, ubo.currentFrame, ps, &o);
+    const uint32_t nb = ubo.numberOfBounces;
+    const bool sky = ubo.hasSkyBox != 0u;
+    float p_q = 0.0f;
+    bool alive = false;
+#pragma unroll
+    for (uint32_t bounce = 0; bounce < kHeadBounces; ++bounce) {
+        if (bounce >= nb) break;
+        alive = false;
+        const RayPre rp = make_ray_pre(ps->origin, ps->dir);
+        Hit hit;
+        ++*rays;
+        // A hit whose material index is -1 reads as a miss (InitialSample.rgen:102-103).
+        Payload pl;
+        bool found = trace_ray<kBlock, false, kLds>(sc.nodes, sc.tris, rp, 0.0001f, 10000.0f, col, &hit);
+        if (found) {
+            closest_hit(sc, hit, &pl);
+            found = pl.material != -1;
+        }
+        bool go_on = false;
+        if (!found) {
+            if (sky) o.L = add(o.L, ps->thr);
+            if (bounce == 1) {
+                o.x_s = add(ps->origin, muls(ps->dir, 1000.0f));
+                o.n_s = neg(ps->dir);
+            }
+        } else {
+            const float4 m0 = sc.materials[2 * pl.material];
+            const float4 m1 = sc.materials[2 * pl.material + 1];
+            if (__float_as_uint(m1.w) == kDiffuseLight) {
+                // untextured baseColor.rgb * 4 (InitialSample.rgen:117-120)
+                o.L = add(o.L, muls(mul(ps->thr, mk3(m0.x, m0.y, m0.z)), 4.0f));
+                if (bounce == 1) { o.x_s = pl.position; o.n_s = pl.normal; }
+                if (bounce == 0) { o.x_view = pl.position; o.n_view = pl.normal; }
+            } else {
+                if (bounce == 1) { o.x_s = pl.position; o.n_s = pl.normal; }
+                if (cache.enabled) {
+                    if (bounce == 0) {
+                        ps->a0 = pl.t * pl.t / (4.0f * XR_PI * fabsf(dot(pl.normal, ps->dir)));
+                    } else {
+                        o.xs_mat = (uint32_t)pl.material;
+                        const float a1 = pl.t * pl.t / (p_q * fabsf(dot(pl.normal, ps->dir)));
+                        o.cache = a1 > cache.spread_c * ps->a0 ? kCachePendingQuery : kCachePending;
+                    }
+                }
+                const uint32_t seed_here = ps->rng;
+                v3 wi;
+                float pdf, cos_t;
+                const v3 f = sample_bsdf(sc, pl.normal, m0, m1, pl.u, pl.v, &wi, &pdf, &cos_t, &ps->rng);
+                if (bounce == 0) {
+                    o.valid = true;
+                    p_q = pdf;
+                    // what bounce 0 settles, written now
+                    S[0 * n + pix] = f4(pl.position, __uint_as_float((uint32_t)pl.material));
+                    S[1 * n + pix] = f4(pl.normal, __uint_as_float(seed_here));
+                    S[5 * n + pix] = f4(f, 0.0f);
+                    fr.gpos[pix] = f4(pl.position, 1.0f);
+                    fr.gnrm[pix] = f4(pl.normal, 0.0f);
+                    fr.gmatuv[pix] = make_float4((float)pl.material, pl.u, pl.v, 0.0f);
+                    const float cu = ((float)x + 0.5f) / (float)fr.width;
+                    const float cv = ((float)y + 0.5f) / (float)fr.height;
+                    float vp[4], cp[4];
+                    mat_vec(ubo.viewPrev, pl.position.x, pl.position.y, pl.position.z, 1.0f, vp);
+                    mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
+                    const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
+                    const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
+                    const uint32_t mx = f32_to_f16_bits(cu - pu), my = f32_to_f16_bits(cv - pv);
+                    fr.motion[pix] = mx | (my << 16);
+                } else {
+                    ps->thr = mul(ps->thr, divs(muls(f, cos_t), pdf));
+                }
+                const v3 hp = add(ps->origin, muls(ps->dir, pl.t));
+                const v3 g = pl.geo_normal;
+                const v3 od = dot(wi, g) > 0.0f ? g : neg(g);
+                ps->origin = add(hp, muls(od, 0.01f));
+                ps->dir = wi;
+                go_on = true;
+            }
+        }
+        if (!go_on) break;
+        alive = bounce + 1u < nb;  // a path past bounce 1 goes on when the loop has bounces left
+    }
+    if (!alive) path_end(&o);
     // Spare words (zero without the cache): plane 3 .w = material at x_s,
     // plane 4 .w = 1 cache query / 2 training sample.
-    S[0 * n + pix] = f4(o.x_view, __uint_as_float(o.mat_idx));
-    S[1 * n + pix] = f4(o.n_view, __uint_as_float(o.rand_seed));
-    S[2 * n + pix] = f4(o.x_s, o.p_q);
-    S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
-    if (!alive && o.cache < kCachePending) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
-    S[5 * n + pix] = f4(o.f0, 0.0f);
-    if (o.valid) {
-        fr.gpos[pix] = f4(o.x_view, 1.0f);
-        fr.gnrm[pix] = f4(o.n_view, 0.0f);
-        fr.gmatuv[pix] = make_float4((float)o.mat_idx, o.uv_u, o.uv_v, 0.0f);
-        const float cu = ((float)x + 0.5f) / (float)fr.width;
-        const float cv = ((float)y + 0.5f) / (float)fr.height;
-        float vp[4], cp[4];
-        mat_vec(ubo.viewPrev, o.x_view.x, o.x_view.y, o.x_view.z, 1.0f, vp);
-        mat_vec(ubo.projectionPrev, vp[0], vp[1], vp[2], vp[3], cp);
-        const float pu = (cp[0] / cp[3]) * 0.5f + 0.5f;
-        const float pv = (cp[1] / cp[3]) * 0.5f + 0.5f;
-        const uint32_t mx = f32_to_f16_bits(cu - pu), my = f32_to_f16_bits(cv - pv);
-        fr.motion[pix] = mx | (my << 16);
-    } else {
+    if (!o.valid) {
+        S[0 * n + pix] = f4(o.x_view, __uint_as_float(0xFFFFFFFFu));
+        S[1 * n + pix] = f4(o.n_view, 0.0f);
+        S[5 * n + pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         fr.gpos[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         fr.gnrm[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         fr.gmatuv[pix] = make_float4(-1.f, 0.f, 0.f, 0.f);
         fr.motion[pix] = 0u;
     }
+    S[2 * n + pix] = f4(o.x_s, p_q);
+    S[3 * n + pix] = f4(o.n_s, __uint_as_float(o.cache ? o.xs_mat : 0u));
+    if (!alive && o.cache < kCachePending) S[4 * n + pix] = f4(o.L, __uint_as_float(o.cache));
     return alive;
 }
 
