@@ -87,6 +87,11 @@ struct xr_ctx {
     float4* d_radiance = nullptr;
     uint32_t* d_cache_list = nullptr;  // radiance-cache work lists (2 + 2n words)
     uint32_t* d_cache_work = nullptr;  // training choice: n pending slots + block scan (nblocks + 1)
+    float4* d_nbr = nullptr;           // spatial-pass neighbour records (4 float4 per pixel)
+    // The records hold the samples and temporal reservoirs of slot nbr_ping as
+    // the temporal pass left them; a buffer write or resize invalidates them.
+    bool nbr_ok = false;
+    uint32_t nbr_ping = 0;
     float4* d_tail = nullptr;          // initial-pass paths past the head bounces (2 queues of n x 4 float4)
     uint32_t* d_tail_count = nullptr;
     unsigned long long* d_counters = nullptr;
@@ -184,6 +189,8 @@ void free_frame(xr_ctx* c) {
     dev_free(&c->d_radiance);
     dev_free(&c->d_cache_list);
     dev_free(&c->d_cache_work);
+    dev_free(&c->d_nbr);
+    c->nbr_ok = false;
     dev_free(&c->d_tail);
     dev_free(&c->d_tail_count);
     dev_free(&c->d_conv);
@@ -208,6 +215,7 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
     XR_HIP(c, dev_alloc(&c->d_tail, 2 * 4 * n * f4));  // two path queues, one per bounce parity
     XR_HIP(c, dev_alloc(&c->d_tail_count, 3 * sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_nbr, 4 * n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_work, (n + (n + kCacheSelBlock - 1) / kCacheSelBlock + 1) * sizeof(uint32_t)));
     XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));
     XR_HIP(c, hipMemsetAsync(c->d_radiance, 0, n * f4, c->stream));
@@ -254,6 +262,7 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.motion = c->d_motion;
     f.output = c->d_output;
     f.accum = c->d_accum;
+    f.nbr = c->d_nbr;
     f.radiance = nullptr;
     f.counters = c->d_counters;
     f.cache_list = c->d_cache_list;
@@ -711,7 +720,10 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
         }
         XR_HIP(c, hipEventRecord(ev[0], c->stream));
         if (cache_on && (passes & XR_PASS_INITIAL)) XR_HIP(c, launch_clear_u32(fr.cache_list, 2, c->stream));
-        if (passes & XR_PASS_INITIAL) XR_HIP(c, launch_initial_sample(sc, fr, u, cache, c->stream));
+        if (passes & XR_PASS_INITIAL) {
+            XR_HIP(c, launch_initial_sample(sc, fr, u, cache, c->stream));
+            c->nbr_ok = false;  // new samples
+        }
         XR_HIP(c, hipEventRecord(ev[1], c->stream));
         if (cache_on && (passes & XR_PASS_INITIAL)) {
             // one training step on this frame's path suffixes, then the
@@ -722,9 +734,20 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
             XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
         }
         XR_HIP(c, hipEventRecord(ev[2], c->stream));
-        if (passes & XR_PASS_TEMPORAL) XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
+        if (passes & XR_PASS_TEMPORAL) {
+            XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
+            c->nbr_ok = true;
+            c->nbr_ping = u.currentFrame % 2u;
+        }
         XR_HIP(c, hipEventRecord(ev[3], c->stream));
-        if (passes & XR_PASS_SPATIAL) XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+        if (passes & XR_PASS_SPATIAL) {
+            if (!c->nbr_ok || c->nbr_ping != u.currentFrame % 2u) {
+                XR_HIP(c, launch_nbr_build(fr, u.currentFrame, c->stream));
+                c->nbr_ok = true;
+                c->nbr_ping = u.currentFrame % 2u;
+            }
+            XR_HIP(c, launch_spatial(sc, fr, u.currentFrame, u.reuse, pp, c->stream));
+        }
         XR_HIP(c, hipEventRecord(ev[4], c->stream));
         set.n = 5;
     }
@@ -965,6 +988,7 @@ int xr_write_buffer(xr_ctx* c, int which, const void* src, size_t bytes, int lay
     void* dst = buffer_ptr(c, which);
     if (!dst) return set_err(c, XR_ERR_STATE, "buffer %d has not been created yet", which);
     const bool planar = which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL || which == XR_BUF_SPATIAL;
+    if (which == XR_BUF_SAMPLES || which == XR_BUF_TEMPORAL) c->nbr_ok = false;
     if (!planar || layout == XR_LAYOUT_NATIVE) {
         XR_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
         XR_HIP(c, hipStreamSynchronize(c->stream));
@@ -1186,7 +1210,7 @@ struct xr_batch {
     struct View {
         uint32_t w = 0, h = 0;
         float4 *samples = nullptr, *temporal = nullptr, *spatial = nullptr, *gpos = nullptr, *gnrm = nullptr,
-               *gmatuv = nullptr;
+               *gmatuv = nullptr, *nbr = nullptr;
         uint32_t *motion = nullptr, *output = nullptr;
     };
     std::vector<View> views;
@@ -1218,6 +1242,7 @@ void batch_free(xr_batch* b) {
         dev_free(&v.gmatuv);
         dev_free(&v.motion);
         dev_free(&v.output);
+        dev_free(&v.nbr);
     }
     dev_free(&b->d_frames);
     dev_free(&b->d_ubos);
@@ -1355,6 +1380,7 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
         XR_BATCH_HIP(dev_alloc(&vw.gmatuv, n * f4));
         XR_BATCH_HIP(dev_alloc(&vw.motion, n * 4));
         XR_BATCH_HIP(dev_alloc(&vw.output, n * 4));
+        XR_BATCH_HIP(dev_alloc(&vw.nbr, 4 * n * f4));
     }
     XR_BATCH_HIP(dev_alloc(&b->d_frames, sizeof(FrameDev) * view_count));
     XR_BATCH_HIP(dev_alloc(&b->d_ubos, sizeof(UniformBufferObject) * view_count));
@@ -1379,6 +1405,7 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
         f.motion = vw.motion;
         f.output = vw.output;
         f.accum = nullptr;
+        f.nbr = vw.nbr;
         f.radiance = nullptr;
         f.counters = b->d_counters;
         f.cache_list = nullptr;

@@ -34,6 +34,12 @@ struct FrameDev {
     uint32_t* motion;           // binding 15, RG16F
     uint32_t* output;           // binding 2, B8G8R8A8_UNORM
     float4* accum;              // binding 1, RGBA32F
+    // Spatial-reuse neighbour records (no reference binding), one 64-byte
+    // record per pixel, written by the temporal pass: [0] the pixel's sample
+    // x_view + material, [1] its n_view + |L_out| of its temporal reservoir,
+    // [2] that reservoir's x_sample + W, [3] its n_sample + M.  A neighbour
+    // gather reads one record instead of six 16-byte planes.
+    float4* nbr;
     float4* radiance;           // linear colour behind the output (no reference binding), or null
     unsigned long long* counters;  // [0] extend rays, [1] shadow rays, [2] valid hits, [3] / [4] cache
     // Radiance-cache work lists of the frame: [0] query count, [1] training
@@ -169,15 +175,24 @@ __device__ __forceinline__ void wide_slabs(const WideNode* __restrict__ nd, cons
     const float ax = (q.x - r.org.x) * r.idir.x, bx = __uint_as_float((meta & 0xFFu) << 23) * r.idir.x;
     const float ay = (q.y - r.org.y) * r.idir.y, by = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * r.idir.y;
     const float az = (q.z - r.org.z) * r.idir.z, bz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * r.idir.z;
+    // A plane's distance fma(g, b, a) is monotonic in g, increasing where the
+    // reciprocal direction is positive (it is never zero or NaN, see
+    // make_ray_pre): the near plane of every child is its low plane on such
+    // an axis and its high plane otherwise, which is what min/max of the two
+    // distances would pick, with 6 selects per node instead of 6 min/max per
+    // child.
+    const bool px = r.idir.x >= 0.0f, py = r.idir.y >= 0.0f, pzp = r.idir.z >= 0.0f;
+    const uint32_t nx = px ? p.x : p.y, fx = px ? p.y : p.x;
+    const uint32_t ny = py ? p.z : p.w, fy = py ? p.w : p.z;
+    const uint32_t nz = pzp ? pz.x : pz.y, fz = pzp ? pz.y : pz.x;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int sh = 8 * k;
-        const float t0x = fmaf((float)((p.x >> sh) & 0xFFu), bx, ax), t1x = fmaf((float)((p.y >> sh) & 0xFFu), bx, ax);
-        const float t0y = fmaf((float)((p.z >> sh) & 0xFFu), by, ay), t1y = fmaf((float)((p.w >> sh) & 0xFFu), by, ay);
-        const float t0z = fmaf((float)((pz.x >> sh) & 0xFFu), bz, az),
-                    t1z = fmaf((float)((pz.y >> sh) & 0xFFu), bz, az);
-        const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        const float t0x = fmaf((float)((nx >> sh) & 0xFFu), bx, ax), t1x = fmaf((float)((fx >> sh) & 0xFFu), bx, ax);
+        const float t0y = fmaf((float)((ny >> sh) & 0xFFu), by, ay), t1y = fmaf((float)((fy >> sh) & 0xFFu), by, ay);
+        const float t0z = fmaf((float)((nz >> sh) & 0xFFu), bz, az), t1z = fmaf((float)((fz >> sh) & 0xFFu), bz, az);
+        const float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
+        float tf = fminf(fminf(t1x, t1y), t1z);
         tf = fminf(tf * 1.0000003576f, tb);
         h[k] = tn <= tf && (uint32_t)k < cnt;
         e[k] = tn;

@@ -26,6 +26,9 @@ uint32_t batch_config_count();
 void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds);
 hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
                               const PassParams& pp, hipStream_t st);
+// Neighbour records of the spatial pass from the samples and the temporal
+// reservoirs of frame's slot (when the temporal pass did not write them).
+hipError_t launch_nbr_build(const FrameDev& fr, uint32_t frame, hipStream_t st);
 hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
                           const PassParams& pp, hipStream_t st);
 hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo, uint32_t max_depth,

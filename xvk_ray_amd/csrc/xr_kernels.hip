@@ -803,6 +803,29 @@ __device__ __forceinline__ v3 shade_reservoir(const Res& r, v3 x_curr, v3 n_curr
     return mk3(Lo.x * W * f.x * c, Lo.y * W * f.y * c, Lo.z * W * f.z * c);
 }
 
+// The spatial pass's neighbour record of pixel pix: its sample's planes 0 and
+// 1 (x_view + material, n_view) and, of its temporal reservoir r, |L_out|
+// (the p_hat of Spatial.rgen:131, computed as there), W, x_sample, n_sample, M.
+__device__ __forceinline__ void store_nbr(float4* __restrict__ nbr, uint32_t pix, float4 s0, float4 s1, const Res& r) {
+    float4* q = nbr + 4 * (size_t)pix;
+    q[0] = s0;
+    q[1] = make_float4(s1.x, s1.y, s1.z, length(xyz(r.p[4])));
+    q[2] = make_float4(r.p[2].x, r.p[2].y, r.p[2].z, r_W(r));
+    q[3] = make_float4(r.p[3].x, r.p[3].y, r.p[3].z, r.p[5].w);
+}
+
+// Neighbour records from the samples and the temporal reservoirs of slot
+// `ping` (the spatial pass run without the temporal pass before it).
+__global__ __launch_bounds__(256) void k_nbr_build(FrameDev fr, uint32_t ping) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= fr.n) return;
+    const uint32_t n = fr.n;
+    const float4* rd = fr.temporal + (size_t)ping * kPlanes * n;
+    Res r;
+    load_planes(rd, n, pix, r.p);
+    store_nbr(fr.nbr, pix, fr.samples[pix], fr.samples[n + pix], r);
+}
+
 // ---------------------------------------------------------------------------
 // Temporal reuse pass (HBM bound; no rays)
 // ---------------------------------------------------------------------------
@@ -824,6 +847,7 @@ __device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t fram
         store_planes(cur, n, pix, r0.p);
         store_planes(spat_cur, n, pix, r0.p);
         if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        store_nbr(fr.nbr, pix, s[0], s[1], r0);
         return;
     }
     bool found = false;
@@ -869,6 +893,7 @@ __device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t fram
     if (tp_final > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * tp_final));
     else set_W(&r, 0.0f);
     store_planes(cur, n, pix, r.p);
+    store_nbr(fr.nbr, pix, s[0], s[1], r);
     v3 color = mk3(0.f, 0.f, 0.f);
     if (r_W(r) > 0.0f) color = shade_reservoir(r, x_curr, xyz(s[1]));
     fr.output[pix] = resolve_bgra8(color, 1.0f);
@@ -925,8 +950,9 @@ __device__ __forceinline__ bool spatial_begin(SpatialTask& t, const FrameDev& fr
     t.pix = pix;
     t.x = x;
     t.y = y;
-    t.s0 = fr.samples[pix];
-    t.s1 = fr.samples[n + pix];
+    const float4* c = fr.nbr + 4 * (size_t)pix;
+    t.s0 = c[0];
+    t.s1 = c[1];
     const float4 s3 = fr.samples[3 * n + pix];
     if (length(xyz(t.s1)) == 0.0f) {
         fr.output[pix] = 0xFF000000u;
@@ -945,9 +971,8 @@ __device__ __forceinline__ bool spatial_begin(SpatialTask& t, const FrameDev& fr
     t.ws = 0.0f;
     t.i = 0;
     // MergeReservoir(r, centerRes, |L|)
-    const float4 c4 = rd[4 * n + pix], c5 = rd[5 * n + pix];
-    const uint32_t mc = __float_as_uint(c5.w);
-    const float wc = length(xyz(c4)) * c4.w * (float)mc;
+    const uint32_t mc = __float_as_uint(c[3].w);
+    const float wc = t.s1.w * c[2].w * (float)mc;
     t.ws = t.ws + wc;
     if (rand01(&t.rng) < wc / maxf(1e-10f, t.ws)) {
         t.sel = pix;
@@ -990,11 +1015,12 @@ __device__ __forceinline__ bool spatial_step(SpatialTask& t, const FrameDev& fr,
         nx = nx < 0 ? 0 : (nx > (int)w - 1 ? (int)w - 1 : nx);
         ny = ny < 0 ? 0 : (ny > (int)h - 1 ? (int)h - 1 : ny);
         const uint32_t nidx = (uint32_t)ny * w + (uint32_t)nx;
-        const float4 q0 = fr.samples[nidx];
-        const float4 q1 = fr.samples[n + nidx];
+        const float4* nb = fr.nbr + 4 * (size_t)nidx;
+        const float4 q0 = nb[0];
+        const float4 q1 = nb[1];
         if (length(xyz(q1)) == 0.0f || !similar(t.s0, t.s1, q0, q1)) continue;
-        const float4 rn2 = rd[2 * n + nidx], rn3 = rd[3 * n + nidx], rn4 = rd[4 * n + nidx];
-        const uint32_t mn = __float_as_uint(rd[5 * n + nidx].w);
+        const float4 rn2 = nb[2], rn3 = nb[3];
+        const uint32_t mn = __float_as_uint(rn3.w);
         const v3 xs = xyz(rn2);
         const v3 ns = xyz(rn3);
         const v3 vo = sub(xyz(q0), xs);
@@ -1010,7 +1036,7 @@ __device__ __forceinline__ bool spatial_step(SpatialTask& t, const FrameDev& fr,
             if (co > 1e-6f) jac = (cc * d2o) / (co * d2c);
         }
         jac = clampf(jac, 0.1f, 10.0f);
-        const float ph = length(xyz(rn4)) * jac;
+        const float ph = q1.w * jac;
         // CheckVisibility (Spatial.rgen:44-62) from the currently selected
         // sample's visible point, any-hit.  Its only effect is to zero ph,
         // so it is traced only when ph is not already zero (same result,
@@ -1022,12 +1048,12 @@ __device__ __forceinline__ bool spatial_step(SpatialTask& t, const FrameDev& fr,
             t.rdir = normalize(dv);
             t.tmax = dist - 0.01f;
             t.pph = ph;
-            t.pW = rn4.w;
+            t.pW = rn2.w;
             t.pM = mn;
             t.pnidx = nidx;
             return true;
         }
-        spatial_merge(t, rd, n, ph, rn4.w, mn, nidx);
+        spatial_merge(t, rd, n, ph, rn2.w, mn, nidx);
     }
     return false;
 }
@@ -1463,6 +1489,10 @@ hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const U
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
                            hipStream_t st) {
     hipLaunchKernelGGL(k_temporal, dim3((fr.n + 255) / 256), dim3(256), 0, st, fr, frame, pp, gpos_hist);
+    return hipGetLastError();
+}
+hipError_t launch_nbr_build(const FrameDev& fr, uint32_t frame, hipStream_t st) {
+    hipLaunchKernelGGL(k_nbr_build, dim3((fr.n + 255) / 256), dim3(256), 0, st, fr, frame % 2u);
     return hipGetLastError();
 }
 hipError_t launch_spatial(const SceneDev& sc, const FrameDev& fr, uint32_t frame, uint32_t reuse,
