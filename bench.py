@@ -66,10 +66,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-mlp-study", action="store_true")
     ap.add_argument("--study-frames", type=int, default=128)
-    ap.add_argument("--conv-spp", type=int, default=65536)
-    ap.add_argument("--builder", choices=("lbvh", "ploc", "sah"), default="sah",
-                    help="BVH builder (the build itself is outside the timed region): binned SAH on host "
-                         "threads (default: the best trees), or the GPU builders LBVH / PLOC")
+    ap.add_argument("--conv-spp", type=int, default=65536, help="converged image of the cache study")
+    ap.add_argument("--c5-conv-spp", type=int, default=4096,
+                    help="converged images of the 3840x2160 workload (three checkpoint views)")
+    ap.add_argument("--builder", choices=("lbvh", "ploc", "sah", "sah_host"), default="sah",
+                    help="BVH builder (the build itself is outside the timed region): binned SAH on the "
+                         "GPU (default: the best trees), the same SAH tree built on host threads, or the "
+                         "GPU builders LBVH / PLOC")
     ap.add_argument("--workload", choices=("c2", "c1", "c3", "c5", "scenes"), default="c2",
                     help="c2: 1080p ReSTIR GI (the headline line); c1: 640x360 plain path tracing on the "
                          "Cornell box; c3: 12 batched views of the 1080p scene (HIP graph, autotuned); "
@@ -295,7 +298,8 @@ def log(msg: str) -> None:
 
 def builder_id(args) -> int:
     from xvk_ray_amd import native as N
-    return {"lbvh": N.XR_BUILDER_LBVH, "ploc": N.XR_BUILDER_PLOC, "sah": N.XR_BUILDER_SAH}[args.builder]
+    return {"lbvh": N.XR_BUILDER_LBVH, "ploc": N.XR_BUILDER_PLOC, "sah": N.XR_BUILDER_SAH,
+            "sah_host": N.XR_BUILDER_SAH_HOST}[args.builder]
 
 
 def base_result(args, metric, value, unit, ms, dtype, config) -> dict:
@@ -489,8 +493,8 @@ def run_c5(args) -> dict:
         ubo = saved[(False, f)][1]
         ctx.converge_reset()
         done = 0
-        while done < args.conv_spp:
-            k = min(32, args.conv_spp - done)
+        while done < args.c5_conv_spp:
+            k = min(32, args.c5_conv_spp - done)
             ctx.converge(ubo, k, (1 << 20) + done)
             ctx.sync()
             done += k
@@ -508,7 +512,7 @@ def run_c5(args) -> dict:
                        "triangles": scene.triangle_count, "max_depth": depth, "frames": frames,
                        "camera": "seeded 128-frame orbit", "parallelism": "single_gpu"})
     res["steps"], res["warmup"] = frames, 0
-    res.update({"plain_restir": res_plain, "mlp_cache_fp8": res_cache, "converged_spp": args.conv_spp,
+    res.update({"plain_restir": res_plain, "mlp_cache_fp8": res_cache, "converged_spp": args.c5_conv_spp,
                 "time_ratio_cache_vs_plain": round(ms / res_plain["ms_per_frame_gpu"], 4)})
     return res
 

@@ -184,12 +184,14 @@ int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ct
 /* Builds the acceleration structure (CreateAccelerationStructures,
  * ApplicationRT.cpp:55-66).  builder: XR_BUILDER_LBVH (Morton-order radix
  * tree on the GPU, fastest build), XR_BUILDER_PLOC (locally-ordered
- * clustering on the GPU) or XR_BUILDER_SAH (binned surface-area heuristic,
- * top down on host threads: the slowest build, usually the best trees).
- * Every builder gives the same ray/triangle hits. */
+ * clustering on the GPU), XR_BUILDER_SAH (binned surface-area heuristic, top
+ * down on the GPU: usually the best trees) or XR_BUILDER_SAH_HOST (the same
+ * binned SAH on host threads; the same tree, for comparison).  Every builder
+ * gives the same ray/triangle hits. */
 #define XR_BUILDER_LBVH 0
 #define XR_BUILDER_PLOC 1
 #define XR_BUILDER_SAH 2
+#define XR_BUILDER_SAH_HOST 3
 int xr_build_accel(xr_ctx* ctx, int builder);
 
 /* Base-colour textures (descriptor binding 8, TextureSamplers[],

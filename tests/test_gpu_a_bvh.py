@@ -91,7 +91,7 @@ def check_bvh(nodes: np.ndarray, tris: np.ndarray, tri_count: int) -> int:
     return max_depth
 
 
-@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH])
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH, N.XR_BUILDER_SAH_HOST])
 @pytest.mark.parametrize("which", ["cornell", "terrain"])
 def test_bvh_structure(which, builder):
     scene = cornell_scene() if which == "cornell" else small_terrain()
@@ -133,7 +133,7 @@ def test_bvh_structure(which, builder):
     assert np.array_equal(canon, v["pos"][vi])
 
 
-@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH])
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH, N.XR_BUILDER_SAH_HOST])
 def test_bvh_two_triangles_and_rebuild(builder):
     """Smallest clustered tree, and a rebuild of one context with the other builder."""
     verts = np.zeros(6, N.VERTEX_DTYPE)
@@ -171,3 +171,38 @@ def test_bvh_single_triangle():
     h = ctx.trace(r)
     assert h["t"][0] == np.float32(1.0) and h["triangle"][0] == 0
     assert h["t"][1] == -1.0 and h["triangle"][1] == 0xFFFFFFFF
+
+
+def _leaf_sets(nodes: np.ndarray, tris: np.ndarray):
+    """The tree's leaves as sorted tuples of canonical triangle ids, sorted."""
+    _, _, links, counts = decode_nodes(nodes)
+    ids = tris[:, 3].copy().view(np.uint32)
+    out = []
+    for i in range(len(nodes)):
+        for k in range(int(counts[i])):
+            if links[i, k] < 0:
+                code = ~int(links[i, k])
+                first, n = code >> 4, (code & 15) + 1
+                out.append(tuple(sorted(ids[first:first + n].tolist())))
+    return sorted(out)
+
+
+@pytest.mark.parametrize("which", ["cornell", "terrain", "terrain_big"])
+def test_gpu_sah_builds_the_host_sah_tree(which):
+    """The GPU binned-SAH builder makes the host builder's decisions: the same
+    leaves (as triangle sets), node count and depth, on scenes whose
+    triangle centroids are distinct (the Cornell box, a 128^2 and a 512^2
+    terrain with spheres: ranges above and below the GPU's 512-triangle
+    one-wave threshold)."""
+    scene = {"cornell": cornell_scene, "terrain": small_terrain,
+             "terrain_big": lambda: N.Scene.terrain(3, 512, 128, 16)}[which]()
+    say(f"[bvh] GPU vs host SAH on {which} ({scene.triangle_count} tris)")
+    ctx = N.Context(scene, 8, 8, builder=N.XR_BUILDER_SAH)
+    gpu = ctx.read_accel()
+    g_info = ctx.accel_info()
+    ctx.build_accel(N.XR_BUILDER_SAH_HOST)
+    host = ctx.read_accel()
+    h_info = ctx.accel_info()
+    assert g_info.max_depth == h_info.max_depth and len(gpu[0]) == len(host[0])
+    assert _leaf_sets(*gpu) == _leaf_sets(*host)
+    say(f"[bvh] build ms GPU {g_info.build_ms:.1f} host {h_info.build_ms:.1f}")

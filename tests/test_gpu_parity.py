@@ -30,7 +30,7 @@ def _max_lsb_diff(a: np.ndarray, b: np.ndarray) -> int:
 # ---------------------------------------------------------------------------
 # TraceRay
 # ---------------------------------------------------------------------------
-BUILDERS = [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH]
+BUILDERS = [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH, N.XR_BUILDER_SAH_HOST]
 
 
 @pytest.mark.parametrize("builder", BUILDERS)

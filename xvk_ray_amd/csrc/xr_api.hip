@@ -589,7 +589,8 @@ int xr_set_textures(xr_ctx* c, const xr_texture* t, uint32_t count) {
 
 int xr_build_accel(xr_ctx* c, int builder) {
     if (!c) return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    if (builder != XR_BUILDER_LBVH && builder != XR_BUILDER_PLOC && builder != XR_BUILDER_SAH)
+    if (builder != XR_BUILDER_LBVH && builder != XR_BUILDER_PLOC && builder != XR_BUILDER_SAH &&
+        builder != XR_BUILDER_SAH_HOST)
         return set_err(c, XR_ERR_INVALID_ARGUMENT, "unknown builder %d", builder);
     dev_free(&c->d_nodes);
     dev_free(&c->d_tris);
@@ -597,7 +598,10 @@ int xr_build_accel(xr_ctx* c, int builder) {
     ++c->scene_gen;
     BvhBuildResult r;
     char msg[256] = {0};
-    hipError_t e = (builder == XR_BUILDER_PLOC ? build_ploc : builder == XR_BUILDER_SAH ? build_sah : build_lbvh)(
+    hipError_t e = (builder == XR_BUILDER_PLOC       ? build_ploc
+                    : builder == XR_BUILDER_SAH      ? build_sah_gpu
+                    : builder == XR_BUILDER_SAH_HOST ? build_sah
+                                                     : build_lbvh)(
         c->d_vertices, c->d_indices, c->d_offsets, c->d_inst_first, c->instance_count, c->tri_count, c->stream, &r,
         msg, sizeof(msg));
     if (e != hipSuccess) return set_err(c, XR_ERR_DEVICE, "BVH build failed: %s", msg);

@@ -14,7 +14,7 @@
 //   2. morton   63-bit Morton code of each centroid
 //   3. sort     (code, id) pairs, rocPRIM/hipCUB radix sort
 //   4. karras   internal-node topology from the sorted codes
-//   5. refit    bottom-up boxes (padded, see box_pad), one atomic counter per internal node
+//   5. refit    bottom-up boxes (padded, see box_pad) and SAH costs, one launch per tree level
 //   6. emit     64-byte two-box nodes and the BVH-ordered triangle array
 // PLOC shares 1-3 and replaces 4-5 by bottom-up clustering of the sorted
 // triangles (each cluster merges with its nearest neighbour by merged-box
@@ -51,15 +51,21 @@ struct Box {
 };
 
 // bounds[0..2] / [3..5]: centroid min / max; bounds[6..8] / [9..11]: vertex min / max
-// (order-preserving int encodings of the floats).
-__global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restrict__ idx,
-                         const uint2* __restrict__ offsets, const uint32_t* __restrict__ first_tri,
-                         uint32_t ninst, uint32_t ntri, BvhTri* __restrict__ ctris, float4* __restrict__ cent,
-                         int* __restrict__ bounds) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    float c[3] = {0.f, 0.f, 0.f}, vlo[3] = {0.f, 0.f, 0.f}, vhi[3] = {0.f, 0.f, 0.f};
-    bool live = g < ntri;
-    if (live) {
+// (order-preserving int encodings of the floats).  Grid-stride: each
+// workgroup reduces its triangles' bounds in registers and shuffles and adds
+// one atomic per value (at most kGatherBlocks workgroups contend per address).
+constexpr uint32_t kGatherBlocks = 2048;
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ vtx, const uint32_t* __restrict__ idx,
+                                                const uint2* __restrict__ offsets,
+                                                const uint32_t* __restrict__ first_tri, uint32_t ninst,
+                                                uint32_t ntri, BvhTri* __restrict__ ctris,
+                                                float4* __restrict__ cent, int* __restrict__ bounds) {
+    int v[12];
+    for (int a = 0; a < 3; ++a) {
+        v[a] = v[6 + a] = 0x7FFFFFFF;
+        v[3 + a] = v[9 + a] = (int)0x80000000;
+    }
+    for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < ntri; g += gridDim.x * 256u) {
         // instance = last i with first_tri[i] <= g
         uint32_t lo = 0, hi = ninst;
         while (hi - lo > 1) {
@@ -80,39 +86,42 @@ __global__ void k_gather(const float* __restrict__ vtx, const uint32_t* __restri
         t.v1 = make_float4(p[1][0], p[1][1], p[1][2], __uint_as_float(inst));
         t.v2 = make_float4(p[2][0], p[2][1], p[2][2], __uint_as_float(prim));
         ctris[g] = t;
+        float c[3];
         for (int a = 0; a < 3; ++a) {
             c[a] = (p[0][a] + p[1][a] + p[2][a]) * (1.0f / 3.0f);
-            vlo[a] = fminf(fminf(p[0][a], p[1][a]), p[2][a]);
-            vhi[a] = fmaxf(fmaxf(p[0][a], p[1][a]), p[2][a]);
+            v[a] = min(v[a], ord_enc(c[a]));
+            v[3 + a] = max(v[3 + a], ord_enc(c[a]));
+            v[6 + a] = min(v[6 + a], ord_enc(fminf(fminf(p[0][a], p[1][a]), p[2][a])));
+            v[9 + a] = max(v[9 + a], ord_enc(fmaxf(fmaxf(p[0][a], p[1][a]), p[2][a])));
         }
         cent[g] = make_float4(c[0], c[1], c[2], 0.f);
     }
-    // block reduction of the bounds, then one atomic per block and value
-    __shared__ int smin[6][256], smax[6][256];
-    for (int a = 0; a < 3; ++a) {
-        smin[a][threadIdx.x] = live ? ord_enc(c[a]) : 0x7FFFFFFF;
-        smax[a][threadIdx.x] = live ? ord_enc(c[a]) : (int)0x80000000;
-        smin[3 + a][threadIdx.x] = live ? ord_enc(vlo[a]) : 0x7FFFFFFF;
-        smax[3 + a][threadIdx.x] = live ? ord_enc(vhi[a]) : (int)0x80000000;
-    }
-    __syncthreads();
-    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            for (int a = 0; a < 6; ++a) {
-                smin[a][threadIdx.x] = min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
-                smax[a][threadIdx.x] = max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
+    for (int off = 32; off > 0; off >>= 1)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&bounds[a], smin[a][0]);
-            atomicMax(&bounds[3 + a], smax[a][0]);
-            atomicMin(&bounds[6 + a], smin[3 + a][0]);
-            atomicMax(&bounds[9 + a], smax[3 + a][0]);
+            v[a] = min(v[a], __shfl_xor(v[a], off));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], off));
+            v[6 + a] = min(v[6 + a], __shfl_xor(v[6 + a], off));
+            v[9 + a] = max(v[9 + a], __shfl_xor(v[9 + a], off));
         }
+    __shared__ int part[4][12];
+    if ((threadIdx.x & 63u) == 0u)
+        for (int k = 0; k < 12; ++k) part[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12u) {
+        const int k = (int)threadIdx.x;
+        const bool is_min = k < 3 || (k >= 6 && k < 9);
+        int r = part[0][k];
+        for (int w = 1; w < 4; ++w) r = is_min ? min(r, part[w][k]) : max(r, part[w][k]);
+        if (is_min) atomicMin(&bounds[k], r);
+        else atomicMax(&bounds[k], r);
     }
+}
+
+void launch_gather(const float* vtx, const uint32_t* idx, const uint2* offsets, const uint32_t* first_tri,
+                   uint32_t ninst, uint32_t ntri, BvhTri* ctris, float4* cent, int* bounds, hipStream_t st) {
+    const uint32_t blocks = std::max(1u, std::min((ntri + 255u) / 256u, kGatherBlocks));
+    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, st, vtx, idx, offsets, first_tri, ninst, ntri, ctris,
+                       cent, bounds);
 }
 
 // Box padding.  The watertight FP32 test is consistent across shared edges
@@ -217,19 +226,14 @@ __device__ __forceinline__ float box_area(const Box& b) {
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
-__device__ __forceinline__ float ld_agent(const float* p) {
-    return __uint_as_float(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Bottom-up boxes and SAH costs.  Node ids: internal 0..n-2, leaf j -> n-1+j.
-// The second lane to reach an internal node computes its box and cost and
-// walks on.  cost(leaf of k tris) = kSahTri*k*A; cost(split) = kSahNode*A +
-// cost(left) + cost(right); a non-root node whose subtree holds at most
-// kMaxLeafTris triangles becomes one leaf when that is cheaper (collapse[p]).
-__global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
-                        const int2* __restrict__ child, const int* __restrict__ parent, Box* boxes,
-                        int* __restrict__ flags, uint32_t* __restrict__ max_depth, const int* __restrict__ bounds,
-                        const int2* __restrict__ range, float* cost, int* __restrict__ collapse) {
+// Bottom-up boxes and SAH costs, one depth level of the tree per launch
+// (deepest first), so every child is complete before its parent reads it and
+// no lane waits on another.  Node ids: internal 0..n-2, leaf j -> n-1+j.
+// cost(leaf of k tris) = kSahTri*k*A; cost(split) = kSahNode*A + cost(left) +
+// cost(right); a non-root node whose subtree holds at most kMaxLeafTris
+// triangles becomes one leaf when that is cheaper (collapse[p]).
+__global__ void k_leaf_boxes(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ sorted_ids, int n,
+                             const int* __restrict__ bounds, Box* __restrict__ boxes, float* __restrict__ cost) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const BvhTri t = ctris[sorted_ids[j]];
@@ -241,41 +245,68 @@ __global__ void k_refit(const BvhTri* __restrict__ ctris, const uint32_t* __rest
     b.hi[0] = fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x) + pad;
     b.hi[1] = fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y) + pad;
     b.hi[2] = fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z) + pad;
-    int node = (n - 1) + j;
+    const int node = (n - 1) + j;
     boxes[node] = b;
     cost[node] = kSahTri * box_area(b);
-    // Depth of this leaf.  A valid tree reaches the root in < n steps; the
-    // bound keeps a malformed one from looping (it then reports depth n).
-    uint32_t depth = 0;
-    {
-        int p = node;
-        while (p != 0 && n > 1 && depth < (uint32_t)n) { p = parent[p]; ++depth; }
+}
+
+// Depth (edges from the root) of every internal node, the sort key of the
+// level order; a walk longer than kMaxLbvhDepth marks the build as failed.
+constexpr int kMaxLbvhDepth = 255;
+__global__ void k_node_depth(const int* __restrict__ parent, int ninner, uint32_t* __restrict__ depth,
+                             uint32_t* __restrict__ ids, uint32_t* __restrict__ too_deep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninner) return;
+    int p = i, d = 0;
+    while (p != 0 && d <= kMaxLbvhDepth) {
+        p = parent[p];
+        ++d;
     }
-    atomicMax(max_depth, depth);
-    if (n == 1) return;
-    for (int steps = 0; node != 0 && steps < n; ++steps) {
-        const int p = parent[node];
-        __threadfence();
-        const int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;
-        __threadfence();
-        const int2 c = child[p];
-        Box m;
-        for (int a = 0; a < 3; ++a) {
-            m.lo[a] = fminf(ld_agent(&boxes[c.x].lo[a]), ld_agent(&boxes[c.y].lo[a]));
-            m.hi[a] = fmaxf(ld_agent(&boxes[c.x].hi[a]), ld_agent(&boxes[c.y].hi[a]));
-        }
-        boxes[p] = m;
-        const float area = box_area(m);
-        const float split = kSahNode * area + ld_agent(&cost[c.x]) + ld_agent(&cost[c.y]);
-        const int2 r = range[p];
-        const int cnt = r.y - r.x + 1;
-        const float as_leaf = kSahTri * (float)cnt * area;
-        const bool leaf = p != 0 && cnt <= kMaxLeafTris && as_leaf <= split;
-        collapse[p] = leaf ? 1 : 0;
-        cost[p] = leaf ? as_leaf : split;
-        node = p;
+    if (d > kMaxLbvhDepth) *too_deep = 1u;
+    depth[i] = (uint32_t)min(d, kMaxLbvhDepth);
+    ids[i] = (uint32_t)i;
+}
+
+// level_start[d] = first position of depth d in the sorted keys (levels are
+// never empty between 0 and the deepest).
+__global__ void k_level_starts(const uint32_t* __restrict__ sorted_depth, int ninner,
+                               uint32_t* __restrict__ level_start) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninner) return;
+    const uint32_t d = sorted_depth[i];
+    if (i == 0 || sorted_depth[i - 1] != d) level_start[d] = (uint32_t)i;
+    if (i == ninner - 1) level_start[kMaxLbvhDepth + 1] = d;  // the deepest level
+}
+
+// One level: every node's children are leaves or nodes one level deeper.
+// height[p] counts edges down to the deepest leaf of the emitted tree (a
+// collapsed node is a leaf); the root's height is the tree's depth.
+__global__ void k_refit_level(const uint32_t* __restrict__ level_ids, int count, int n,
+                              const int2* __restrict__ child, const int2* __restrict__ range, Box* __restrict__ boxes,
+                              float* __restrict__ cost, int* __restrict__ collapse, int* __restrict__ height) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int p = (int)level_ids[k];
+    const int leaf_base = n - 1;
+    const int2 c = child[p];
+    const Box bx = boxes[c.x], by = boxes[c.y];
+    Box m;
+    for (int a = 0; a < 3; ++a) {
+        m.lo[a] = fminf(bx.lo[a], by.lo[a]);
+        m.hi[a] = fmaxf(bx.hi[a], by.hi[a]);
     }
+    boxes[p] = m;
+    const float area = box_area(m);
+    const float split = kSahNode * area + cost[c.x] + cost[c.y];
+    const int2 r = range[p];
+    const int cnt = r.y - r.x + 1;
+    const float as_leaf = kSahTri * (float)cnt * area;
+    const bool leaf = p != 0 && cnt <= kMaxLeafTris && as_leaf <= split;
+    collapse[p] = leaf ? 1 : 0;
+    cost[p] = leaf ? as_leaf : split;
+    const int hx = c.x >= leaf_base ? 0 : height[c.x];
+    const int hy = c.y >= leaf_base ? 0 : height[c.y];
+    height[p] = leaf ? 0 : 1 + max(hx, hy);
 }
 
 // Internal nodes below a collapsed node are emitted too but never linked.
@@ -537,13 +568,14 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     int2* child = nullptr;
     int* parent = nullptr;
     Box* boxes = nullptr;
-    int* flags = nullptr;
     int2* range = nullptr;
     float* cost = nullptr;
     int* collapse = nullptr;
-    uint32_t* dmax = nullptr;
+    int* height = nullptr;
+    uint32_t* lvl = nullptr;  // level starts [0, kMaxLbvhDepth], deepest level, too-deep flag
+    std::vector<uint32_t> hl(kMaxLbvhDepth + 3, 0u);
     void* temp = nullptr;
-    size_t temp_bytes = 0;
+    size_t temp_bytes = 0, level_sort_bytes = 0;
     BvhNode* nodes = nullptr;
     BvhTri* tris = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -570,39 +602,68 @@ hipError_t build_lbvh(const float* vertices, const uint32_t* indices, const uint
     XR_TRY(hipMalloc(&child, sizeof(int2) * ninner));
     XR_TRY(hipMalloc(&parent, sizeof(int) * (ninner + tri_count)));
     XR_TRY(hipMalloc(&boxes, sizeof(Box) * (ninner + tri_count)));
-    XR_TRY(hipMalloc(&flags, sizeof(int) * ninner));
     XR_TRY(hipMalloc(&range, sizeof(int2) * ninner));
     XR_TRY(hipMalloc(&cost, sizeof(float) * (ninner + tri_count)));
     XR_TRY(hipMalloc(&collapse, sizeof(int) * ninner));
+    XR_TRY(hipMalloc(&height, sizeof(int) * ninner));
     XR_TRY(hipMemsetAsync(collapse, 0, sizeof(int) * ninner, st));
-    XR_TRY(hipMalloc(&dmax, sizeof(uint32_t)));
+    XR_TRY(hipMalloc(&lvl, sizeof(uint32_t) * (kMaxLbvhDepth + 3)));
     XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * ninner));
     XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * tri_count));
     XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
-    XR_TRY(hipMemsetAsync(flags, 0, sizeof(int) * ninner, st));
-    XR_TRY(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
+    XR_TRY(hipMemsetAsync(lvl, 0, sizeof(uint32_t) * (kMaxLbvhDepth + 3), st));
     XR_TRY(hipMemsetAsync(parent, 0, sizeof(int) * (ninner + tri_count), st));
 
-    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
-                       instance_count, tri_count, ctris, cent, bounds);
+    launch_gather(vertices, indices, offsets, inst_first_tri, instance_count, tri_count, ctris, cent, bounds, st);
     XR_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, st, cent, tri_count, bounds, keys, vals);
     XR_TRY(hipGetLastError());
     XR_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
+    XR_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, level_sort_bytes, (uint32_t*)keys, (uint32_t*)keys2, vals,
+                                              vals, (int)ninner, 0, 8, st));
+    temp_bytes = std::max(temp_bytes, level_sort_bytes);
     XR_TRY(hipMalloc(&temp, temp_bytes));
     XR_TRY(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys2, vals, vals2, n, 0, 63, st));
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, keys2, n, child, parent, range);
         XR_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, child, parent, boxes, flags, dmax,
-                       bounds, range, cost, collapse);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(blocks), dim3(256), 0, st, ctris, vals2, n, bounds, boxes, cost);
     XR_TRY(hipGetLastError());
+    if (n > 1) {
+        // the Morton keys are no longer needed: depth keys and node ids in
+        // their place, sorted into keys2 (keys first, ids after them)
+        uint32_t* dkey = (uint32_t*)keys;
+        uint32_t* dkey2 = (uint32_t*)keys2;
+        uint32_t* nid = vals;
+        uint32_t* nid2 = (uint32_t*)keys2 + tri_count;
+        const uint32_t ib = (ninner + 255) / 256;
+        hipLaunchKernelGGL(k_node_depth, dim3(ib), dim3(256), 0, st, parent, (int)ninner, dkey, nid,
+                           lvl + kMaxLbvhDepth + 2);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipcub::DeviceRadixSort::SortPairs(temp, level_sort_bytes, dkey, dkey2, nid, nid2, (int)ninner, 0, 8,
+                                                  st));
+        hipLaunchKernelGGL(k_level_starts, dim3(ib), dim3(256), 0, st, dkey2, (int)ninner, lvl);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipMemcpyAsync(hl.data(), lvl, sizeof(uint32_t) * hl.size(), hipMemcpyDeviceToHost, st));
+        XR_TRY(hipStreamSynchronize(st));
+        if (hl[kMaxLbvhDepth + 2]) {
+            snprintf(err, err_len, "LBVH deeper than %d levels", kMaxLbvhDepth);
+            goto fail;
+        }
+        const int deepest = (int)hl[kMaxLbvhDepth + 1];
+        for (int d = deepest; d >= 0; --d) {
+            const uint32_t lb = hl[d], le = d == deepest ? ninner : hl[d + 1];
+            hipLaunchKernelGGL(k_refit_level, dim3((le - lb + 255) / 256), dim3(256), 0, st, nid2 + lb,
+                               (int)(le - lb), n, child, range, boxes, cost, collapse, height);
+        }
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipMemcpyAsync(&host_depth, height, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    }
     hipLaunchKernelGGL(k_emit, dim3((tri_count + 255) / 256), dim3(256), 0, st, n, child, boxes, nodes, ctris,
                        vals2, tris, range, collapse);
     XR_TRY(hipGetLastError());
     XR_TRY(hipEventRecord(e1, st));
-    XR_TRY(hipMemcpyAsync(&host_depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     XR_TRY(hipStreamSynchronize(st));
     XR_TRY(hipEventElapsedTime(&ms, e0, e1));
 
@@ -629,11 +690,11 @@ done:
     hipFree(child);
     hipFree(parent);
     hipFree(boxes);
-    hipFree(flags);
     hipFree(range);
     hipFree(cost);
     hipFree(collapse);
-    hipFree(dmax);
+    hipFree(height);
+    hipFree(lvl);
     hipFree(temp);
     hipFree(nodes);
     hipFree(tris);
@@ -713,8 +774,7 @@ hipError_t build_ploc(const float* vertices, const uint32_t* indices, const uint
     XR_TRY(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
     XR_TRY(hipMemsetAsync(parent, 0, sizeof(int) * nall, st));
 
-    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
-                       instance_count, tri_count, ctris, cent, bounds);
+    launch_gather(vertices, indices, offsets, inst_first_tri, instance_count, tri_count, ctris, cent, bounds, st);
     XR_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, st, cent, tri_count, bounds, keys, vals);
     XR_TRY(hipGetLastError());
@@ -1112,8 +1172,7 @@ hipError_t build_sah(const float* vertices, const uint32_t* indices, const uint2
     XR_TRY(hipMalloc(&cent, sizeof(float4) * n));
     XR_TRY(hipMalloc(&bounds, sizeof(int) * 12));
     XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, vertices, indices, offsets, inst_first_tri,
-                       instance_count, n, ctris, cent, bounds);
+    launch_gather(vertices, indices, offsets, inst_first_tri, instance_count, n, ctris, cent, bounds, st);
     XR_TRY(hipGetLastError());
     XR_TRY(hipMemcpyAsync(h.data(), ctris, sizeof(BvhTri) * n, hipMemcpyDeviceToHost, st));
     XR_TRY(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
@@ -1167,6 +1226,903 @@ done:
     hipFree(ctris);
     hipFree(cent);
     hipFree(bounds);
+    hipFree(nodes);
+    hipFree(tris);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Binned SAH on the GPU.  Makes the host builder's decisions (64 centroid bins
+// per axis, the first lowest-cost bin in axis then bin order, the same float
+// arithmetic for boxes, areas and costs, leaves of at most kMaxLeafTris
+// triangles where they are cheaper than their best split), so it builds the
+// same tree as build_sah: every split is a function of the set of triangles
+// in the node, which does not depend on the order either builder keeps them
+// in.  (Ranges whose centroids all coincide are halved in index order; the
+// host takes a median by its own order there, so such ranges may be cut
+// differently.)
+//
+// Levels of large ranges (more than kGsSmall triangles) are split with every
+// workgroup on the device, one kernel per step: centroid bounds, binning
+// (LDS bins per 2048-triangle chunk, merged with one global atomic per bin
+// and value), the SAH sweep (one wave per range, a lane per bin), a stable
+// partition (per-chunk counts, a scan per range, a scatter).  The host reads
+// back the next level's ranges (one synchronisation per level) to lay out its
+// chunks.  Ranges of at most kGsSmall triangles are then built one per wave,
+// depth first with a job stack, bins and triangle ids in LDS.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kGsChunk = 2048;
+constexpr uint32_t kGsSmall = 512;
+constexpr int kGsAxisBins = 3 * kSahBins;
+
+struct GsSeg {
+    uint32_t b, e;    // triangle range [b, e) of the id array
+    int parent;       // node index, -1 for the root
+    uint32_t sd;      // side | depth << 1 (depth: edges from the root)
+    float lo[3], hi[3];  // the range's box, as its parent stored it
+};
+
+// Per large range of a level: centroid bounds and the 192 bins
+// (order-preserving int encodings of the box floats), then the split.
+struct GsBins {
+    int cb[6];
+    int box[kGsAxisBins][6];
+    uint32_t cnt[kGsAxisBins];
+};
+struct GsSplit {
+    int axis;          // -1: halve in index order
+    int bin;
+    float lo, scale;   // bin of centroid c: min(63, (int)((c - lo) * scale))
+    uint32_t m;        // first triangle of the right range
+    uint32_t node;     // the range's node
+};
+struct GsChunk {
+    uint32_t seg, b, e;
+    uint32_t first_chunk;  // first chunk of the same range
+};
+
+__device__ __forceinline__ float gs_area(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+__device__ __forceinline__ int gs_bin(float c, float lo, float scale) {
+    const int k = (int)((c - lo) * scale);
+    return k < kSahBins - 1 ? k : kSahBins - 1;
+}
+
+// Padded triangle boxes (the host's SahPrim.box).
+__global__ void k_gs_prims(const BvhTri* __restrict__ ctris, uint32_t n, float pad, float4* __restrict__ plo,
+                           float4* __restrict__ phi, uint32_t* __restrict__ ids) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const BvhTri t = ctris[i];
+    plo[i] = make_float4(fminf(fminf(t.v0.x, t.v1.x), t.v2.x) - pad, fminf(fminf(t.v0.y, t.v1.y), t.v2.y) - pad,
+                         fminf(fminf(t.v0.z, t.v1.z), t.v2.z) - pad, 0.0f);
+    phi[i] = make_float4(fmaxf(fmaxf(t.v0.x, t.v1.x), t.v2.x) + pad, fmaxf(fmaxf(t.v0.y, t.v1.y), t.v2.y) + pad,
+                         fmaxf(fmaxf(t.v0.z, t.v1.z), t.v2.z) + pad, 0.0f);
+    ids[i] = i;
+}
+
+__global__ void k_gs_init_bins(GsBins* __restrict__ bins, uint32_t nseg) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nseg) return;
+    GsBins& B = bins[s];
+    for (uint32_t i = threadIdx.x; i < 6; i += blockDim.x) B.cb[i] = i < 3 ? 0x7FFFFFFF : (int)0x80000000;
+    for (uint32_t i = threadIdx.x; i < kGsAxisBins; i += blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            B.box[i][k] = 0x7FFFFFFF;
+            B.box[i][3 + k] = (int)0x80000000;
+        }
+        B.cnt[i] = 0u;
+    }
+}
+
+// Centroid bounds of each chunk's range.
+__global__ __launch_bounds__(256) void k_gs_cbounds(const GsChunk* __restrict__ chunks, const uint32_t* __restrict__ ids,
+                                                    const float4* __restrict__ cent, GsBins* __restrict__ bins) {
+    const GsChunk ch = chunks[blockIdx.x];
+    int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+    for (uint32_t i = ch.b + threadIdx.x; i < ch.e; i += 256u) {
+        const float4 c = cent[ids[i]];
+        const int o[3] = {ord_enc(c.x), ord_enc(c.y), ord_enc(c.z)};
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], o[a]);
+            v[3 + a] = max(v[3 + a], o[a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int off = 32; off > 0; off >>= 1) {
+            v[a] = min(v[a], __shfl_xor(v[a], off));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], off));
+        }
+    if ((threadIdx.x & 63u) == 0u) {
+        GsBins& B = bins[ch.seg];
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&B.cb[a], v[a]);
+            atomicMax(&B.cb[3 + a], v[3 + a]);
+        }
+    }
+}
+
+__device__ __forceinline__ void gs_axis_frame(const int cb[6], int a, float* lo, float* scale, bool* valid) {
+    const float l = ord_dec(cb[a]), h = ord_dec(cb[3 + a]);
+    const float ext = h - l;
+    *valid = ext > 0.0f;
+    *lo = l;
+    *scale = *valid ? (float)kSahBins / ext : 0.0f;
+}
+
+// Bins of each chunk in LDS, merged into its range's bins.
+__global__ __launch_bounds__(256) void k_gs_bin(const GsChunk* __restrict__ chunks, const uint32_t* __restrict__ ids,
+                                                const float4* __restrict__ cent, const float4* __restrict__ plo,
+                                                const float4* __restrict__ phi, GsBins* __restrict__ bins) {
+    __shared__ int sbox[kGsAxisBins][6];
+    __shared__ uint32_t scnt[kGsAxisBins];
+    const GsChunk ch = chunks[blockIdx.x];
+    GsBins& B = bins[ch.seg];
+    for (uint32_t i = threadIdx.x; i < kGsAxisBins; i += 256u) {
+        for (int k = 0; k < 3; ++k) {
+            sbox[i][k] = 0x7FFFFFFF;
+            sbox[i][3 + k] = (int)0x80000000;
+        }
+        scnt[i] = 0u;
+    }
+    float lo[3], scale[3];
+    bool valid[3];
+    for (int a = 0; a < 3; ++a) gs_axis_frame(B.cb, a, &lo[a], &scale[a], &valid[a]);
+    __syncthreads();
+    for (uint32_t i = ch.b + threadIdx.x; i < ch.e; i += 256u) {
+        const uint32_t id = ids[i];
+        const float4 c = cent[id], l = plo[id], h = phi[id];
+        const float cc[3] = {c.x, c.y, c.z};
+        const int ol[3] = {ord_enc(l.x), ord_enc(l.y), ord_enc(l.z)};
+        const int oh[3] = {ord_enc(h.x), ord_enc(h.y), ord_enc(h.z)};
+        for (int a = 0; a < 3; ++a) {
+            if (!valid[a]) continue;
+            const int k = a * kSahBins + gs_bin(cc[a], lo[a], scale[a]);
+            for (int q = 0; q < 3; ++q) {
+                atomicMin(&sbox[k][q], ol[q]);
+                atomicMax(&sbox[k][3 + q], oh[q]);
+            }
+            atomicAdd(&scnt[k], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kGsAxisBins; i += 256u) {
+        if (scnt[i] == 0u) continue;
+        for (int q = 0; q < 3; ++q) {
+            atomicMin(&B.box[i][q], sbox[i][q]);
+            atomicMax(&B.box[i][3 + q], sbox[i][3 + q]);
+        }
+        atomicAdd(&B.cnt[i], scnt[i]);
+    }
+}
+
+// The SAH sweep of one axis with a lane per bin: the left side of split k is
+// bins 0..k, the right side bins k+1..63 (unions are exact, so these are the
+// host's accumulated boxes).  Returns per lane its split's cost (infinite
+// where a side is empty or k = 63) and the two sides' boxes and counts.
+struct GsSide {
+    float lo[3], hi[3];
+    uint32_t n;
+};
+__device__ __forceinline__ void gs_sweep(const int (*box)[6], const uint32_t* cnt, int lane, float* cost, GsSide* L,
+                                         GsSide* R) {
+    int bl[6], br[6];
+    for (int q = 0; q < 6; ++q) bl[q] = br[q] = box[lane][q];
+    uint32_t nl = cnt[lane], nr = cnt[lane];
+    // inclusive prefix (left) and suffix (right) over the lanes
+    for (int off = 1; off < 64; off <<= 1) {
+        int ul[6], ur[6];
+        for (int q = 0; q < 6; ++q) {
+            ul[q] = __shfl_up(bl[q], off);
+            ur[q] = __shfl_down(br[q], off);
+        }
+        const uint32_t cl = (uint32_t)__shfl_up((int)nl, off), cr = (uint32_t)__shfl_down((int)nr, off);
+        if (lane >= off) {
+            for (int q = 0; q < 3; ++q) {
+                bl[q] = min(bl[q], ul[q]);
+                bl[3 + q] = max(bl[3 + q], ul[3 + q]);
+            }
+            nl += cl;
+        }
+        if (lane + off < 64) {
+            for (int q = 0; q < 3; ++q) {
+                br[q] = min(br[q], ur[q]);
+                br[3 + q] = max(br[3 + q], ur[3 + q]);
+            }
+            nr += cr;
+        }
+    }
+    // right side of split k: the suffix from bin k + 1
+    int rb[6];
+    for (int q = 0; q < 6; ++q) rb[q] = __shfl_down(br[q], 1);
+    // every lane takes part in the shuffle (lane 63 is the source of lane 62)
+    const uint32_t rn_next = (uint32_t)__shfl_down((int)nr, 1);
+    const uint32_t rn = lane + 1 < 64 ? rn_next : 0u;
+    for (int q = 0; q < 3; ++q) {
+        L->lo[q] = ord_dec(bl[q]);
+        L->hi[q] = ord_dec(bl[3 + q]);
+        R->lo[q] = ord_dec(rb[q]);
+        R->hi[q] = ord_dec(rb[3 + q]);
+    }
+    L->n = nl;
+    R->n = lane + 1 < 64 ? rn : 0u;
+    if (lane == 63 || L->n == 0u || R->n == 0u) {
+        *cost = INFINITY;
+    } else {
+        *cost = (float)L->n * gs_area(L->lo, L->hi) + (float)R->n * gs_area(R->lo, R->hi);
+    }
+}
+
+// The first lowest (cost, axis, bin) of the wave: returns the winning lane's
+// packed key (axis * 64 + bin), or -1 when no split is finite.
+__device__ __forceinline__ int gs_argmin(float cost, int key) {
+    float c = cost < INFINITY ? cost : INFINITY;  // NaN never wins
+    int k = cost < INFINITY ? key : 0x7FFFFFFF;
+    for (int off = 32; off > 0; off >>= 1) {
+        const float oc = __shfl_xor(c, off);
+        const int ok = __shfl_xor(k, off);
+        if (oc < c || (oc == c && ok < k)) {
+            c = oc;
+            k = ok;
+        }
+    }
+    return k == 0x7FFFFFFF ? -1 : k;
+}
+
+__device__ __forceinline__ void gs_set_child(BvhNode* nd, int side, const float lo[3], const float hi[3]) {
+    if (side == 0) {
+        nd->a = make_float4(lo[0], hi[0], lo[1], hi[1]);
+        nd->c.x = lo[2];
+        nd->c.y = hi[2];
+    } else {
+        nd->b = make_float4(lo[0], hi[0], lo[1], hi[1]);
+        nd->c.z = lo[2];
+        nd->c.w = hi[2];
+    }
+}
+__device__ __forceinline__ void gs_set_link(BvhNode* nodes, int parent, int side, int v) {
+    if (side == 0) nodes[parent].d.x = v;
+    else nodes[parent].d.y = v;
+}
+
+// One wave per large range: the split, its node, its two children (appended
+// to the next level's large ranges or to the small ones).  Ranges whose
+// centroids coincide are halved in index order (their child boxes come from
+// k_gs_median_boxes).
+__global__ __launch_bounds__(64) void k_gs_split(const GsSeg* __restrict__ segs, uint32_t nseg,
+                                                 const GsBins* __restrict__ bins, GsSplit* __restrict__ split,
+                                                 BvhNode* __restrict__ nodes, uint32_t* __restrict__ counters,
+                                                 GsSeg* __restrict__ next, GsSeg* __restrict__ small) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nseg) return;
+    const int lane = (int)threadIdx.x;
+    const GsSeg g = segs[s];
+    const GsBins& B = bins[s];
+    float best_cost = INFINITY;
+    int best_key = -1;
+    GsSide L, R;
+    GsSide bestL, bestR;
+    for (int a = 0; a < 3; ++a) {
+        float lo, scale;
+        bool valid;
+        gs_axis_frame(B.cb, a, &lo, &scale, &valid);
+        if (!valid) continue;
+        float c;
+        gs_sweep(&B.box[a * kSahBins], &B.cnt[a * kSahBins], lane, &c, &L, &R);
+        const int key = gs_argmin(c, a * kSahBins + lane);
+        if (key < 0) continue;
+        const int wl = key % kSahBins;
+        const float wc = __shfl(c, wl);
+        if (wc < best_cost) {  // axes in order, strictly lower: the host's rule
+            best_cost = wc;
+            best_key = key;
+            for (int q = 0; q < 3; ++q) {
+                bestL.lo[q] = __shfl(L.lo[q], wl);
+                bestL.hi[q] = __shfl(L.hi[q], wl);
+                bestR.lo[q] = __shfl(R.lo[q], wl);
+                bestR.hi[q] = __shfl(R.hi[q], wl);
+            }
+            bestL.n = (uint32_t)__shfl((int)L.n, wl);
+            bestR.n = (uint32_t)__shfl((int)R.n, wl);
+        }
+    }
+    if (lane != 0) return;
+    GsSplit sp;
+    const uint32_t cnt = g.e - g.b;
+    const uint32_t k = atomicAdd(&counters[0], 1u);
+    sp.node = k;
+    if (best_key >= 0) {
+        sp.axis = best_key / kSahBins;
+        sp.bin = best_key % kSahBins;
+        bool valid;
+        gs_axis_frame(B.cb, sp.axis, &sp.lo, &sp.scale, &valid);
+        sp.m = g.b + bestL.n;
+    } else {
+        sp.axis = -1;
+        sp.bin = 0;
+        sp.lo = 0.0f;
+        sp.scale = 0.0f;
+        sp.m = g.b + cnt / 2u;
+    }
+    split[s] = sp;
+    BvhNode nd;
+    nd.a = nd.b = nd.c = make_float4(0.f, 0.f, 0.f, 0.f);
+    nd.d = make_int4(0, 0, 0, 0);
+    if (best_key >= 0) {
+        gs_set_child(&nd, 0, bestL.lo, bestL.hi);
+        gs_set_child(&nd, 1, bestR.lo, bestR.hi);
+    }
+    nodes[k] = nd;
+    if (g.parent >= 0) gs_set_link(nodes, g.parent, (int)(g.sd & 1u), (int)k);
+    const uint32_t depth = (g.sd >> 1) + 1u;
+    for (int side = 0; side < 2; ++side) {
+        GsSeg c;
+        c.b = side == 0 ? g.b : sp.m;
+        c.e = side == 0 ? sp.m : g.e;
+        c.parent = (int)k;
+        c.sd = (uint32_t)side | (depth << 1);
+        const GsSide& S = side == 0 ? bestL : bestR;
+        for (int q = 0; q < 3; ++q) {
+            c.lo[q] = best_key >= 0 ? S.lo[q] : 0.0f;
+            c.hi[q] = best_key >= 0 ? S.hi[q] : 0.0f;
+        }
+        if (c.e - c.b > kGsSmall) next[atomicAdd(&counters[1], 1u)] = c;
+        else small[atomicAdd(&counters[2], 1u)] = c;
+    }
+}
+
+// Goes-left test of triangle id under range split sp.
+__device__ __forceinline__ bool gs_left(const GsSplit& sp, const float4* __restrict__ cent, uint32_t id, uint32_t pos) {
+    if (sp.axis < 0) return pos < sp.m;
+    const float4 c = cent[id];
+    const float cc = sp.axis == 0 ? c.x : (sp.axis == 1 ? c.y : c.z);
+    return gs_bin(cc, sp.lo, sp.scale) <= sp.bin;
+}
+
+// Per chunk: how many of its triangles go left.
+__global__ __launch_bounds__(256) void k_gs_count(const GsChunk* __restrict__ chunks, const uint32_t* __restrict__ ids,
+                                                  const float4* __restrict__ cent, const GsSplit* __restrict__ split,
+                                                  uint32_t* __restrict__ chunk_left) {
+    __shared__ uint32_t part[4];
+    const GsChunk ch = chunks[blockIdx.x];
+    const GsSplit sp = split[ch.seg];
+    uint32_t n = 0;
+    for (uint32_t i = ch.b + threadIdx.x; i < ch.e; i += 256u) n += gs_left(sp, cent, ids[i], i) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) n += (uint32_t)__shfl_xor((int)n, off);
+    if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_left[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of chunk_left over each range's chunks (one workgroup per
+// range; its chunks are consecutive): chunk_off[c] = left triangles of the
+// range's chunks before c.
+__global__ __launch_bounds__(256) void k_gs_chunk_scan(const GsChunk* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ seg_first_chunk,
+                                                       const uint32_t* __restrict__ chunk_left,
+                                                       uint32_t* __restrict__ chunk_off) {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    const uint32_t c0 = seg_first_chunk[blockIdx.x], c1 = seg_first_chunk[blockIdx.x + 1];
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    for (uint32_t base = c0; base < c1; base += 256u) {
+        const uint32_t c = base + threadIdx.x;
+        const uint32_t v = c < c1 ? chunk_left[c] : 0u;
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int k = 0; k < wv; ++k) before += wsum[k];
+        if (c < c1) chunk_off[c] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
+// Stable partition of every chunk into ids_out; each chunk's left triangles go
+// to b + chunk_off + rank, its right ones to m + (chunk start - b - chunk_off) + rank.
+__global__ __launch_bounds__(256) void k_gs_scatter(const GsChunk* __restrict__ chunks, const GsSeg* __restrict__ segs,
+                                                    const uint32_t* __restrict__ ids, const float4* __restrict__ cent,
+                                                    const GsSplit* __restrict__ split,
+                                                    const uint32_t* __restrict__ chunk_off,
+                                                    uint32_t* __restrict__ ids_out) {
+    __shared__ uint32_t wl[4], wr[4];
+    const GsChunk ch = chunks[blockIdx.x];
+    const GsSeg g = segs[ch.seg];
+    const GsSplit sp = split[ch.seg];
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    uint32_t lbase = g.b + chunk_off[blockIdx.x];
+    uint32_t rbase = sp.m + (ch.b - g.b - chunk_off[blockIdx.x]);
+    for (uint32_t base = ch.b; base < ch.e; base += 256u) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < ch.e;
+        const uint32_t id = live ? ids[i] : 0u;
+        const bool left = live && gs_left(sp, cent, id, i);
+        const unsigned long long ml = __ballot(left), mr = __ballot(live && !left);
+        if (lane == 0) {
+            wl[wv] = (uint32_t)__popcll(ml);
+            wr[wv] = (uint32_t)__popcll(mr);
+        }
+        __syncthreads();
+        uint32_t pl = lbase, pr = rbase;
+        for (int k = 0; k < wv; ++k) {
+            pl += wl[k];
+            pr += wr[k];
+        }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (left) ids_out[pl + (uint32_t)__popcll(ml & below)] = id;
+        else if (live) ids_out[pr + (uint32_t)__popcll(mr & below)] = id;
+        lbase += wl[0] + wl[1] + wl[2] + wl[3];
+        rbase += wr[0] + wr[1] + wr[2] + wr[3];
+        __syncthreads();
+    }
+}
+
+__global__ void k_gs_copy_back(const GsChunk* __restrict__ chunks, const uint32_t* __restrict__ src,
+                               uint32_t* __restrict__ dst) {
+    const GsChunk ch = chunks[blockIdx.x];
+    for (uint32_t i = ch.b + threadIdx.x; i < ch.e; i += blockDim.x) dst[i] = src[i];
+}
+
+// Child boxes of ranges halved in index order: unions of their triangles' boxes.
+__global__ __launch_bounds__(64) void k_gs_median_boxes(const GsSeg* __restrict__ segs, uint32_t nseg,
+                                                        const GsSplit* __restrict__ split,
+                                                        const uint32_t* __restrict__ ids,
+                                                        const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                        BvhNode* __restrict__ nodes, GsSeg* __restrict__ next,
+                                                        GsSeg* __restrict__ small, const uint32_t* __restrict__ counters) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nseg || split[s].axis >= 0) return;
+    const GsSeg g = segs[s];
+    const GsSplit sp = split[s];
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t b = side == 0 ? g.b : sp.m, e = side == 0 ? sp.m : g.e;
+        int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+        for (uint32_t i = b + threadIdx.x; i < e; i += 64u) {
+            const float4 l = plo[ids[i]], h = phi[ids[i]];
+            v[0] = min(v[0], ord_enc(l.x)); v[1] = min(v[1], ord_enc(l.y)); v[2] = min(v[2], ord_enc(l.z));
+            v[3] = max(v[3], ord_enc(h.x)); v[4] = max(v[4], ord_enc(h.y)); v[5] = max(v[5], ord_enc(h.z));
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            for (int q = 0; q < 3; ++q) {
+                v[q] = min(v[q], __shfl_xor(v[q], off));
+                v[3 + q] = max(v[3 + q], __shfl_xor(v[3 + q], off));
+            }
+        if (threadIdx.x != 0) continue;
+        float lo[3], hi[3];
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = ord_dec(v[q]);
+            hi[q] = ord_dec(v[3 + q]);
+        }
+        gs_set_child(&nodes[sp.node], side, lo, hi);
+        // the child range carries its box to its own node (either list)
+        GsSeg* lists[2] = {next, small};
+        const uint32_t ns[2] = {counters[1], counters[2]};
+        for (int l = 0; l < 2; ++l)
+            for (uint32_t j = 0; j < ns[l]; ++j)
+                if (lists[l][j].parent == (int)sp.node && (lists[l][j].sd & 1u) == (uint32_t)side)
+                    for (int q = 0; q < 3; ++q) {
+                        lists[l][j].lo[q] = lo[q];
+                        lists[l][j].hi[q] = hi[q];
+                    }
+    }
+}
+
+// ---- ranges of at most kGsSmall triangles: one wave each ----
+struct GsJob {
+    uint32_t b, e;  // local range of the wave's id copy
+    int parent;
+    uint32_t sd;
+    float lo[3], hi[3];
+};
+// Depth first, the larger child pushed first and the smaller taken first:
+// at most log2(kGsSmall) + 1 jobs wait at any time.
+constexpr int kGsJobs = 16;
+
+struct GsWave {
+    int box[kGsAxisBins][6];
+    uint32_t cnt[kGsAxisBins];
+    uint32_t ids[kGsSmall];
+    uint32_t tmp[kGsSmall];
+    GsJob jobs[kGsJobs];
+};
+
+__device__ __forceinline__ void gs_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The host's sah_split on W.ids[b, b + cnt): returns the left count m
+// (0 < m < cnt) and the sides' boxes; with `partition` the range is
+// reordered left side first (stably).
+__device__ uint32_t gs_wave_split(GsWave& W, uint32_t b, uint32_t cnt, const float4* __restrict__ cent,
+                                  const float4* __restrict__ plo, const float4* __restrict__ phi, GsSide* Lo,
+                                  GsSide* Ro, bool partition) {
+    const int lane = (int)threadIdx.x & 63;
+    int cb[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+    for (uint32_t i = lane; i < cnt; i += 64u) {
+        const float4 c = cent[W.ids[b + i]];
+        cb[0] = min(cb[0], ord_enc(c.x)); cb[1] = min(cb[1], ord_enc(c.y)); cb[2] = min(cb[2], ord_enc(c.z));
+        cb[3] = max(cb[3], ord_enc(c.x)); cb[4] = max(cb[4], ord_enc(c.y)); cb[5] = max(cb[5], ord_enc(c.z));
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 3; ++q) {
+            cb[q] = min(cb[q], __shfl_xor(cb[q], off));
+            cb[3 + q] = max(cb[3 + q], __shfl_xor(cb[3 + q], off));
+        }
+    for (int i = lane; i < kGsAxisBins; i += 64) {
+        for (int q = 0; q < 3; ++q) {
+            W.box[i][q] = 0x7FFFFFFF;
+            W.box[i][3 + q] = (int)0x80000000;
+        }
+        W.cnt[i] = 0u;
+    }
+    float alo[3], ascale[3];
+    bool avalid[3];
+    for (int a = 0; a < 3; ++a) gs_axis_frame(cb, a, &alo[a], &ascale[a], &avalid[a]);
+    gs_wave_sync();
+    for (uint32_t i = lane; i < cnt; i += 64u) {
+        const uint32_t id = W.ids[b + i];
+        const float4 c = cent[id], l = plo[id], h = phi[id];
+        const float cc[3] = {c.x, c.y, c.z};
+        const int ol[3] = {ord_enc(l.x), ord_enc(l.y), ord_enc(l.z)};
+        const int oh[3] = {ord_enc(h.x), ord_enc(h.y), ord_enc(h.z)};
+        for (int a = 0; a < 3; ++a) {
+            if (!avalid[a]) continue;
+            const int k = a * kSahBins + gs_bin(cc[a], alo[a], ascale[a]);
+            for (int q = 0; q < 3; ++q) {
+                atomicMin(&W.box[k][q], ol[q]);
+                atomicMax(&W.box[k][3 + q], oh[q]);
+            }
+            atomicAdd(&W.cnt[k], 1u);
+        }
+    }
+    gs_wave_sync();
+    float best_cost = INFINITY;
+    int best_key = -1;
+    GsSide L, R;
+    for (int a = 0; a < 3; ++a) {
+        if (!avalid[a]) continue;
+        float c;
+        gs_sweep(&W.box[a * kSahBins], &W.cnt[a * kSahBins], lane, &c, &L, &R);
+        const int key = gs_argmin(c, a * kSahBins + lane);
+        if (key < 0) continue;
+        const int wl = key % kSahBins;
+        const float wc = __shfl(c, wl);
+        if (wc < best_cost) {
+            best_cost = wc;
+            best_key = key;
+            for (int q = 0; q < 3; ++q) {
+                Lo->lo[q] = __shfl(L.lo[q], wl);
+                Lo->hi[q] = __shfl(L.hi[q], wl);
+                Ro->lo[q] = __shfl(R.lo[q], wl);
+                Ro->hi[q] = __shfl(R.hi[q], wl);
+            }
+            Lo->n = (uint32_t)__shfl((int)L.n, wl);
+            Ro->n = (uint32_t)__shfl((int)R.n, wl);
+        }
+    }
+    if (best_key < 0) {  // every centroid in one place: halve in index order
+        const uint32_t m = cnt / 2u;
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t sb = side == 0 ? 0u : m, se = side == 0 ? m : cnt;
+            int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+            for (uint32_t i = sb + lane; i < se; i += 64u) {
+                const float4 l = plo[W.ids[b + i]], h = phi[W.ids[b + i]];
+                v[0] = min(v[0], ord_enc(l.x)); v[1] = min(v[1], ord_enc(l.y)); v[2] = min(v[2], ord_enc(l.z));
+                v[3] = max(v[3], ord_enc(h.x)); v[4] = max(v[4], ord_enc(h.y)); v[5] = max(v[5], ord_enc(h.z));
+            }
+            for (int off = 32; off > 0; off >>= 1)
+                for (int q = 0; q < 3; ++q) {
+                    v[q] = min(v[q], __shfl_xor(v[q], off));
+                    v[3 + q] = max(v[3 + q], __shfl_xor(v[3 + q], off));
+                }
+            GsSide* S = side == 0 ? Lo : Ro;
+            for (int q = 0; q < 3; ++q) {
+                S->lo[q] = ord_dec(v[q]);
+                S->hi[q] = ord_dec(v[3 + q]);
+            }
+            S->n = se - sb;
+        }
+        return m;
+    }
+    const uint32_t m = Lo->n;
+    if (partition) {
+        const int a = best_key / kSahBins, bin = best_key % kSahBins;
+        uint32_t pl = 0u, pr = m;
+        for (uint32_t base = 0; base < cnt; base += 64u) {
+            const uint32_t i = base + lane;
+            const bool live = i < cnt;
+            const uint32_t id = live ? W.ids[b + i] : 0u;
+            bool left = false;
+            if (live) {
+                const float4 c = cent[id];
+                const float cc = a == 0 ? c.x : (a == 1 ? c.y : c.z);
+                left = gs_bin(cc, alo[a], ascale[a]) <= bin;
+            }
+            const unsigned long long ml = __ballot(left), mr = __ballot(live && !left);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (left) W.tmp[pl + (uint32_t)__popcll(ml & below)] = id;
+            else if (live) W.tmp[pr + (uint32_t)__popcll(mr & below)] = id;
+            pl += (uint32_t)__popcll(ml);
+            pr += (uint32_t)__popcll(mr);
+        }
+        gs_wave_sync();
+        for (uint32_t i = lane; i < cnt; i += 64u) W.ids[b + i] = W.tmp[i];
+        gs_wave_sync();
+    }
+    return m;
+}
+
+// Builds each small range's subtree with one wave (4 waves per workgroup):
+// the host's job loop (sah_subtree) with its leaf rule.
+__global__ __launch_bounds__(256) void k_gs_small(const GsSeg* __restrict__ segs, uint32_t nseg,
+                                                  const float4* __restrict__ cent, const float4* __restrict__ plo,
+                                                  const float4* __restrict__ phi, uint32_t* __restrict__ ids,
+                                                  BvhNode* __restrict__ nodes, uint32_t* __restrict__ counters) {
+    __shared__ GsWave waves[4];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    const uint32_t s = blockIdx.x * 4u + (uint32_t)wv;
+    if (s >= nseg) return;
+    GsWave& W = waves[wv];
+    const GsSeg g = segs[s];
+    const uint32_t n = g.e - g.b;
+    for (uint32_t i = lane; i < n; i += 64u) W.ids[i] = ids[g.b + i];
+    if (lane == 0) {
+        GsJob& j = W.jobs[0];
+        j.b = 0u;
+        j.e = n;
+        j.parent = g.parent;
+        j.sd = g.sd;
+        for (int q = 0; q < 3; ++q) {
+            j.lo[q] = g.lo[q];
+            j.hi[q] = g.hi[q];
+        }
+    }
+    int top = 1;
+    uint32_t max_depth = 0;
+    gs_wave_sync();
+    while (top > 0) {
+        --top;
+        const GsJob j = W.jobs[top];
+        gs_wave_sync();
+        const uint32_t cnt = j.e - j.b;
+        const uint32_t depth = j.sd >> 1;
+        GsSide L, R;
+        if (cnt == 1u) {  // always a leaf (the root range holds at least two triangles)
+            if (lane == 0) gs_set_link(nodes, j.parent, (int)(j.sd & 1u), ~(int)((g.b + j.b) * 16u));
+            max_depth = max(max_depth, depth);
+            continue;
+        }
+        // the split (the range reordered left side first: a leaf's triangle
+        // set does not depend on the order)
+        const uint32_t m = gs_wave_split(W, j.b, cnt, cent, plo, phi, &L, &R, true);
+        if (j.parent >= 0 && cnt <= (uint32_t)kMaxLeafTris) {
+            // a leaf if that is cheaper than the best split
+            const float area = gs_area(j.lo, j.hi);
+            const float split =
+                kSahNode * area + kSahTri * ((float)m * gs_area(L.lo, L.hi) + (float)(cnt - m) * gs_area(R.lo, R.hi));
+            if (kSahTri * (float)cnt * area <= split) {
+                if (lane == 0)
+                    gs_set_link(nodes, j.parent, (int)(j.sd & 1u), ~(int)((g.b + j.b) * 16u + (cnt - 1u)));
+                max_depth = max(max_depth, depth);
+                continue;
+            }
+        }
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&counters[0], 1u);
+        k = (uint32_t)__shfl((int)k, 0);
+        if (lane == 0) {
+            BvhNode nd;
+            nd.a = nd.b = nd.c = make_float4(0.f, 0.f, 0.f, 0.f);
+            nd.d = make_int4(0, 0, 0, 0);
+            gs_set_child(&nd, 0, L.lo, L.hi);
+            gs_set_child(&nd, 1, R.lo, R.hi);
+            nodes[k] = nd;
+            if (j.parent >= 0) gs_set_link(nodes, j.parent, (int)(j.sd & 1u), (int)k);
+            GsJob cl, cr;
+            cl.b = j.b;
+            cl.e = j.b + m;
+            cr.b = j.b + m;
+            cr.e = j.e;
+            cl.parent = cr.parent = (int)k;
+            cl.sd = 0u | ((depth + 1u) << 1);
+            cr.sd = 1u | ((depth + 1u) << 1);
+            for (int q = 0; q < 3; ++q) {
+                cl.lo[q] = L.lo[q];
+                cl.hi[q] = L.hi[q];
+                cr.lo[q] = R.lo[q];
+                cr.hi[q] = R.hi[q];
+            }
+            const bool left_small = m <= cnt - m;
+            W.jobs[top] = left_small ? cr : cl;      // the larger waits
+            W.jobs[top + 1] = left_small ? cl : cr;  // the smaller goes next
+        }
+        top += 2;
+        gs_wave_sync();
+    }
+    for (uint32_t i = lane; i < n; i += 64u) ids[g.b + i] = W.ids[i];
+    for (int off = 32; off > 0; off >>= 1) max_depth = max(max_depth, (uint32_t)__shfl_xor((int)max_depth, off));
+    if (lane == 0) atomicMax(&counters[3], max_depth);
+}
+
+__global__ void k_gs_gather_tris(const BvhTri* __restrict__ ctris, const uint32_t* __restrict__ ids, uint32_t n,
+                                 BvhTri* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = ctris[ids[i]];
+}
+
+}  // namespace
+
+hipError_t build_sah_gpu(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                         const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count, hipStream_t st,
+                         BvhBuildResult* out, char* err, size_t err_len) {
+    if (tri_count < 2)
+        return build_lbvh(vertices, indices, offsets, inst_first_tri, instance_count, tri_count, st, out, err,
+                          err_len);
+    hipError_t rc = hipSuccess;
+    const uint32_t n = tri_count;
+    const uint32_t big_cap = n / kGsSmall + 2u;
+    const uint32_t chunk_cap = n / kGsChunk + big_cap + 2u;
+    BvhTri* ctris = nullptr;
+    float4 *cent = nullptr, *plo = nullptr, *phi = nullptr;
+    int* bounds = nullptr;
+    uint32_t *ids = nullptr, *ids_tmp = nullptr, *counters = nullptr, *chunk_left = nullptr, *chunk_off = nullptr,
+             *seg_first = nullptr;
+    GsSeg *segs = nullptr, *segs_next = nullptr, *small = nullptr;
+    GsBins* bins = nullptr;
+    GsSplit* split = nullptr;
+    GsChunk* chunks = nullptr;
+    BvhNode* nodes = nullptr;
+    BvhTri* tris = nullptr;
+    const int init_bounds[12] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000, (int)0x80000000,
+                                 (int)0x80000000, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, (int)0x80000000,
+                                 (int)0x80000000, (int)0x80000000};
+    int hb[12];
+    uint32_t hc[4] = {0u, 0u, 0u, 0u};
+    float pad = 0.0f, ext = 0.0f, mag = 0.0f;
+    std::vector<GsSeg> level;
+    std::vector<GsChunk> hchunks;
+    std::vector<uint32_t> hfirst;
+    GsSeg root;
+    const auto t0 = std::chrono::steady_clock::now();
+    XR_TRY(hipMalloc(&ctris, sizeof(BvhTri) * n));
+    XR_TRY(hipMalloc(&cent, sizeof(float4) * n));
+    XR_TRY(hipMalloc(&plo, sizeof(float4) * n));
+    XR_TRY(hipMalloc(&phi, sizeof(float4) * n));
+    XR_TRY(hipMalloc(&bounds, sizeof(int) * 12));
+    XR_TRY(hipMalloc(&ids, sizeof(uint32_t) * n));
+    XR_TRY(hipMalloc(&ids_tmp, sizeof(uint32_t) * n));
+    XR_TRY(hipMalloc(&counters, sizeof(uint32_t) * 4));
+    XR_TRY(hipMalloc(&segs, sizeof(GsSeg) * big_cap));
+    XR_TRY(hipMalloc(&segs_next, sizeof(GsSeg) * big_cap));
+    XR_TRY(hipMalloc(&small, sizeof(GsSeg) * n));
+    XR_TRY(hipMalloc(&bins, sizeof(GsBins) * big_cap));
+    XR_TRY(hipMalloc(&split, sizeof(GsSplit) * big_cap));
+    XR_TRY(hipMalloc(&chunks, sizeof(GsChunk) * chunk_cap));
+    XR_TRY(hipMalloc(&chunk_left, sizeof(uint32_t) * chunk_cap));
+    XR_TRY(hipMalloc(&chunk_off, sizeof(uint32_t) * chunk_cap));
+    XR_TRY(hipMalloc(&seg_first, sizeof(uint32_t) * (big_cap + 1)));
+    XR_TRY(hipMalloc(&nodes, sizeof(BvhNode) * n));
+    XR_TRY(hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, st));
+    XR_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 4, st));
+    launch_gather(vertices, indices, offsets, inst_first_tri, instance_count, n, ctris, cent, bounds, st);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    // box_pad, as the host builder computes it
+    for (int a = 0; a < 3; ++a) {
+        const float lo = ord_dec_host(hb[6 + a]), hi = ord_dec_host(hb[9 + a]);
+        ext = fmaxf(ext, hi - lo);
+        mag = fmaxf(mag, fmaxf(fabsf(lo), fabsf(hi)));
+    }
+    pad = fmaxf(ext * (1.0f / 65536.0f), mag * (1.0f / 1048576.0f));
+    hipLaunchKernelGGL(k_gs_prims, dim3((n + 255) / 256), dim3(256), 0, st, ctris, n, pad, plo, phi, ids);
+    XR_TRY(hipGetLastError());
+    root.b = 0u;
+    root.e = n;
+    root.parent = -1;
+    root.sd = 0u;
+    for (int a = 0; a < 3; ++a) {  // the root's box is never stored
+        root.lo[a] = 0.0f;
+        root.hi[a] = 0.0f;
+    }
+    if (n > kGsSmall) {
+        level.push_back(root);
+    } else {
+        XR_TRY(hipMemcpyAsync(small, &root, sizeof(GsSeg), hipMemcpyHostToDevice, st));
+        hc[2] = 1u;
+        XR_TRY(hipMemcpyAsync(counters + 2, &hc[2], sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    while (!level.empty()) {
+        const uint32_t ns = (uint32_t)level.size();
+        hchunks.clear();
+        hfirst.assign(ns + 1, 0u);
+        for (uint32_t s = 0; s < ns; ++s) {
+            hfirst[s] = (uint32_t)hchunks.size();
+            for (uint32_t b = level[s].b; b < level[s].e; b += kGsChunk)
+                hchunks.push_back(GsChunk{s, b, std::min(b + kGsChunk, level[s].e), hfirst[s]});
+        }
+        hfirst[ns] = (uint32_t)hchunks.size();
+        const uint32_t nc = (uint32_t)hchunks.size();
+        XR_TRY(hipMemcpyAsync(segs, level.data(), sizeof(GsSeg) * ns, hipMemcpyHostToDevice, st));
+        XR_TRY(hipMemcpyAsync(chunks, hchunks.data(), sizeof(GsChunk) * nc, hipMemcpyHostToDevice, st));
+        XR_TRY(hipMemcpyAsync(seg_first, hfirst.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice, st));
+        XR_TRY(hipMemsetAsync(counters + 1, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(k_gs_init_bins, dim3(ns), dim3(64), 0, st, bins, ns);
+        hipLaunchKernelGGL(k_gs_cbounds, dim3(nc), dim3(256), 0, st, chunks, ids, cent, bins);
+        hipLaunchKernelGGL(k_gs_bin, dim3(nc), dim3(256), 0, st, chunks, ids, cent, plo, phi, bins);
+        hipLaunchKernelGGL(k_gs_split, dim3(ns), dim3(64), 0, st, segs, ns, bins, split, nodes, counters, segs_next,
+                           small);
+        hipLaunchKernelGGL(k_gs_count, dim3(nc), dim3(256), 0, st, chunks, ids, cent, split, chunk_left);
+        hipLaunchKernelGGL(k_gs_chunk_scan, dim3(ns), dim3(256), 0, st, chunks, seg_first, chunk_left, chunk_off);
+        hipLaunchKernelGGL(k_gs_scatter, dim3(nc), dim3(256), 0, st, chunks, segs, ids, cent, split, chunk_off,
+                           ids_tmp);
+        hipLaunchKernelGGL(k_gs_copy_back, dim3(nc), dim3(256), 0, st, chunks, ids_tmp, ids);
+        hipLaunchKernelGGL(k_gs_median_boxes, dim3(ns), dim3(64), 0, st, segs, ns, split, ids, plo, phi, nodes,
+                           segs_next, small, counters);
+        XR_TRY(hipGetLastError());
+        XR_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+        XR_TRY(hipStreamSynchronize(st));
+        level.resize(hc[1]);
+        if (hc[1]) {
+            XR_TRY(hipMemcpyAsync(level.data(), segs_next, sizeof(GsSeg) * hc[1], hipMemcpyDeviceToHost, st));
+            XR_TRY(hipStreamSynchronize(st));
+        }
+    }
+    XR_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    if (hc[2]) hipLaunchKernelGGL(k_gs_small, dim3((hc[2] + 3u) / 4u), dim3(256), 0, st, small, hc[2], cent, plo,
+                                  phi, ids, nodes, counters);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipMalloc(&tris, sizeof(BvhTri) * n));
+    hipLaunchKernelGGL(k_gs_gather_tris, dim3((n + 255) / 256), dim3(256), 0, st, ctris, ids, n, tris);
+    XR_TRY(hipGetLastError());
+    XR_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+    XR_TRY(hipStreamSynchronize(st));
+    out->nodes = nodes;
+    out->tris = tris;
+    out->node_count = hc[0];
+    out->tri_count = n;
+    out->max_depth = hc[3];
+    out->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    nodes = nullptr;
+    tris = nullptr;
+    rc = hipSuccess;
+    goto done;
+fail:
+    rc = hipErrorUnknown;
+done:
+    hipFree(ctris);
+    hipFree(cent);
+    hipFree(plo);
+    hipFree(phi);
+    hipFree(bounds);
+    hipFree(ids);
+    hipFree(ids_tmp);
+    hipFree(counters);
+    hipFree(segs);
+    hipFree(segs_next);
+    hipFree(small);
+    hipFree(bins);
+    hipFree(split);
+    hipFree(chunks);
+    hipFree(chunk_left);
+    hipFree(chunk_off);
+    hipFree(seg_first);
     hipFree(nodes);
     hipFree(tris);
     return rc;

@@ -114,6 +114,11 @@ hipError_t build_ploc(const float* vertices, const uint32_t* indices, const uint
 hipError_t build_sah(const float* vertices, const uint32_t* indices, const uint2* offsets,
                      const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
                      hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
+// The same binned SAH built on the GPU (the same tree: every split is a
+// function of the node's triangle set).
+hipError_t build_sah_gpu(const float* vertices, const uint32_t* indices, const uint2* offsets,
+                         const uint32_t* inst_first_tri, uint32_t instance_count, uint32_t tri_count,
+                         hipStream_t st, BvhBuildResult* out, char* err, size_t err_len);
 
 // The traversed tree: the binary tree of a build collapsed top-down into
 // nodes of up to `width` (2..4) children.  `nodes` is a device allocation

@@ -31,6 +31,7 @@ XR_PASS_INITIAL, XR_PASS_TEMPORAL, XR_PASS_SPATIAL = 1, 2, 4
 XR_BUILDER_LBVH = 0
 XR_BUILDER_PLOC = 1
 XR_BUILDER_SAH = 2
+XR_BUILDER_SAH_HOST = 3
 XRH_GLTF_UNTEXTURED = 0x1
 
 XR_BUF_ACCUMULATION = 1
