@@ -16,10 +16,12 @@ def test_kernel_resources():
     assert set(info) == {"k_initial_sample", "k_tail_bounce", "k_temporal", "k_spatial", "k_plain", "k_converge"}
     for k in info.values():
         assert 0 < k["vgprs"] <= 512 and k["blocks_per_cu"] >= 1 and k["block"] in (256,)
-    # the spatial pass: 20 stack levels of 256 lanes in LDS, 7 waves per SIMD
-    assert info["k_spatial"]["lds_bytes"] >= 20 * 256 * 4
-    assert info["k_spatial"]["waves_per_simd"] == 7
+    # the spatial pass and the initial-sample head: 20 stack levels of 256
+    # lanes in LDS, 7 waves per SIMD
+    for name in ("k_spatial", "k_initial_sample"):
+        assert info[name]["lds_bytes"] >= 20 * 256 * 4
+        assert info[name]["waves_per_simd"] == 7
     # the other traversal kernels: 24 levels, 6 waves
-    for name in ("k_initial_sample", "k_tail_bounce", "k_plain"):
+    for name in ("k_tail_bounce", "k_plain"):
         assert info[name]["lds_bytes"] >= 24 * 256 * 4
         assert info[name]["waves_per_simd"] == 6

@@ -214,7 +214,7 @@ int alloc_frame(xr_ctx* c, uint32_t w, uint32_t h) {
     XR_HIP(c, dev_alloc(&c->d_radiance, n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_list, (2 + 2 * n) * sizeof(uint32_t)));
     XR_HIP(c, dev_alloc(&c->d_tail, 2 * 4 * n * f4));  // two path queues, one per bounce parity
-    XR_HIP(c, dev_alloc(&c->d_tail_count, 3 * sizeof(uint32_t)));
+    XR_HIP(c, dev_alloc(&c->d_tail_count, kTailCounters * sizeof(uint32_t)));
     XR_HIP(c, dev_alloc(&c->d_nbr, 4 * n * f4));
     XR_HIP(c, dev_alloc(&c->d_cache_work, (n + (n + kCacheSelBlock - 1) / kCacheSelBlock + 1) * sizeof(uint32_t)));
     XR_HIP(c, hipMemsetAsync(c->d_cache_list, 0, (2 + 2 * n) * sizeof(uint32_t), c->stream));

@@ -52,11 +52,18 @@ struct FrameDev {
     uint32_t* cache_blocks;
     // Paths alive after the head bounces of the initial-sample pass (4 float4
     // each, n capacity) and their count; the single-view pass keeps a second
-    // queue of n paths at tail + 4n, count at tail_count[1], which in cache
-    // mode first holds the candidates' pending records (count tail_count[2]).
+    // queue of n paths at tail + 4n, which in cache mode first holds the
+    // candidates' pending records (count tail_count[kPendingCount]).  Tail
+    // launch j (bounce kHeadBounces + j) reads queue j % 2 with count
+    // tail_count[j % 3], appends to the other queue with count
+    // tail_count[(j + 1) % 3] and clears tail_count[(j + 2) % 3], the next
+    // launch's output count (the previous launch's input, read by no one
+    // during launch j), so one clear before the head serves every launch.
     float4* tail;
     uint32_t* tail_count;
 };
+constexpr uint32_t kPendingCount = 3;  // word of tail_count counting pending cache records
+constexpr uint32_t kTailCounters = 4;  // words of tail_count
 
 // Parameters of one ReSTIR GI / plain frame as the kernels see them.
 struct PassParams {

@@ -418,10 +418,14 @@ __device__ __forceinline__ void hand_on(bool alive, float4* tail, uint32_t* tail
     if (alive) store_path(tail + 4 * (size_t)q, pix, view, ps, o);
 }
 
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sample(SceneDev sc, FrameDev fr,
-                                                                                   UniformBufferObject ubo,
-                                                                                   CacheCfg cache) {
-    __shared__ int stack[kLdsDepth * kBlock];
+// The head kernel keeps 20 stack levels in LDS (20 KiB per workgroup) and
+// asks for 7 waves per SIMD (72 VGPRs, 224 bytes of scratch per lane): the
+// initial pass of the 1080p terrain frame took 1.003 ms against 1.035 at 24
+// levels and 6 waves (18 levels at 8 waves: 1.005, 22 at 7: 1.003).
+constexpr int kHeadLds = 20;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_initial_sample(
+    SceneDev sc, FrameDev fr, UniformBufferObject ubo, CacheCfg cache) {
+    __shared__ int stack[kHeadLds * kBlock];
     __shared__ uint32_t lds_count;
     __shared__ uint32_t lds_base;
     const TileCoord tc = tile_pixel(fr.width, fr.height);
@@ -430,7 +434,7 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
     PathState ps;
     PathOut o;
     if (tc.in_image) {
-        alive = initial_head_pixel<kLdsDepth>(sc, fr, ubo, tc.x, tc.y, cache, stack + threadIdx.x, &ps, &o, &rays);
+        alive = initial_head_pixel<kHeadLds>(sc, fr, ubo, tc.x, tc.y, cache, stack + threadIdx.x, &ps, &o, &rays);
         kind = o.cache;
         valid = o.valid ? 1u : 0u;
     }
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_initial_sampl
     hand_on(alive && !pending, fr.tail, fr.tail_count, pix, 0u, ps, o, &lds_count, &lds_base);
     if (cache.enabled) {  // uniform over the launch
         // candidates park their record (alive or not) in the second queue
-        const uint32_t q = block_slot(pending, fr.tail_count + 2, &lds_count, &lds_base);
+        const uint32_t q = block_slot(pending, fr.tail_count + kPendingCount, &lds_count, &lds_base);
         if (pending) {
             o.cache = kind | (alive ? kRecordAlive : 0u);
             store_path(fr.tail + 4 * (size_t)fr.n + 4 * (size_t)q, pix, 0u, ps, o);
@@ -627,10 +631,11 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_tail_bounce(S
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     __shared__ uint32_t lds_base;
-    const uint32_t side = (bounce - kHeadBounces) & 1u;
+    const uint32_t j = bounce - kHeadBounces, side = j & 1u;
     const float4* in = fr.tail + (size_t)side * 4 * fr.n;
     float4* out = fr.tail + (size_t)(side ^ 1u) * 4 * fr.n;
-    const uint32_t count = fr.tail_count[side];
+    const uint32_t count = fr.tail_count[j % 3u];
+    if (blockIdx.x == 0u && threadIdx.x == 0u) fr.tail_count[(j + 2u) % 3u] = 0u;  // the next launch's output
     if (blockIdx.x * kBlock >= count) return;  // uniform over the workgroup
     const bool one = (uint64_t)count * 4u > (uint64_t)fr.n;  // uniform over the launch
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
@@ -656,7 +661,7 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_tail_bounce(S
                                         stack + threadIdx.x, &o, cache, &rays);
         if (!alive) fr.samples[4 * (size_t)fr.n + pix] = f4(o.L, __uint_as_float(o.cache));
     }
-    if (one) hand_on(alive, out, fr.tail_count + (side ^ 1u), pix, 0u, ps, o, &lds_count, &lds_base);
+    if (one) hand_on(alive, out, fr.tail_count + (j + 1u) % 3u, pix, 0u, ps, o, &lds_count, &lds_base);
     count_rays(&fr.counters[0], rays, &lds_count);
 }
 
@@ -1408,10 +1413,11 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain_bounce(
     __shared__ int stack[kLdsDepth * kBlock];
     __shared__ uint32_t lds_count;
     __shared__ uint32_t lds_base;
-    const uint32_t side = (bounce - kHeadBounces) & 1u;
+    const uint32_t j = bounce - kHeadBounces, side = j & 1u;
     const float4* in = fr.tail + (size_t)side * 4 * fr.n;
     float4* out = fr.tail + (size_t)(side ^ 1u) * 4 * fr.n;
-    const uint32_t count = fr.tail_count[side];
+    const uint32_t count = fr.tail_count[j % 3u];
+    if (blockIdx.x == 0u && threadIdx.x == 0u) fr.tail_count[(j + 2u) % 3u] = 0u;  // the next launch's output
     if (blockIdx.x * kBlock >= count) return;  // uniform over the workgroup
     const bool one = !last && (uint64_t)count * 4u > (uint64_t)fr.n;  // uniform over the launch
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
@@ -1432,7 +1438,7 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_plain_bounce(
                               stack + threadIdx.x, &rays);
         if (!alive) plain_finish(fr, ubo, pix, add(mk3(0.f, 0.f, 0.f), pp.color));
     }
-    if (one) plain_hand_on(alive, out, fr.tail_count + (side ^ 1u), pix, pp, &lds_count, &lds_base);
+    if (one) plain_hand_on(alive, out, fr.tail_count + (j + 1u) % 3u, pix, pp, &lds_count, &lds_base);
     count_rays(&fr.counters[0], rays, &lds_count);
 }
 
@@ -1468,7 +1474,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __r
 // ---------------------------------------------------------------------------
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
                                  const CacheCfg& cache, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, cache.enabled ? 3u : 1u);
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, kTailCounters);
     hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        cache);
     if (cache.enabled) {
@@ -1478,12 +1484,9 @@ hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const U
         hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(256), 0, st, fr, cache);
     }
     const uint32_t nb = ubo.numberOfBounces;
-    for (uint32_t b = kHeadBounces; b < nb; ++b) {
-        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count + ((b - kHeadBounces + 1u) & 1u),
-                           1u);
+    for (uint32_t b = kHeadBounces; b < nb; ++b)
         hipLaunchKernelGGL(k_tail_bounce, dim3((fr.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, sc, fr, ubo,
                            cache, b);
-    }
     return hipGetLastError();
 }
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
@@ -1514,16 +1517,13 @@ hipError_t launch_plain(const SceneDev& sc, const FrameDev& fr, const UniformBuf
     // Launches past the head: one per bounce up to the depth cap, at most
     // kPlainLaunches (unbounded paths: the last launch finishes every path).
     constexpr uint32_t kPlainLaunches = 12;
-    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, 1u);
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, kTailCounters);
     hipLaunchKernelGGL(k_plain_head, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        max_depth);
     const uint32_t end = max_depth != 0u ? max_depth : kHeadBounces + kPlainLaunches;
-    for (uint32_t b = kHeadBounces; b < end; ++b) {
-        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count + ((b - kHeadBounces + 1u) & 1u),
-                           1u);
+    for (uint32_t b = kHeadBounces; b < end; ++b)
         hipLaunchKernelGGL(k_plain_bounce, dim3((fr.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, sc, fr, ubo,
                            max_depth, b, b + 1u == end ? 1u : 0u);
-    }
     return hipGetLastError();
 }
 hipError_t launch_converge(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
