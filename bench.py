@@ -41,16 +41,24 @@ FOV = 45.0
 ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 
 # HBM bytes per launch of each pass's kernel on this workload, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1
-# --no-cpu-baseline --no-mlp-study` (KiB per dispatch, averaged over the
-# frames; the initial pass adds its head kernel and its per-bounce tail
-# launches of one frame), each scaled by what the counter reads for a plain 1 GiB
-# device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
-# 524,302 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
+# --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 20 --warmup 3
+# --no-cpu-baseline --no-mlp-study` (KiB per dispatch, averaged over the 20
+# timed frames; the initial pass adds its head kernel and the two tail
+# launches of one frame), each scaled by what the counter reads for a plain
+# 1 GiB device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
+# 524,316 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 171701.1 + 20709.8, "temporal": 236400.8, "spatial": 787109.2}
-PMC_WRITE_KIB = {"initial": 750600.7 + 20043.4, "temporal": 215782.0, "spatial": 253955.2}
+PMC_FETCH_KIB = {"initial": 25314.2 + 23280.8, "temporal": 258418.1, "spatial": 622632.7}
+PMC_WRITE_KIB = {"initial": 380620.6 + 33164.3, "temporal": 340198.5, "spatial": 282938.0}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+# The traversal kernels are bound by VALU issue, not by HBM: wave64 VALU
+# instructions per frame (SQ_INSTS_VALU in a rocprofv3 --pmc run of the same
+# command, averaged over the same frames; the initial pass adds its tail
+# launches, which its timing events bracket too) against the issue peak of 1,024
+# SIMDs, each retiring one wave64 VALU instruction per 4 cycles at 2.4 GHz.
+PMC_VALU_INSTS = {"initial": 462129976.5 + 83751663.1, "spatial": 668453687.0}
+VALU_PEAK_GINST_S = 1024 * 2.4 / 4  # 614.4 G wave64 instructions per second
 
 
 def pmc_traffic(name: str) -> int:
@@ -655,6 +663,14 @@ def main():
         "traffic": pmc_traffic(dom),
         "algo_bytes_per_launch": ALGO_BYTES[dom] * W * H,
     }
+    # head kernel and spatial kernel against the VALU issue roofline
+    issue = {}
+    for name, kern in (("initial", "k_initial_sample+k_tail_bounce"), ("spatial", "k_spatial")):
+        ms = pass_ms[names.index(name)] / frames
+        ginst = PMC_VALU_INSTS[name] / (ms * 1e-3) / 1e9
+        issue[kern] = {"bound": "valu", "achieved": round(ginst, 1), "peak": VALU_PEAK_GINST_S,
+                       "unit": "G wave64 VALU instructions/s", "frac": round(ginst / VALU_PEAK_GINST_S, 4),
+                       "valu_insts_per_frame": int(PMC_VALU_INSTS[name])}
     result = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -685,6 +701,7 @@ def main():
         "passes": per_pass,
         "temporal_plus_spatial_ms": round(per_pass["temporal"]["ms"] + per_pass["spatial"]["ms"], 4),
         "roofline": roofline,
+        "issue_roofline": issue,
         # registers, scratch, LDS and resident waves of the traversal kernels
         "kernels": {k["name"]: {f: k[f] for f in ("vgprs", "scratch_bytes", "lds_bytes", "waves_per_simd")}
                     for k in N.kernel_info() if k["name"] in ("k_initial_sample", "k_tail_bounce", "k_spatial")},

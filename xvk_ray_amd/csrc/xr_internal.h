@@ -255,6 +255,8 @@ __device__ __forceinline__ bool trace_ray_ww(const WideNode* __restrict__ nodes,
     // A well-formed tree visits each node at most once per ray; the cap only
     // guarantees termination should a malformed tree ever reach the kernel.
     uint32_t guard = 0;
+    // an any-hit ray returns at its first hit, so its far bound never changes
+    const float tb_any = tmax * 1.0000003576f;
     while (node != kDone) {
         while (node >= 0 && node != kDone) {
             if (++guard >= kMaxNodeVisits) {
@@ -265,7 +267,7 @@ __device__ __forceinline__ bool trace_ray_ww(const WideNode* __restrict__ nodes,
             int4 lk;
             bool h[4];
             float e[4];
-            wide_slabs(nodes + node, r, tmin, best_t * 1.0000003576f, &lk, h, e);
+            wide_slabs(nodes + node, r, tmin, kAnyHit ? tb_any : best_t * 1.0000003576f, &lk, h, e);
             int next = 0;
             bool have = false;
             if (h[3]) { next = lk.w; have = true; }
