@@ -388,8 +388,10 @@ def run_c3(args) -> dict:
     2x 1280x720, 8,326,144 pixels) of the 1080p terrain scene, each on its own
     seeded camera path (orbit seeds 2..13, 64 frames) with its own reservoir
     buffers, ReSTIR GI as in the headline workload; one launch per pass for
-    all views, the frame replayed as a HIP graph; tile shape and LDS stack
-    depth chosen by the autotuner on the first frame's views."""
+    all views, the frame replayed as a HIP graph; tile shape, LDS stack
+    depth and launch form (a workgroup per tile, or persistent workgroups
+    looping over per-XCD tile queues) chosen by the autotuner on the first
+    frame's views."""
     from xvk_ray_amd import native as N
 
     frames = 64
@@ -409,7 +411,7 @@ def run_c3(args) -> dict:
     tune = batch.autotune(all_ubos[0], params, frames=4)
     cfg = batch.config
     log(f"c3 autotune ({time.perf_counter() - t:.1f} s): {[round(x, 3) for x in tune]} -> config {cfg} "
-        f"{N.Batch.config_info(cfg)}")
+        f"{N.Batch.config_info(cfg)} persistent {N.Batch.config_persistent(cfg)}")
     warm = min(args.warmup, frames)
     for f in range(warm):
         batch.render(all_ubos[f], params)
@@ -434,8 +436,11 @@ def run_c3(args) -> dict:
     res.update({"ms_per_frame": round(ms, 4), "ns_per_pixel": round(ms * 1e6 / pixels, 3),
                 "rays_per_frame": (ext + shadow) // timed,
                 "autotune": {"ms_per_config": [round(x, 4) for x in tune],
-                             "configs": [N.Batch.config_info(k) for k in range(N.Batch.config_count())],
-                             "chosen": {"tile": f"{tw}x{th}", "lds_stack_depth": lds}}})
+                             "configs": [list(N.Batch.config_info(k)) + [N.Batch.config_persistent(k)]
+                                         for k in range(N.Batch.config_count())],
+                             "config_fields": ["tile_w", "tile_h", "lds_stack_depth", "persistent"],
+                             "chosen": {"tile": f"{tw}x{th}", "lds_stack_depth": lds,
+                                        "persistent": N.Batch.config_persistent(cfg)}}})
     batch.close()
     ctx.close()
     return res

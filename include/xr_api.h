@@ -418,8 +418,9 @@ int xr_mlp_params(xr_ctx* ctx, float* params, int set);
 /* ---- batched views (no reference counterpart) ----
  * Several cameras on the scene of one context, each with its own frame
  * buffers and reservoir history, rendered in ReSTIR GI mode with one kernel
- * launch per pass for all views; the frame is captured once as a HIP graph
- * and replayed.  Each view's results equal those of rendering it alone.
+ * launch per pass for all views (one workgroup per tile, or persistent
+ * workgroups looping over the views' tiles); the frame is captured once as a
+ * HIP graph and replayed.  Each view's results equal those of rendering it alone.
  * Errors are reported through xr_last_error(ctx).  Destroy a batch before
  * its context. */
 typedef struct xr_batch xr_batch;
@@ -435,9 +436,15 @@ int xr_batch_frame_time(xr_batch* batch, float* ms);
  * parameters or the kernel configuration change, or xr_set_textures /
  * xr_build_accel replaced the scene's device memory since the capture. */
 int xr_batch_set_graph(xr_batch* batch, int enable);
-/* Kernel configurations (tile shape and LDS traversal-stack depth). */
+/* Kernel configurations: tile shape, LDS traversal-stack depth and launch
+ * form.  A grid configuration launches one workgroup per tile; a persistent
+ * one launches as many workgroups as the GPU holds at once, each looping
+ * over tiles taken from a per-XCD queue (workgroup g works in XCD g % 8's
+ * band of tile rows).  Results are identical in every configuration. */
 uint32_t xr_batch_config_count(void);
 int xr_batch_config_info(uint32_t config, uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_depth);
+/* *persistent = 1: persistent launches, 0: one workgroup per tile. */
+int xr_batch_config_persistent(uint32_t config, int* persistent);
 int xr_batch_set_config(xr_batch* batch, uint32_t config);
 uint32_t xr_batch_config(const xr_batch* batch);
 /* Autotuner: renders frames (1..64) of the given views with every

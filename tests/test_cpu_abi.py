@@ -93,6 +93,23 @@ def test_kernel_info_arguments():
     assert L.xr_kernel_info_get(0, None) == N.XR_ERR_INVALID_ARGUMENT
 
 
+def test_batch_configurations():
+    """Nine tile shape x stack depth combinations, each once with one
+    workgroup per tile and once persistent, in that order."""
+    L = N.lib()
+    n = N.Batch.config_count()
+    assert n == 18
+    info = [N.Batch.config_info(k) for k in range(n)]
+    assert info[:9] == info[9:]
+    assert len(set(info[:9])) == 9
+    for tw, th, lds in info:
+        assert tw * th == 256 and lds in (16, 20, 24)
+    assert [N.Batch.config_persistent(k) for k in range(n)] == [False] * 9 + [True] * 9
+    p = C.c_int()
+    assert L.xr_batch_config_persistent(n, C.byref(p)) == N.XR_ERR_INVALID_ARGUMENT
+    assert L.xr_batch_config_persistent(0, None) == N.XR_ERR_INVALID_ARGUMENT
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_kernel_info_without_device():
     k = N.XrKernelInfo()

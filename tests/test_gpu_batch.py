@@ -66,15 +66,15 @@ def _check(batch, ref, label):
 
 
 @pytest.mark.parametrize("graph", [True, False])
-@pytest.mark.parametrize("cfg", range(9))
+@pytest.mark.parametrize("cfg", range(18))
 def test_batch_equals_single_views(scene, singles, cfg, graph):
     seq, ref = singles
-    assert N.Batch.config_count() == 9
+    assert N.Batch.config_count() == 18
     ctx = N.Context(scene, 8, 8)
     batch = N.Batch(ctx, SIZES)
     batch.set_config(cfg)
     batch.set_graph(graph)
-    say(f"[batch] cfg {cfg} {N.Batch.config_info(cfg)} graph {graph}")
+    say(f"[batch] cfg {cfg} {N.Batch.config_info(cfg)} persistent {N.Batch.config_persistent(cfg)} graph {graph}")
     p = N.default_params(**PARAMS)
     for f, row in enumerate(seq, 1):
         batch.render(row, p)
@@ -111,7 +111,7 @@ def test_autotune_picks_fastest_and_clears(scene):
     row = ubo_sequence(1)[0]
     ms = batch.autotune(row, N.default_params(**PARAMS), frames=2)
     say(f"[batch] autotune ms {['%.3f' % x for x in ms]} -> {batch.config}")
-    assert len(ms) == 9 and all(x > 0 for x in ms)
+    assert len(ms) == 18 and all(x > 0 for x in ms)
     assert batch.config == int(np.argmin(ms))
     for v in range(len(SIZES)):
         assert not batch.read(v, N.XR_BUF_TEMPORAL, N.XR_LAYOUT_NATIVE).any()

@@ -1287,6 +1287,7 @@ BatchDev batch_dev(const xr_batch* b) {
     bd.nviews = b->nviews;
     bd.tail = b->d_tail;
     bd.tail_count = b->d_tail_count;
+    bd.tile_queue = b->d_tail_count + 1;
     bd.counters = b->d_counters;
     return bd;
 }
@@ -1389,7 +1390,7 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
     XR_BATCH_HIP(dev_alloc(&b->d_frames, sizeof(FrameDev) * view_count));
     XR_BATCH_HIP(dev_alloc(&b->d_ubos, sizeof(UniformBufferObject) * view_count));
     XR_BATCH_HIP(dev_alloc(&b->d_tail, 4 * total * f4));
-    XR_BATCH_HIP(dev_alloc(&b->d_tail_count, sizeof(uint32_t)));
+    XR_BATCH_HIP(dev_alloc(&b->d_tail_count, kBatchCountWords * sizeof(uint32_t)));
     XR_BATCH_HIP(dev_alloc(&b->d_counters, sizeof(unsigned long long) * kCounters));
     XR_BATCH_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
     XR_BATCH_HIP(hipEventCreate(&b->e0));
@@ -1472,7 +1473,18 @@ uint32_t xr_batch_config_count(void) { return batch_config_count(); }
 int xr_batch_config_info(uint32_t config, uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_depth) {
     if (config >= batch_config_count() || !tile_w || !tile_h || !lds_depth)
         return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "config %u / NULL argument", config);
-    batch_config_info(config, tile_w, tile_h, lds_depth);
+    bool persistent = false;
+    batch_config_info(config, tile_w, tile_h, lds_depth, &persistent);
+    return XR_OK;
+}
+
+int xr_batch_config_persistent(uint32_t config, int* persistent) {
+    if (config >= batch_config_count() || !persistent)
+        return set_err(nullptr, XR_ERR_INVALID_ARGUMENT, "config %u / NULL argument", config);
+    uint32_t tw, th, lds;
+    bool p = false;
+    batch_config_info(config, &tw, &th, &lds, &p);
+    *persistent = p ? 1 : 0;
     return XR_OK;
 }
 

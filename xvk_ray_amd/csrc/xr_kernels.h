@@ -19,11 +19,13 @@ struct BatchDev {
     const UniformBufferObject* ubos;
     uint32_t nviews;
     float4* tail;
-    uint32_t* tail_count;
+    uint32_t* tail_count;   // 1 + 16 words: the tail count, then the tile queues
+    uint32_t* tile_queue;   // tail_count + 1: per-XCD tile counters of the persistent initial / spatial kernels
     unsigned long long* counters;
 };
+constexpr uint32_t kBatchCountWords = 17;
 uint32_t batch_config_count();
-void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds);
+void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds, bool* persistent);
 hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
                               const PassParams& pp, hipStream_t st);
 // Neighbour records of the spatial pass from the samples and the temporal

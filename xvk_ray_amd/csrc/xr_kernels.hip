@@ -11,6 +11,9 @@
 // bands (the hardware hands workgroup i to XCD i % 8), so each XCD's L2 holds
 // one band of the image, its G-buffer/reservoir neighbourhoods and the BVH
 // nodes its rays touch.
+#include <algorithm>
+#include <mutex>
+
 #include "xr_internal.h"
 #include "xr_kernels.h"
 
@@ -1153,26 +1156,54 @@ __device__ __forceinline__ uint32_t batch_view(const BatchDev& bd, uint32_t b, u
     return v;
 }
 
-template <int kTw, int kTh, int kLds>
-__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_initial(SceneDev sc, BatchDev bd) {
+// Next tile of a batched tiled kernel.  Grid form: the workgroup's own tile,
+// once.  Persistent form (a grid of as many workgroups as the GPU holds at
+// once, each looping): workgroup g serves XCD g % 8 (the hardware deals
+// workgroup g to XCD g % 8, checked with the XCC_ID register on a 1,792- and
+// a 32,768-workgroup grid) and takes that XCD's tiles t * 8 + g % 8 from the
+// XCD's queue counter in order, so each XCD still walks its own band of tile
+// rows while its workgroups balance the load among themselves; false when
+// the XCD has no tile left (tiles is a multiple of 8).  Called by every
+// thread of the workgroup.
+template <bool kPersistent>
+__device__ __forceinline__ bool batch_tile(uint32_t* queue, uint32_t tiles, uint32_t it, uint32_t* lds_tile,
+                                           uint32_t* b) {
+    if (!kPersistent) {
+        *b = blockIdx.x;
+        return it == 0u;
+    }
+    const uint32_t x = blockIdx.x % kXcds;
+    __syncthreads();  // every thread has read the previous tile
+    if (threadIdx.x == 0) *lds_tile = atomicAdd(queue + x, 1u) * kXcds + x;
+    __syncthreads();
+    *b = *lds_tile;
+    return *b < tiles;
+}
+
+template <int kTw, int kTh, int kLds, bool kPersistent>
+__global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_initial(SceneDev sc, BatchDev bd,
+                                                                                  uint32_t tiles) {
     __shared__ int stack[kLds * kBlock];
     __shared__ uint32_t lds_count;
     __shared__ uint32_t lds_base;
-    uint32_t local;
-    const uint32_t v = batch_view(bd, blockIdx.x, kTw, kTh, &local);
-    const FrameDev fr = bd.views[v];
-    const UniformBufferObject& ubo = bd.ubos[v];
-    const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
+    __shared__ uint32_t lds_tile;
     const CacheCfg nocache = {0u, 1u, 0u, 0.0f};
-    uint32_t rays = 0, valid = 0;
-    bool alive = false;
-    PathState ps;
-    PathOut o;
-    if (tc.in_image) {
-        alive = initial_head_pixel<kLds>(sc, fr, ubo, tc.x, tc.y, nocache, stack + threadIdx.x, &ps, &o, &rays);
-        valid = o.valid ? 1u : 0u;
+    uint32_t rays = 0, valid = 0, b = 0;
+    for (uint32_t it = 0; batch_tile<kPersistent>(bd.tile_queue, tiles, it, &lds_tile, &b); ++it) {
+        uint32_t local;
+        const uint32_t v = batch_view(bd, b, kTw, kTh, &local);
+        const FrameDev fr = bd.views[v];
+        const UniformBufferObject& ubo = bd.ubos[v];
+        const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
+        bool alive = false;
+        PathState ps;
+        PathOut o;
+        if (tc.in_image) {
+            alive = initial_head_pixel<kLds>(sc, fr, ubo, tc.x, tc.y, nocache, stack + threadIdx.x, &ps, &o, &rays);
+            valid += o.valid ? 1u : 0u;
+        }
+        hand_on(alive, bd.tail, bd.tail_count, tc.y * fr.width + tc.x, v, ps, o, &lds_count, &lds_base);
     }
-    hand_on(alive, bd.tail, bd.tail_count, tc.y * fr.width + tc.x, v, ps, o, &lds_count, &lds_base);
     count_rays(&bd.counters[0], rays, &lds_count);
     count_rays(&bd.counters[2], valid, &lds_count);
 }
@@ -1206,61 +1237,92 @@ __global__ __launch_bounds__(256) void k_batch_temporal(BatchDev bd, PassParams 
 
 // With at most 20 LDS stack levels (20 KiB) 7 workgroups fit a CU: ask for 7
 // waves, as the single-view spatial kernel does.
-template <int kTw, int kTh, int kLds>
+template <int kTw, int kTh, int kLds, bool kPersistent>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLds <= 20 ? 7 : 6))) void k_batch_spatial(
-    SceneDev sc, BatchDev bd, PassParams pp) {
+    SceneDev sc, BatchDev bd, PassParams pp, uint32_t tiles) {
     __shared__ int stack[kLds * kBlock];
     __shared__ uint32_t lds_count;
-    uint32_t local;
-    const uint32_t v = batch_view(bd, blockIdx.x, kTw, kTh, &local);
-    const FrameDev fr = bd.views[v];
-    const UniformBufferObject& ubo = bd.ubos[v];
-    const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
-    uint32_t rays = 0;
-    if (tc.in_image)
-        rays = spatial_pixel<kLds>(sc, fr, ubo.currentFrame, ubo.reuse, pp, tc.x, tc.y, stack + threadIdx.x);
+    __shared__ uint32_t lds_tile;
+    uint32_t rays = 0, b = 0;
+    for (uint32_t it = 0; batch_tile<kPersistent>(bd.tile_queue + kXcds, tiles, it, &lds_tile, &b); ++it) {
+        uint32_t local;
+        const uint32_t v = batch_view(bd, b, kTw, kTh, &local);
+        const FrameDev fr = bd.views[v];
+        const UniformBufferObject& ubo = bd.ubos[v];
+        const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
+        if (tc.in_image)
+            rays += spatial_pixel<kLds>(sc, fr, ubo.currentFrame, ubo.reuse, pp, tc.x, tc.y, stack + threadIdx.x);
+    }
     count_rays(&bd.counters[1], rays, &lds_count);
 }
 
-// Tile shapes x LDS stack depths the autotuner chooses from.
+// Tile shapes x LDS stack depths x launch forms the autotuner chooses from.
 struct BatchKernels {
     uint32_t tw, th, lds;
-    void (*initial)(SceneDev, BatchDev);
+    bool persistent;
+    void (*initial)(SceneDev, BatchDev, uint32_t);
     void (*tail)(SceneDev, BatchDev);
-    void (*spatial)(SceneDev, BatchDev, PassParams);
+    void (*spatial)(SceneDev, BatchDev, PassParams, uint32_t);
 };
-#define XR_BATCH_CFG(TW, TH, L) \
-    {TW, TH, L, k_batch_initial<TW, TH, L>, k_batch_tail<L>, k_batch_spatial<TW, TH, L>}
+#define XR_BATCH_CFG(TW, TH, L, P) \
+    {TW, TH, L, P, k_batch_initial<TW, TH, L, P>, k_batch_tail<L>, k_batch_spatial<TW, TH, L, P>}
 static const BatchKernels kBatchKernels[] = {
-    XR_BATCH_CFG(32, 8, 20),  XR_BATCH_CFG(32, 8, 16),  XR_BATCH_CFG(32, 8, 24),
-    XR_BATCH_CFG(16, 16, 20), XR_BATCH_CFG(16, 16, 16), XR_BATCH_CFG(16, 16, 24),
-    XR_BATCH_CFG(8, 32, 20),  XR_BATCH_CFG(8, 32, 16),  XR_BATCH_CFG(8, 32, 24),
+    XR_BATCH_CFG(32, 8, 20, false),  XR_BATCH_CFG(32, 8, 16, false),  XR_BATCH_CFG(32, 8, 24, false),
+    XR_BATCH_CFG(16, 16, 20, false), XR_BATCH_CFG(16, 16, 16, false), XR_BATCH_CFG(16, 16, 24, false),
+    XR_BATCH_CFG(8, 32, 20, false),  XR_BATCH_CFG(8, 32, 16, false),  XR_BATCH_CFG(8, 32, 24, false),
+    XR_BATCH_CFG(32, 8, 20, true),   XR_BATCH_CFG(32, 8, 16, true),   XR_BATCH_CFG(32, 8, 24, true),
+    XR_BATCH_CFG(16, 16, 20, true),  XR_BATCH_CFG(16, 16, 16, true),  XR_BATCH_CFG(16, 16, 24, true),
+    XR_BATCH_CFG(8, 32, 20, true),   XR_BATCH_CFG(8, 32, 16, true),   XR_BATCH_CFG(8, 32, 24, true),
 };
 #undef XR_BATCH_CFG
+constexpr uint32_t kBatchConfigs = sizeof(kBatchKernels) / sizeof(kBatchKernels[0]);
 
-uint32_t batch_config_count() { return (uint32_t)(sizeof(kBatchKernels) / sizeof(kBatchKernels[0])); }
+// Workgroups of kernel k resident on the whole GPU at once (a multiple of 8),
+// found once per configuration and kernel.
+uint32_t resident_grid(uint32_t cfg, int which) {
+    static std::mutex mu;
+    static uint32_t cached[kBatchConfigs][2] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!cached[cfg][which]) {
+        int dev = 0, cus = 0, per_cu = 0;
+        const void* fn = which == 0 ? (const void*)kBatchKernels[cfg].initial : (const void*)kBatchKernels[cfg].spatial;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1, cus = cus > 0 ? cus : 256;
+        const uint32_t g = (uint32_t)(cus * per_cu) / kXcds * kXcds;
+        cached[cfg][which] = g >= kXcds ? g : kXcds;
+    }
+    return cached[cfg][which];
+}
 
-void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds) {
+uint32_t batch_config_count() { return kBatchConfigs; }
+
+void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds, bool* persistent) {
     *tw = kBatchKernels[cfg].tw;
     *th = kBatchKernels[cfg].th;
     *lds = kBatchKernels[cfg].lds;
+    *persistent = kBatchKernels[cfg].persistent;
 }
 
 // One ReSTIR GI frame of every view (initial head + tail, temporal, spatial).
-// views: host copy of the view table (sizes), bd: its device twin.
+// views: host copy of the view table (sizes), bd: its device twin.  One
+// launch clears the tail count and the persistent kernels' tile queues.
 hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
                               const PassParams& pp, hipStream_t st) {
     const BatchKernels& k = kBatchKernels[cfg];
-    uint32_t blocks = 0, pixels = 0;
+    uint32_t tiles = 0, pixels = 0;
     for (uint32_t v = 0; v < bd.nviews; ++v) {
-        blocks += image_grid(views[v].width, views[v].height, k.tw, k.th);
+        tiles += image_grid(views[v].width, views[v].height, k.tw, k.th);
         pixels += views[v].n;
     }
-    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, bd.tail_count, 1u);
-    hipLaunchKernelGGL(k.initial, dim3(blocks), dim3(kBlock), 0, st, sc, bd);
+    const uint32_t gi = k.persistent ? std::min(tiles, resident_grid(cfg, 0)) : tiles;
+    const uint32_t gs = k.persistent ? std::min(tiles, resident_grid(cfg, 1)) : tiles;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, bd.tail_count, kBatchCountWords);
+    hipLaunchKernelGGL(k.initial, dim3(gi), dim3(kBlock), 0, st, sc, bd, tiles);
     hipLaunchKernelGGL(k.tail, dim3(2048), dim3(kBlock), 0, st, sc, bd);
     hipLaunchKernelGGL(k_batch_temporal, dim3((pixels + 255) / 256), dim3(256), 0, st, bd, pp);
-    hipLaunchKernelGGL(k.spatial, dim3(blocks), dim3(kBlock), 0, st, sc, bd, pp);
+    hipLaunchKernelGGL(k.spatial, dim3(gs), dim3(kBlock), 0, st, sc, bd, pp, tiles);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -139,7 +140,7 @@ XR_API_SYMBOLS = [
     "xr_mlp_reset", "xr_mlp_eval", "xr_mlp_gradient", "xr_mlp_gradient_noalloc", "xr_mlp_train_rows", "xr_mlp_set_weight_format",
     "xr_mlp_fp8_weights", "xr_mlp_params",
     "xr_batch_create", "xr_batch_render", "xr_batch_frame_time", "xr_batch_set_graph", "xr_batch_config_count",
-    "xr_batch_config_info", "xr_batch_set_config", "xr_batch_config", "xr_batch_autotune", "xr_batch_reset",
+    "xr_batch_config_info", "xr_batch_config_persistent", "xr_batch_set_config", "xr_batch_config", "xr_batch_autotune", "xr_batch_reset",
     "xr_batch_counters", "xr_batch_reset_counters", "xr_batch_buffer_bytes", "xr_batch_read_buffer",
     "xr_batch_destroy",
     "xr_last_error", "xr_global_last_error", "xr_set_textures",
@@ -217,6 +218,7 @@ def lib() -> C.CDLL:
         L.xr_batch_config_count.argtypes = []
         L.xr_batch_config_count.restype = U32
         L.xr_batch_config_info.argtypes = [U32, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]
+        L.xr_batch_config_persistent.argtypes = [U32, C.POINTER(C.c_int)]
         L.xr_batch_set_config.argtypes = [P, U32]
         L.xr_batch_config.argtypes = [P]
         L.xr_batch_config.restype = U32
@@ -516,6 +518,7 @@ class Context:
         if rc:
             raise XrError(rc, lib().xr_global_last_error().decode())
         self._h = h
+        self._batches = weakref.WeakSet()  # closed before the context: a batch must not outlive it
         self.width, self.height = width, height
         tex = scene.textures()
         if tex:
@@ -700,6 +703,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None) is not None:
+            for b in list(getattr(self, "_batches", ())):
+                b.close()
             lib().xr_destroy(self._h)
             self._h = None
 
@@ -778,6 +783,7 @@ class Batch:
         h = C.c_void_p()
         ctx.check(lib().xr_batch_create(ctx.handle, len(self.sizes), ws.ctypes.data, hs.ctypes.data, C.byref(h)))
         self._h = h
+        ctx._batches.add(self)
 
     @staticmethod
     def config_count() -> int:
@@ -789,6 +795,13 @@ class Batch:
         if lib().xr_batch_config_info(k, C.byref(a), C.byref(b), C.byref(c)):
             raise XrError(XR_ERR_INVALID_ARGUMENT, lib().xr_global_last_error().decode())
         return int(a.value), int(b.value), int(c.value)
+
+    @staticmethod
+    def config_persistent(k: int) -> bool:
+        p = C.c_int()
+        if lib().xr_batch_config_persistent(k, C.byref(p)):
+            raise XrError(XR_ERR_INVALID_ARGUMENT, lib().xr_global_last_error().decode())
+        return p.value == 1
 
     def _ubos(self, ubos):
         assert len(ubos) == len(self.sizes)
