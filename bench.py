@@ -433,7 +433,28 @@ def run_c3(args) -> dict:
                        "pixels": pixels, "triangles": scene.triangle_count, "max_depth": BOUNCES,
                        "frames": frames, "graph": True, "parallelism": "single_gpu"})
     res["steps"], res["warmup"] = timed, warm
+    # the same views and frames with the BF16 radiance cache: each view chooses
+    # its own training samples, one training step covers all of them
+    batch.reset()
+    cparams = N.default_params(visibility=N.XR_VIS_TRACE, flags=N.XR_FLAG_MLP_CACHE, **PARAMS)
+    ctx.mlp_reset(1)
+    for f in range(warm):
+        batch.render(all_ubos[f], cparams)
+    ctx.sync()
+    batch.reset_counters()
+    t0 = time.perf_counter()
+    for f in range(warm, frames):
+        batch.render(all_ubos[f], cparams)
+    ctx.sync()
+    cwall = time.perf_counter() - t0
+    cext, cshadow, _, cq, ct = batch.counters()
+    cms = cwall * 1e3 / timed
+    log(f"c3 with the BF16 cache: {cms:.3f} ms/frame, {ct // timed} training samples and {cq // timed} queries "
+        f"per frame")
     res.update({"ms_per_frame": round(ms, 4), "ns_per_pixel": round(ms * 1e6 / pixels, 3),
+                "mlp_cache_bf16": {"ms_per_frame": round(cms, 4), "mrays_per_s": round((cext + cshadow) / cwall / 1e6, 1),
+                                   "train_samples_per_frame": ct // timed, "queries_per_frame": cq // timed,
+                                   "train_target_per_view": cparams.mlp_train_pixels},
                 "rays_per_frame": (ext + shadow) // timed,
                 "autotune": {"ms_per_config": [round(x, 4) for x in tune],
                              "configs": [list(N.Batch.config_info(k)) + [N.Batch.config_persistent(k)]

@@ -427,14 +427,22 @@ typedef struct xr_batch xr_batch;
 int xr_batch_create(xr_ctx* ctx, uint32_t view_count, const uint32_t* widths, const uint32_t* heights,
                     xr_batch** out);
 /* One ReSTIR GI frame of every view; ubos[view_count].  params as for
- * xr_render (no radiance cache, every pass).  Asynchronous. */
+ * xr_render (every pass).  With XR_FLAG_MLP_CACHE each view chooses its own
+ * training samples exactly as xr_render would for it alone (up to
+ * mlp_train_pixels, rotated by its UBO's frame number), then one training
+ * step of the context's network covers the samples of all views (the mean
+ * loss over them), then every view's query and training pixels take the
+ * updated network's radiance.  With the graph enabled the initial-sample
+ * pass and the temporal + spatial passes replay as two graphs and the
+ * training step runs between them.  Asynchronous. */
 int xr_batch_render(xr_batch* batch, const xr_ubo* ubos, const xr_restir_params* params);
 /* Milliseconds of the last xr_batch_render (events around it).  Waits. */
 int xr_batch_frame_time(xr_batch* batch, float* ms);
 /* enable != 0 (default): replay a captured HIP graph; 0: launch eagerly.
  * The graph is captured again on the next xr_batch_render whenever the
- * parameters or the kernel configuration change, or xr_set_textures /
- * xr_build_accel replaced the scene's device memory since the capture. */
+ * parameters (cache settings included) or the kernel configuration change,
+ * or xr_set_textures / xr_build_accel replaced the scene's device memory
+ * since the capture. */
 int xr_batch_set_graph(xr_batch* batch, int enable);
 /* Kernel configurations: tile shape, LDS traversal-stack depth and launch
  * form.  A grid configuration launches one workgroup per tile; a persistent

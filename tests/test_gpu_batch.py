@@ -124,7 +124,7 @@ def test_batch_rejects_bad_input(scene):
         N.Batch(ctx, [(0, 16)])
     batch = N.Batch(ctx, SIZES)
     p = N.default_params(**PARAMS)
-    p.flags = N.XR_FLAG_MLP_CACHE
+    p.passes = 1  # batched views run every pass
     with pytest.raises(N.XrError):
         batch.render(ubo_sequence(1)[0], p)
     with pytest.raises(N.XrError):

@@ -4,7 +4,9 @@ pixel subsets (needs an MI355X).
   * 12 batched views of uneven size (4x 256x256, 4x 960x540, 2x 1920x1080,
     2x 1280x720) of the 2.75 M-triangle terrain, each on its own seeded
     camera path, one launch per pass for all views, replayed as a HIP graph
-    with the autotuner's kernel configuration;
+    with the autotuner's kernel configuration, and again with the BF16
+    radiance cache (each view choosing its own training samples, one training
+    step on all of them);
   * 1920x1080 ReSTIR GI with the BF16 radiance cache (65,536 training
     samples per frame) on the same terrain;
   * 3840x2160 ReSTIR GI at depth 6 with FP8 cache weights on the
@@ -25,6 +27,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from oracle import binding as B
 from tests.helpers import bytes_equal_rows, describe_mismatch, oracle_scene, say
 from xvk_ray_amd import native as N
 
@@ -103,18 +106,57 @@ def test_c3_batched_views_match_oracle(terrain):
     ctx.close()
 
 
+def test_c3_batched_views_with_cache_match_oracle(terrain):
+    """The 12 views of the batched workload with the BF16 radiance cache: each
+    view chooses its own 65,536 training samples (or all its candidates) as a
+    single-view frame would, one training step covers all views, and each
+    view's subset matches the oracle (cache kinds and traced fields bit-exact,
+    cached radiance from the updated network, temporal and spatial passes)."""
+    scene, orc = terrain
+    frames = 2
+    ctx = N.Context(scene, 8, 8)
+    cams = [N.Camera.orbit(2 + v, 8, 64) for v in range(len(C3_SIZES))]
+    views = [[c.view(f) for f in range(frames)] for c in cams]
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, **PARAMS)
+    ubos = [[N.make_ubo(views[v][f], views[v][max(f - 1, 0)], 45.0, w, h, bounces=4, frame=f + 1)
+             for v, (w, h) in enumerate(C3_SIZES)] for f in range(frames)]
+    batch = N.Batch(ctx, C3_SIZES)
+    rng = np.random.default_rng(29)
+    for f in range(frames):
+        frame = f + 1
+        batch.reset_counters()
+        batch.render(ubos[f], params)
+        ctx.sync()
+        w_after = ctx.mlp_params()
+        total = 0
+        for v, (w, h) in enumerate(C3_SIZES):
+            px = np.sort(rng.choice(w * h, size=min(2000, w * h), replace=False)).astype(np.uint32)
+            say(f"[configs] C3 + cache frame {frame} view {v} ({w}x{h})")
+            read = lambda which, v=v: batch.read(v, which)  # noqa: E731
+            nt, _ = check_cache_frame(read, w_after, orc, scene, w, h, ubos[f][v], px, params, fp8=False)
+            assert 0 < nt <= params.mlp_train_pixels
+            if w * h >= 1280 * 720:  # more candidates than the target
+                assert nt == params.mlp_train_pixels
+            _passes_after(read, orc, w, h, frame, px, params)
+            total += nt
+        say(f"[configs] C3 + cache frame {frame}: {total} training samples over {len(C3_SIZES)} views")
+        assert batch.counters()[4] == total
+    batch.close()
+    ctx.close()
+
+
 def _features(samples, mats, lo, inv, idx):
     from tests.test_gpu_mlp import _features as f
     return f(samples, mats, lo, inv, idx)
 
 
-def check_cache_frame(ctx, orc, scene, W, H, ubo, frame, px, params, fp8):
-    """One radiance-cache frame on the pixels px: every traced field and the
-    cache kind bit-exact; query and training pixels carry the updated
-    network's prediction (within 1e-2 of the output scale); the exact number
-    of training samples; temporal and spatial passes bit-exact."""
-    from oracle import binding as B
-    samples = ctx.read(N.XR_BUF_SAMPLES).copy()
+def check_cache_frame(read, w_after, orc, scene, W, H, ubo, px, params, fp8):
+    """One radiance-cache frame of one view on the pixels px (`read(which)`
+    returns the view's buffer): every traced field and the cache kind
+    bit-exact; query and training pixels carry the updated network's
+    prediction (within 1e-2 of the output scale).  Returns the view's whole
+    training-set size as the oracle chooses it and the cached pixels in px."""
+    samples = read(N.XR_BUF_SAMPLES).copy()
     ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H, pixels=px, cache=(params.mlp_train_pixels,
                                                                            params.mlp_spread))
     rs = ref["samples"]
@@ -126,10 +168,7 @@ def check_cache_frame(ctx, orc, scene, W, H, ubo, frame, px, params, fp8):
     g["l_out"][cached] = 0
     r["l_out"][cached] = 0
     _eq("cache-frame samples", g, r)
-    counts = ctx.counters_all()
-    assert counts[4] == train_all.sum()  # the frame's whole training set, as the oracle chose it
     # the network's radiance at query / training pixels (a bounded sample of them)
-    w_after = ctx.mlp_params()
     verts, _, mats, _ = scene.arrays()
     lo = verts["pos"].min(0).astype(np.float32)
     inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
@@ -138,11 +177,10 @@ def check_cache_frame(ctx, orc, scene, W, H, ubo, frame, px, params, fp8):
     got = samples["l_out"][idx].astype(np.float64)
     scale = max(1.0, np.abs(pred).max())
     assert np.abs(got - pred).max() <= 1e-2 * scale, float(np.abs(got - pred).max())
-    return int(counts[4]), int(cached.sum())
+    return int(train_all.sum()), int(cached.sum())
 
 
-def _passes_after(ctx, orc, W, H, frame, px, params):
-    read = ctx.read
+def _passes_after(read, orc, W, H, frame, px, params):
     from oracle.binding import temporal as orc_temporal
     n = W * H
     samples = read(N.XR_BUF_SAMPLES).copy()
@@ -186,10 +224,11 @@ def test_c4_cache_1080p_matches_oracle(terrain):
         ctx.sync()
         if k >= 2:
             say(f"[configs] C4 frame {k + 1}")
-            nt, nc = check_cache_frame(ctx, orc, scene, W, H, ubo, k + 1, px, params, fp8=False)
+            nt, nc = check_cache_frame(ctx.read, ctx.mlp_params(), orc, scene, W, H, ubo, px, params, fp8=False)
+            assert ctx.counters_all()[4] == nt  # the frame's whole training set, as the oracle chose it
             say(f"[configs] C4 frame {k + 1}: {nt} training samples, {nc} cached pixels in the subset")
             assert nt == 65536
-            _passes_after(ctx, orc, W, H, k + 1, px, params)
+            _passes_after(ctx.read, orc, W, H, k + 1, px, params)
 
 
 def test_c5_4k_fp8_depth6_matches_oracle():
@@ -213,7 +252,8 @@ def test_c5_4k_fp8_depth6_matches_oracle():
         ctx.render(ubo, N.XR_MODE_RESTIR, params)
         ctx.sync()
         say(f"[configs] C5 frame {k + 1}")
-        nt, nc = check_cache_frame(ctx, orc, scene, W, H, ubo, k + 1, px, params, fp8=True)
+        nt, nc = check_cache_frame(ctx.read, ctx.mlp_params(), orc, scene, W, H, ubo, px, params, fp8=True)
+        assert ctx.counters_all()[4] == nt
         say(f"[configs] C5 frame {k + 1}: {nt} training samples, {nc} cached pixels in the subset")
         assert nt == 65536
-        _passes_after(ctx, orc, W, H, k + 1, px, params)
+        _passes_after(ctx.read, orc, W, H, k + 1, px, params)

@@ -270,6 +270,8 @@ FrameDev frame_dev(const xr_ctx* c) {
     f.cache_blocks = c->d_cache_work ? c->d_cache_work + f.n : nullptr;
     f.tail = c->d_tail;
     f.tail_count = c->d_tail_count;
+    f.pending = c->d_tail ? c->d_tail + 4 * (size_t)f.n : nullptr;
+    f.pending_count = c->d_tail_count ? c->d_tail_count + kPendingCount : nullptr;
     return f;
 }
 
@@ -1216,6 +1218,9 @@ struct xr_batch {
         float4 *samples = nullptr, *temporal = nullptr, *spatial = nullptr, *gpos = nullptr, *gnrm = nullptr,
                *gmatuv = nullptr, *nbr = nullptr;
         uint32_t *motion = nullptr, *output = nullptr;
+        // radiance cache (allocated by the first frame that uses it)
+        uint32_t *cache_list = nullptr, *cache_work = nullptr;
+        float4* pending = nullptr;
     };
     std::vector<View> views;
     std::vector<FrameDev> frames;  // host copy of d_frames
@@ -1224,13 +1229,23 @@ struct xr_batch {
     float4* d_tail = nullptr;
     uint32_t* d_tail_count = nullptr;
     unsigned long long* d_counters = nullptr;
+    uint32_t* d_pending_counts = nullptr;  // per view (radiance cache)
+    bool cache_ready = false;
     uint32_t cfg = 0;
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    PassParams graph_pp{};
-    uint32_t graph_cfg = 0;
-    uint64_t graph_scene_gen = ~0ull;  // the context's scene generation the graph was captured with
+    // Captured graphs: [0] the whole frame; with the radiance cache [1] the
+    // initial-sample pass and [2] the temporal and spatial passes, the
+    // training step running eagerly between them (its Adam bias correction
+    // changes every step).
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        PassParams pp{};
+        CacheCfg cache{};
+        uint32_t cfg = 0;
+        uint64_t scene_gen = ~0ull;  // the context's scene generation at the capture
+    };
+    Graph graphs[3];
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -1247,18 +1262,23 @@ void batch_free(xr_batch* b) {
         dev_free(&v.motion);
         dev_free(&v.output);
         dev_free(&v.nbr);
+        dev_free(&v.cache_list);
+        dev_free(&v.cache_work);
+        dev_free(&v.pending);
     }
+    dev_free(&b->d_pending_counts);
     dev_free(&b->d_frames);
     dev_free(&b->d_ubos);
     dev_free(&b->d_tail);
     dev_free(&b->d_tail_count);
     dev_free(&b->d_counters);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto& g : b->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        g = xr_batch::Graph();
+    }
     if (b->e0) hipEventDestroy(b->e0);
     if (b->e1) hipEventDestroy(b->e1);
-    b->exec = nullptr;
-    b->graph = nullptr;
 }
 
 // Zeroes every view's buffers (history included).
@@ -1306,36 +1326,106 @@ PassParams pass_params(const xr_restir_params& p) {
 
 bool same_pp(const PassParams& a, const PassParams& b) { return memcmp(&a, &b, sizeof(a)) == 0; }
 
-// One frame of every view: eager launches, or the captured graph (captured
-// again when the parameters, the kernel configuration or the scene's device
-// memory -- textures, BVH -- changed since the capture).
-int batch_frame(xr_batch* b, const PassParams& pp) {
+// Per-view radiance-cache buffers (work lists, training choice, pending
+// records), allocated by the first frame that uses the cache.
+int ensure_batch_cache(xr_batch* b) {
+    if (b->cache_ready) return XR_OK;
+    xr_ctx* c = b->ctx;
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    XR_HIP(c, dev_alloc(&b->d_pending_counts, sizeof(uint32_t) * b->nviews));
+    for (uint32_t v = 0; v < b->nviews; ++v) {
+        xr_batch::View& vw = b->views[v];
+        const size_t n = (size_t)vw.w * vw.h;
+        XR_HIP(c, dev_alloc(&vw.cache_list, (2 + 2 * n) * sizeof(uint32_t)));
+        XR_HIP(c, dev_alloc(&vw.cache_work, (n + (n + kCacheSelBlock - 1) / kCacheSelBlock + 1) * sizeof(uint32_t)));
+        XR_HIP(c, dev_alloc(&vw.pending, 4 * n * sizeof(float4)));
+        FrameDev& f = b->frames[v];
+        f.cache_list = vw.cache_list;
+        f.cache_slot = vw.cache_work;
+        f.cache_blocks = vw.cache_work + n;
+        f.pending = vw.pending;
+        f.pending_count = b->d_pending_counts + v;
+    }
+    XR_HIP(c, hipMemcpyAsync(b->d_frames, b->frames.data(), sizeof(FrameDev) * b->nviews, hipMemcpyHostToDevice,
+                             c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& g : b->graphs) {  // their launches took the view table by value
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        g = xr_batch::Graph();
+    }
+    b->cache_ready = true;
+    return XR_OK;
+}
+
+// Replays graph slot g, capturing it again first when the parameters, the
+// cache settings, the kernel configuration or the scene's device memory
+// (textures, BVH) changed since its capture.
+template <class Launch>
+int batch_graph(xr_batch* b, xr_batch::Graph& g, const PassParams& pp, const CacheCfg& cache, Launch launch) {
+    xr_ctx* c = b->ctx;
+    const bool same = g.exec && same_pp(pp, g.pp) && g.cfg == b->cfg && g.scene_gen == c->scene_gen &&
+                      g.cache.enabled == cache.enabled && g.cache.train_target == cache.train_target &&
+                      g.cache.spread_c == cache.spread_c;
+    if (!same) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        g = xr_batch::Graph();
+        XR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch();
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        XR_HIP(c, le);
+        XR_HIP(c, ce);
+        g.graph = graph;
+        XR_HIP(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        g.pp = pp;
+        g.cache = cache;
+        g.cfg = b->cfg;
+        g.scene_gen = c->scene_gen;
+    }
+    XR_HIP(c, hipGraphLaunch(g.exec, c->stream));
+    return XR_OK;
+}
+
+// One frame of every view: eager launches or captured graphs.  With the
+// radiance cache the initial-sample pass (each view choosing its own training
+// samples, as a single-view frame would) is followed by one training step on
+// the samples of all views (the mean loss over their union), then the
+// network's radiance for every view's query and training pixels, then the
+// temporal and spatial passes.
+int batch_frame(xr_batch* b, const PassParams& pp, const CacheCfg& cache, float loss_scale, float lr) {
     xr_ctx* c = b->ctx;
     const SceneDev sc = scene_dev(c);
     const BatchDev bd = batch_dev(b);
+    const FrameDev* views = b->frames.data();
+    hipStream_t st = c->stream;
+    if (!cache.enabled) {
+        if (!b->use_graph) {
+            XR_HIP(c, launch_batch_frame(sc, bd, views, b->cfg, pp, st));
+            return XR_OK;
+        }
+        return batch_graph(b, b->graphs[0], pp, cache, [&] { return launch_batch_frame(sc, bd, views, b->cfg, pp, st); });
+    }
     if (!b->use_graph) {
-        XR_HIP(c, launch_batch_frame(sc, bd, b->frames.data(), b->cfg, pp, c->stream));
+        XR_HIP(c, launch_batch_initial(sc, bd, views, b->cfg, cache, st));
+    } else {
+        const int rc = batch_graph(b, b->graphs[1], pp, cache,
+                                   [&] { return launch_batch_initial(sc, bd, views, b->cfg, cache, st); });
+        if (rc) return rc;
+    }
+    uint32_t* total = b->d_tail_count + kBatchTrainTotal;
+    XR_HIP(c, launch_batch_train_total(bd, total, st));
+    for (uint32_t v = 0; v < b->nviews; ++v)
+        XR_HIP(c, launch_mlp_train_frame(c->mlp, views[v], c->d_materials, loss_scale, st, total));
+    c->mlp.steps++;
+    XR_HIP(c, launch_mlp_adam(c->mlp, lr, 1.0f / loss_scale, true, st));
+    for (uint32_t v = 0; v < b->nviews; ++v) XR_HIP(c, launch_mlp_infer(c->mlp, views[v], c->d_materials, st));
+    if (!b->use_graph) {
+        XR_HIP(c, launch_batch_reuse(sc, bd, views, b->cfg, pp, st));
         return XR_OK;
     }
-    if (!b->exec || !same_pp(pp, b->graph_pp) || b->graph_cfg != b->cfg || b->graph_scene_gen != c->scene_gen) {
-        if (b->exec) hipGraphExecDestroy(b->exec);
-        if (b->graph) hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-        XR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t le = launch_batch_frame(sc, bd, b->frames.data(), b->cfg, pp, c->stream);
-        hipGraph_t g = nullptr;
-        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
-        XR_HIP(c, le);
-        XR_HIP(c, ce);
-        b->graph = g;
-        XR_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        b->graph_pp = pp;
-        b->graph_cfg = b->cfg;
-        b->graph_scene_gen = c->scene_gen;
-    }
-    XR_HIP(c, hipGraphLaunch(b->exec, c->stream));
-    return XR_OK;
+    return batch_graph(b, b->graphs[2], pp, cache, [&] { return launch_batch_reuse(sc, bd, views, b->cfg, pp, st); });
 }
 
 }  // namespace
@@ -1390,7 +1480,7 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
     XR_BATCH_HIP(dev_alloc(&b->d_frames, sizeof(FrameDev) * view_count));
     XR_BATCH_HIP(dev_alloc(&b->d_ubos, sizeof(UniformBufferObject) * view_count));
     XR_BATCH_HIP(dev_alloc(&b->d_tail, 4 * total * f4));
-    XR_BATCH_HIP(dev_alloc(&b->d_tail_count, kBatchCountWords * sizeof(uint32_t)));
+    XR_BATCH_HIP(dev_alloc(&b->d_tail_count, kBatchWords * sizeof(uint32_t)));
     XR_BATCH_HIP(dev_alloc(&b->d_counters, sizeof(unsigned long long) * kCounters));
     XR_BATCH_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * kCounters, c->stream));
     XR_BATCH_HIP(hipEventCreate(&b->e0));
@@ -1416,6 +1506,8 @@ int xr_batch_create(xr_ctx* c, uint32_t view_count, const uint32_t* widths, cons
         f.cache_list = nullptr;
         f.cache_slot = nullptr;
         f.cache_blocks = nullptr;
+        f.pending = nullptr;
+        f.pending_count = nullptr;
         f.tail = b->d_tail;
         f.tail_count = b->d_tail_count;
         b->frames[v] = f;
@@ -1441,15 +1533,28 @@ int xr_batch_render(xr_batch* b, const xr_ubo* ubos, const xr_restir_params* par
     else xr_default_params(&p);
     int rc = check_params(c, &p);
     if (rc) return rc;
-    if (p.flags & (XR_FLAG_MLP_CACHE | XR_FLAG_MLP_FP8))
-        return set_err(c, XR_ERR_UNSUPPORTED, "batched views do not use the radiance cache");
     if (p.passes && p.passes != 7u) return set_err(c, XR_ERR_UNSUPPORTED, "batched views run every pass");
     if (!c->built) return set_err(c, XR_ERR_STATE, "acceleration structure not built");
+    CacheCfg cache = {0u, 0u, 0u, 0.0f};
+    if (p.flags & XR_FLAG_MLP_CACHE) {
+        rc = ensure_mlp(c);
+        if (rc) return rc;
+        const bool fp8 = (p.flags & XR_FLAG_MLP_FP8) != 0;
+        if (c->mlp.fp8 != fp8) {  // rebuild the GEMM copies in the requested format
+            c->mlp.fp8 = fp8;
+            XR_HIP(c, launch_mlp_adam(c->mlp, 0.0f, 1.0f, false, c->stream));
+        }
+        rc = ensure_batch_cache(b);
+        if (rc) return rc;
+        cache.enabled = 1u;
+        cache.train_target = p.mlp_train_pixels;
+        cache.spread_c = p.mlp_spread;  // the frame number comes from each view's UBO
+    }
     // pageable source: the copy is staged before the call returns
     XR_HIP(c, hipMemcpyAsync(b->d_ubos, ubos, sizeof(UniformBufferObject) * b->nviews, hipMemcpyHostToDevice,
                              c->stream));
     XR_HIP(c, hipEventRecord(b->e0, c->stream));
-    rc = batch_frame(b, pass_params(p));
+    rc = batch_frame(b, pass_params(p), cache, p.mlp_loss_scale, p.mlp_lr);
     if (rc) return rc;
     XR_HIP(c, hipEventRecord(b->e1, c->stream));
     return XR_OK;

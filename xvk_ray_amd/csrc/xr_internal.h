@@ -50,10 +50,15 @@ struct FrameDev {
     // candidates before it (exclusive scan; entry nblocks = the total).
     uint32_t* cache_slot;
     uint32_t* cache_blocks;
+    // The candidates' pending path records (cache mode: 4 float4 each, n
+    // capacity) and their count.  The single-view pass keeps them in its
+    // second tail queue (tail + 4n, count tail_count[kPendingCount]); each
+    // batched view has its own.
+    float4* pending;
+    uint32_t* pending_count;
     // Paths alive after the head bounces of the initial-sample pass (4 float4
     // each, n capacity) and their count; the single-view pass keeps a second
-    // queue of n paths at tail + 4n, which in cache mode first holds the
-    // candidates' pending records (count tail_count[kPendingCount]).  Tail
+    // queue of n paths at tail + 4n.  Tail
     // launch j (bounce kHeadBounces + j) reads queue j % 2 with count
     // tail_count[j % 3], appends to the other queue with count
     // tail_count[(j + 1) % 3] and clears tail_count[(j + 2) % 3], the next

@@ -24,10 +24,23 @@ struct BatchDev {
     unsigned long long* counters;
 };
 constexpr uint32_t kBatchCountWords = 17;
+// The batch's per-frame words after the tail count and tile queues: the
+// training-sample total of the views (cache mode).
+constexpr uint32_t kBatchTrainTotal = kBatchCountWords;
+constexpr uint32_t kBatchWords = kBatchCountWords + 1;
 uint32_t batch_config_count();
 void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds, bool* persistent);
 hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
                               const PassParams& pp, hipStream_t st);
+// The two halves of a batched frame, for the radiance cache's training step
+// between them: the initial-sample pass of every view (with each view's
+// training choice when cache.enabled), then the temporal and spatial passes.
+hipError_t launch_batch_initial(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                                const CacheCfg& cache, hipStream_t st);
+hipError_t launch_batch_reuse(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                              const PassParams& pp, hipStream_t st);
+// *total = the views' training samples of this frame (cache mode).
+hipError_t launch_batch_train_total(const BatchDev& bd, uint32_t* total, hipStream_t st);
 // Neighbour records of the spatial pass from the samples and the temporal
 // reservoirs of frame's slot (when the temporal pass did not write them).
 hipError_t launch_nbr_build(const FrameDev& fr, uint32_t frame, hipStream_t st);
@@ -69,8 +82,11 @@ hipError_t launch_mlp_adam(const MlpState& st, float lr, float inv_scale, bool s
 hipError_t launch_clear_u32(uint32_t* p, uint32_t n, hipStream_t s);
 hipError_t launch_mlp_infer(const MlpState& st, const FrameDev& fr, const float4* materials, hipStream_t s);
 hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows, float* out, hipStream_t s);
+// Gradient of the frame's training samples, added into the partial copies;
+// total (device, may be null): the loss divides by *total samples instead of
+// the frame's own count (a batch's views trained as one step).
 hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, float loss_scale,
-                                  hipStream_t s);
+                                  hipStream_t s, const uint32_t* total = nullptr);
 hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const float* targets, uint32_t rows,
                                  float loss_scale, hipStream_t s);
 // Gradient of caller-given device rows with the partial copies in the
@@ -90,6 +106,11 @@ struct CacheCfg {
 };
 // Pixels per workgroup of the training-choice kernels (256 threads x 16).
 constexpr uint32_t kCacheSelBlock = 4096;
+// The frame's training choice among its pending candidates (after the head
+// kernel); frame_ubo, when not null, is the device UBO holding the frame
+// number (else cache.frame).
+hipError_t launch_cache_choice(const FrameDev& fr, const CacheCfg& cache, const UniformBufferObject* frame_ubo,
+                               hipStream_t st);
 
 // Output of the GPU builder.  All arrays are device allocations owned by the
 // caller after a successful build (free with hipFree).

@@ -446,10 +446,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
     hand_on(alive && !pending, fr.tail, fr.tail_count, pix, 0u, ps, o, &lds_count, &lds_base);
     if (cache.enabled) {  // uniform over the launch
         // candidates park their record (alive or not) in the second queue
-        const uint32_t q = block_slot(pending, fr.tail_count + kPendingCount, &lds_count, &lds_base);
+        const uint32_t q = block_slot(pending, fr.pending_count, &lds_count, &lds_base);
         if (pending) {
             o.cache = kind | (alive ? kRecordAlive : 0u);
-            store_path(fr.tail + 4 * (size_t)fr.n + 4 * (size_t)q, pix, 0u, ps, o);
+            store_path(fr.pending + 4 * (size_t)q, pix, 0u, ps, o);
         }
         if (tc.in_image) fr.cache_slot[pix] = pending ? q + 1u : 0u;
     }
@@ -519,7 +519,10 @@ __global__ __launch_bounds__(1024) void k_cache_scan(FrameDev fr, uint32_t nbloc
 // its 4096 pixels first (slots from LDS counters), then takes one global slot
 // range per list and writes: one global atomic per list and workgroup (a
 // wave-level append hammered two counters 65 k times per frame: 0.58 ms).
-__global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cache) {
+// frame_ubo: when not null, the frame number comes from that device UBO (a
+// batched view's, which a replayed graph cannot take as an argument).
+__global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cache,
+                                                      const UniformBufferObject* frame_ubo) {
     constexpr uint32_t kRounds = kCacheSelBlock / 256;
     __shared__ uint32_t wcount[4];
     __shared__ uint32_t cnt[3];   // go on, query list, training list
@@ -530,11 +533,12 @@ __global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cach
     const uint32_t nblocks = gridDim.x;
     const uint32_t C = fr.cache_blocks[nblocks];
     const uint64_t T = cache.train_target;
-    const uint32_t o = C ? (cache.frame * 7919u) % C : 0u;
+    const uint32_t frame = frame_ubo ? frame_ubo->currentFrame : cache.frame;
+    const uint32_t o = C ? (frame * 7919u) % C : 0u;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x < 3) cnt[threadIdx.x] = 0u;
     uint32_t running = fr.cache_blocks[blockIdx.x];
-    const float4* pend = fr.tail + 4 * (size_t)fr.n;
+    const float4* pend = fr.pending;
     for (uint32_t r = 0; r < kRounds; ++r) {
         const uint32_t j = r * 256 + threadIdx.x;
         const uint32_t p = blockIdx.x * kCacheSelBlock + j;
@@ -1180,14 +1184,16 @@ __device__ __forceinline__ bool batch_tile(uint32_t* queue, uint32_t tiles, uint
     return *b < tiles;
 }
 
+// With the radiance cache (cache.enabled; its frame number comes from each
+// view's UBO) the candidates park their records in their view's pending
+// queue, as k_initial_sample does, for the view's training choice.
 template <int kTw, int kTh, int kLds, bool kPersistent>
 __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_initial(SceneDev sc, BatchDev bd,
-                                                                                  uint32_t tiles) {
+                                                                                  uint32_t tiles, CacheCfg cache) {
     __shared__ int stack[kLds * kBlock];
     __shared__ uint32_t lds_count;
     __shared__ uint32_t lds_base;
     __shared__ uint32_t lds_tile;
-    const CacheCfg nocache = {0u, 1u, 0u, 0.0f};
     uint32_t rays = 0, valid = 0, b = 0;
     for (uint32_t it = 0; batch_tile<kPersistent>(bd.tile_queue, tiles, it, &lds_tile, &b); ++it) {
         uint32_t local;
@@ -1196,13 +1202,25 @@ __global__ __launch_bounds__(kBlock) XR_TRAVERSAL_OCCUPANCY void k_batch_initial
         const UniformBufferObject& ubo = bd.ubos[v];
         const TileCoord tc = tile_coord<kTw, kTh>(local, fr.width, fr.height);
         bool alive = false;
+        uint32_t kind = 0;
         PathState ps;
         PathOut o;
         if (tc.in_image) {
-            alive = initial_head_pixel<kLds>(sc, fr, ubo, tc.x, tc.y, nocache, stack + threadIdx.x, &ps, &o, &rays);
+            alive = initial_head_pixel<kLds>(sc, fr, ubo, tc.x, tc.y, cache, stack + threadIdx.x, &ps, &o, &rays);
+            kind = o.cache;
             valid += o.valid ? 1u : 0u;
         }
-        hand_on(alive, bd.tail, bd.tail_count, tc.y * fr.width + tc.x, v, ps, o, &lds_count, &lds_base);
+        const uint32_t pix = tc.y * fr.width + tc.x;
+        const bool pending = kind >= kCachePending;
+        hand_on(alive && !pending, bd.tail, bd.tail_count, pix, v, ps, o, &lds_count, &lds_base);
+        if (cache.enabled) {  // uniform over the launch
+            const uint32_t q = block_slot(pending, fr.pending_count, &lds_count, &lds_base);
+            if (pending) {
+                o.cache = kind | (alive ? kRecordAlive : 0u);
+                store_path(fr.pending + 4 * (size_t)q, pix, v, ps, o);
+            }
+            if (tc.in_image) fr.cache_slot[pix] = pending ? q + 1u : 0u;
+        }
     }
     count_rays(&bd.counters[0], rays, &lds_count);
     count_rays(&bd.counters[2], valid, &lds_count);
@@ -1260,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLds <= 
 struct BatchKernels {
     uint32_t tw, th, lds;
     bool persistent;
-    void (*initial)(SceneDev, BatchDev, uint32_t);
+    void (*initial)(SceneDev, BatchDev, uint32_t, CacheCfg);
     void (*tail)(SceneDev, BatchDev);
     void (*spatial)(SceneDev, BatchDev, PassParams, uint32_t);
 };
@@ -1305,10 +1323,54 @@ void batch_config_info(uint32_t cfg, uint32_t* tw, uint32_t* th, uint32_t* lds, 
     *persistent = kBatchKernels[cfg].persistent;
 }
 
-// One ReSTIR GI frame of every view (initial head + tail, temporal, spatial).
-// views: host copy of the view table (sizes), bd: its device twin.  One
-// launch clears the tail count and the persistent kernels' tile queues.
-hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+// Per-frame counts of the radiance cache in every view: query and training
+// list lengths and pending records.
+__global__ void k_batch_clear_cache(BatchDev bd) {
+    const uint32_t v = threadIdx.x;
+    if (v >= bd.nviews) return;
+    bd.views[v].cache_list[0] = 0u;
+    bd.views[v].cache_list[1] = 0u;
+    *bd.views[v].pending_count = 0u;
+}
+
+// Training samples of all views together (the mean loss of the batch's
+// training step divides by it).
+__global__ void k_batch_train_total(BatchDev bd, uint32_t* total) {
+    uint32_t t = 0;
+    for (uint32_t v = threadIdx.x; v < bd.nviews; v += 64u) t += bd.views[v].cache_list[1];
+    for (int off = 32; off > 0; off >>= 1) t += (uint32_t)__shfl_xor((int)t, off);
+    if (threadIdx.x == 0) *total = t;
+}
+
+hipError_t launch_batch_train_total(const BatchDev& bd, uint32_t* total, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_train_total, dim3(1), dim3(64), 0, st, bd, total);
+    return hipGetLastError();
+}
+
+// The initial-sample pass of every view (head, the views' training choices
+// in cache mode, tail).  views: host copy of the view table, bd: its device
+// twin.  One launch clears the tail count and the persistent kernels' tile
+// queues.
+hipError_t launch_batch_initial(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                                const CacheCfg& cache, hipStream_t st) {
+    const BatchKernels& k = kBatchKernels[cfg];
+    uint32_t tiles = 0;
+    for (uint32_t v = 0; v < bd.nviews; ++v) tiles += image_grid(views[v].width, views[v].height, k.tw, k.th);
+    const uint32_t gi = k.persistent ? std::min(tiles, resident_grid(cfg, 0)) : tiles;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, bd.tail_count, kBatchCountWords);
+    if (cache.enabled) hipLaunchKernelGGL(k_batch_clear_cache, dim3(1), dim3(64), 0, st, bd);
+    hipLaunchKernelGGL(k.initial, dim3(gi), dim3(kBlock), 0, st, sc, bd, tiles, cache);
+    if (cache.enabled)
+        for (uint32_t v = 0; v < bd.nviews; ++v) {
+            const hipError_t e = launch_cache_choice(views[v], cache, bd.ubos + v, st);
+            if (e != hipSuccess) return e;
+        }
+    hipLaunchKernelGGL(k.tail, dim3(2048), dim3(kBlock), 0, st, sc, bd);
+    return hipGetLastError();
+}
+
+// Temporal and spatial passes of every view.
+hipError_t launch_batch_reuse(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
                               const PassParams& pp, hipStream_t st) {
     const BatchKernels& k = kBatchKernels[cfg];
     uint32_t tiles = 0, pixels = 0;
@@ -1316,14 +1378,18 @@ hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const Fram
         tiles += image_grid(views[v].width, views[v].height, k.tw, k.th);
         pixels += views[v].n;
     }
-    const uint32_t gi = k.persistent ? std::min(tiles, resident_grid(cfg, 0)) : tiles;
     const uint32_t gs = k.persistent ? std::min(tiles, resident_grid(cfg, 1)) : tiles;
-    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, bd.tail_count, kBatchCountWords);
-    hipLaunchKernelGGL(k.initial, dim3(gi), dim3(kBlock), 0, st, sc, bd, tiles);
-    hipLaunchKernelGGL(k.tail, dim3(2048), dim3(kBlock), 0, st, sc, bd);
     hipLaunchKernelGGL(k_batch_temporal, dim3((pixels + 255) / 256), dim3(256), 0, st, bd, pp);
     hipLaunchKernelGGL(k.spatial, dim3(gs), dim3(kBlock), 0, st, sc, bd, pp, tiles);
     return hipGetLastError();
+}
+
+hipError_t launch_batch_frame(const SceneDev& sc, const BatchDev& bd, const FrameDev* views, uint32_t cfg,
+                              const PassParams& pp, hipStream_t st) {
+    const CacheCfg nocache = {0u, 1u, 0u, 0.0f};
+    const hipError_t e = launch_batch_initial(sc, bd, views, cfg, nocache, st);
+    if (e != hipSuccess) return e;
+    return launch_batch_reuse(sc, bd, views, cfg, pp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1534,17 +1600,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, const float4* __r
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_cache_choice(const FrameDev& fr, const CacheCfg& cache, const UniformBufferObject* frame_ubo,
+                               hipStream_t st) {
+    const uint32_t nblocks = (fr.n + kCacheSelBlock - 1) / kCacheSelBlock;
+    hipLaunchKernelGGL(k_cache_count, dim3(nblocks), dim3(256), 0, st, fr);
+    hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(1024), 0, st, fr, nblocks);
+    hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(256), 0, st, fr, cache, frame_ubo);
+    return hipGetLastError();
+}
+
 hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const UniformBufferObject& ubo,
                                  const CacheCfg& cache, hipStream_t st) {
     hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, fr.tail_count, kTailCounters);
     hipLaunchKernelGGL(k_initial_sample, dim3(image_grid(fr.width, fr.height)), dim3(kBlock), 0, st, sc, fr, ubo,
                        cache);
-    if (cache.enabled) {
-        const uint32_t nblocks = (fr.n + kCacheSelBlock - 1) / kCacheSelBlock;
-        hipLaunchKernelGGL(k_cache_count, dim3(nblocks), dim3(256), 0, st, fr);
-        hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(1024), 0, st, fr, nblocks);
-        hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(256), 0, st, fr, cache);
-    }
+    if (cache.enabled) launch_cache_choice(fr, cache, nullptr, st);
     const uint32_t nb = ubo.numberOfBounces;
     for (uint32_t b = kHeadBounces; b < nb; ++b)
         hipLaunchKernelGGL(k_tail_bounce, dim3((fr.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, sc, fr, ubo,

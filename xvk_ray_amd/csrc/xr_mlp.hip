@@ -349,7 +349,9 @@ constexpr int kTrainWaves = 4;
 // times inv_count.  g = loss_scale * dLoss/dp.
 // kFromFrame: the samples are the frame's training pixels listed by the
 // initial pass (*count of them in list); otherwise nslots rows of
-// caller-given features and targets.  inv_count = 1 / (3 * sample count).
+// caller-given features and targets.  inv_count = 1 / (3 * sample count),
+// where in frame mode the count is *total when given (the training samples
+// of several batched views, trained as one step over their union).
 template <bool kFromFrame>
 __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const float4* __restrict__ samples,
                                                                 uint32_t n, const uint32_t* __restrict__ count,
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
                                                                 const float4* __restrict__ materials,
                                                                 const float* __restrict__ feats,
                                                                 const float* __restrict__ targets, float loss_scale,
-                                                                float inv_count) {
+                                                                float inv_count, const uint32_t* __restrict__ total) {
     __shared__ TrainLds s;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane & 31, lk = 8 * (lane >> 5);
@@ -365,7 +367,8 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
     float* grad = m.grad + (size_t)(blockIdx.x % kGradCopies) * kMlpParams;
     if (kFromFrame) {
         nslots = *count;
-        inv_count = 1.0f / (3.0f * (float)(nslots ? nslots : 1u));
+        const uint32_t all = total ? *total : nslots;
+        inv_count = 1.0f / (3.0f * (float)(all ? all : 1u));
     }
     const uint32_t ntiles = (nslots + 63) / 64;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -647,12 +650,12 @@ hipError_t launch_mlp_eval(const MlpState& st, const float* feats, uint32_t rows
 }
 
 hipError_t launch_mlp_train_frame(const MlpState& st, const FrameDev& fr, const float4* materials, float loss_scale,
-                                  hipStream_t s) {
+                                  hipStream_t s, const uint32_t* total) {
     // The sample count is on the device (cache_list[1]); 256 workgroups
     // stride over its 64-sample tiles.
     hipLaunchKernelGGL(k_mlp_train<true>, dim3(256), dim3(64 * kTrainWaves), 0, s, mlp_dev(st), fr.samples, fr.n,
                        (const uint32_t*)(fr.cache_list + 1), (const uint32_t*)(fr.cache_list + 2 + fr.n), 0u,
-                       materials, (const float*)nullptr, (const float*)nullptr, loss_scale, 0.0f);
+                       materials, (const float*)nullptr, (const float*)nullptr, loss_scale, 0.0f, total);
     return hipGetLastError();
 }
 
@@ -718,7 +721,7 @@ hipError_t launch_mlp_train_rows(const MlpState& st, const float* feats, const f
     hipLaunchKernelGGL(k_mlp_train<false>, dim3(blocks ? blocks : 1), dim3(64 * kTrainWaves), 0, s, mlp_dev(st),
                        (const float4*)nullptr, 0u, (const uint32_t*)nullptr, (const uint32_t*)nullptr, rows,
                        (const float4*)nullptr, feats,
-                       targets, loss_scale, 1.0f / (3.0f * (float)(rows ? rows : 1)));
+                       targets, loss_scale, 1.0f / (3.0f * (float)(rows ? rows : 1)), (const uint32_t*)nullptr);
     return hipGetLastError();
 }
 

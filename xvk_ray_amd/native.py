@@ -8,6 +8,7 @@ fallback for any computation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -507,6 +508,20 @@ def _typed(out: np.ndarray, which: int, layout: int) -> np.ndarray:
     return out.view(np.float32)
 
 
+# Contexts still open when the interpreter exits (e.g. held by a traceback)
+# are closed before the HIP runtime's own teardown, batches first.
+_LIVE_CONTEXTS: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts() -> None:
+    for ctx in list(_LIVE_CONTEXTS):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One xr_ctx: scene on the device, frame buffers, acceleration structure."""
 
@@ -518,7 +533,11 @@ class Context:
         if rc:
             raise XrError(rc, lib().xr_global_last_error().decode())
         self._h = h
-        self._batches = weakref.WeakSet()  # closed before the context: a batch must not outlive it
+        # open batches, held until they close: a batch must not outlive its
+        # context, also when both are collected together (weak references
+        # would already be cleared when the context's finalizer runs)
+        self._batches: dict[int, "Batch"] = {}
+        _LIVE_CONTEXTS.add(self)
         self.width, self.height = width, height
         tex = scene.textures()
         if tex:
@@ -703,7 +722,7 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None) is not None:
-            for b in list(getattr(self, "_batches", ())):
+            for b in list(getattr(self, "_batches", {}).values()):
                 b.close()
             lib().xr_destroy(self._h)
             self._h = None
@@ -783,7 +802,7 @@ class Batch:
         h = C.c_void_p()
         ctx.check(lib().xr_batch_create(ctx.handle, len(self.sizes), ws.ctypes.data, hs.ctypes.data, C.byref(h)))
         self._h = h
-        ctx._batches.add(self)
+        ctx._batches[id(self)] = self
 
     @staticmethod
     def config_count() -> int:
@@ -854,6 +873,7 @@ class Batch:
         if getattr(self, "_h", None) is not None:
             lib().xr_batch_destroy(self._h)
             self._h = None
+            self.ctx._batches.pop(id(self), None)
 
     def __del__(self):
         try:
