@@ -195,7 +195,7 @@ static_assert(offsetof(InferLds, act) % 16 == 0 && sizeof(InferLds) <= 160 * 102
 // W^T is 96 (layer 0) or 64 elements, so no 8-element chunk straddles a row.
 // Element-wise 2-byte loads made this the launch's floor (about 40 us, paid
 // even when the frame has few queries).
-__device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
+__device__ __forceinline__ void load_wt_lds(const MlpDev& m, bool f8, uint16_t* wt, float* bias, float* w5) {
     constexpr int kChunks0 = 64 * kMlpIn / 8;
     constexpr int kChunks = kChunks0 + 4 * 64 * 64 / 8;
     for (int c = threadIdx.x; c < kChunks; c += blockDim.x) {
@@ -212,7 +212,7 @@ __device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
             dst = 64 * kStrideIn + (l - 1) * 64 * kStrideHid + r * kStrideHid + col;
         }
         u32x4 v;
-        if (m.wt_f8) {
+        if (f8) {
             const uint2 q = *(const uint2*)(m.wt_f8 + src);
             const float sc = m.f8_scale[l];
             uint32_t h[4];
@@ -227,10 +227,13 @@ __device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
         } else {
             v = *(const u32x4*)(m.wt_bf + src);
         }
-        *(u32x4*)(s.wt + dst) = v;
+        *(u32x4*)(wt + dst) = v;
     }
-    for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) s.bias[i] = m.w[mlp_boff(i / 64) + i % 64];
-    for (int i = threadIdx.x; i < 64 * 3 + 3; i += blockDim.x) s.w5[i] = m.w[mlp_woff(5) + i];
+    for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) bias[i] = m.w[mlp_boff(i / 64) + i % 64];
+    for (int i = threadIdx.x; i < 64 * 3 + 3; i += blockDim.x) w5[i] = m.w[mlp_woff(5) + i];
+}
+__device__ __forceinline__ void load_weights_lds(const MlpDev& m, InferLds& s) {
+    load_wt_lds(m, m.wt_f8 != nullptr, s.wt, s.bias, s.w5);
 }
 
 // Runs the five hidden layers and the output layer on the tile in act[w]
@@ -309,12 +312,31 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_eval(MlpDev m, const f
 // ---------------------------------------------------------------------------
 // Training: forward + backward of 64-sample tiles, four waves per workgroup
 // ---------------------------------------------------------------------------
+// LDS of a training workgroup (one per CU: the kernel's registers allow one
+// wave per SIMD): the tile's activations and the network.  The MFMA operands
+// of the weights come from padded LDS copies loaded once per workgroup (W^T
+// for the forward products, W of layers 1..4 for dH = dz . W^T) instead of
+// from the BF16 copies in global memory once per tile and layer.
 struct TrainLds {
     alignas(16) uint16_t x[64 * kStrideIn];
     alignas(16) uint16_t h[5][64 * kStrideHid];  // outputs of hidden layers 0..4
     alignas(16) uint16_t dz[64 * kStrideHid];
     alignas(16) float g[64 * 4];
+    alignas(16) uint16_t wt[64 * kStrideIn + 4 * 64 * kStrideHid];
+    alignas(16) uint16_t wb[4 * 64 * kStrideHid];
+    alignas(16) float bias[5 * 64];
+    alignas(16) float w5[64 * 3 + 3];
 };
+static_assert(sizeof(TrainLds) <= 160 * 1024, "TrainLds layout");
+
+// W of hidden layers 1..4 (row-major [64][64]) into rows of kStrideHid.
+__device__ __forceinline__ void load_w_lds(const MlpDev& m, uint16_t* wb) {
+    for (int c = threadIdx.x; c < 4 * 512; c += blockDim.x) {
+        const int l = c / 512, e = c % 512, r = e / 8, col = (e % 8) * 8;
+        *(u32x4*)(wb + l * 64 * kStrideHid + r * kStrideHid + col) =
+            *(const u32x4*)(m.w_bf + l * 4096 + r * 64 + col);
+    }
+}
 
 // One 32x32 output tile (row block tr, column block tc) of a hidden layer:
 // the arithmetic of fwd_layer for that tile.
@@ -371,6 +393,10 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
         inv_count = 1.0f / (3.0f * (float)(all ? all : 1u));
     }
     const uint32_t ntiles = (nslots + 63) / 64;
+    if ((uint32_t)blockIdx.x >= ntiles) return;  // uniform over the workgroup
+    load_wt_lds(m, false, s.wt, s.bias, s.w5);
+    load_w_lds(m, s.wb);
+    __syncthreads();
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         // inputs: wave w < 3 writes the 24 frequency features of axis w, wave 3
         // columns 72..95 (direction, normal, albedo, padding)
@@ -408,18 +434,17 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
         store_row(s.x + lane * kStrideIn + c0, f, 24);
         __syncthreads();
         // forward, keeping every hidden activation: wave wv computes tile (tr, tc)
-        fwd_subtile<kMlpIn>(s.x, kStrideIn, m.wt_bf + wt_off(0), kMlpIn, m.w + mlp_boff(0), s.h[0], kStrideHid, lane,
-                            tr, tc);
+        fwd_subtile<kMlpIn>(s.x, kStrideIn, s.wt, kStrideIn, s.bias, s.h[0], kStrideHid, lane, tr, tc);
         for (int l = 1; l < 5; ++l) {
             __syncthreads();
-            fwd_subtile<64>(s.h[l - 1], kStrideHid, m.wt_bf + wt_off(l), 64, m.w + mlp_boff(l), s.h[l], kStrideHid,
-                            lane, tr, tc);
+            fwd_subtile<64>(s.h[l - 1], kStrideHid, s.wt + 64 * kStrideIn + (l - 1) * 64 * kStrideHid, kStrideHid,
+                            s.bias + 64 * l, s.h[l], kStrideHid, lane, tr, tc);
         }
         __syncthreads();
-        const float* w5 = m.w + mlp_woff(5);
+        const float* w5 = s.w5;
         if (wv == 0) {
             float pred[3];
-            out_layer(s.h[4] + lane * kStrideHid, w5, m.w + mlp_boff(5), pred);
+            out_layer(s.h[4] + lane * kStrideHid, w5, w5 + 192, pred);
             for (int c = 0; c < 3; ++c) {
                 const float d = pred[c] - y[c];
                 const float gc = valid ? loss_scale * inv_count * 2.0f * d / (pred[c] * pred[c] + 0.01f) : 0.0f;
@@ -485,12 +510,12 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
             }
             if (l == 0) break;
             // dH_in = dz . W_l^T  (W_l row-major [64][64] from the BF16 copy): wave wv computes tile (tr, tc)
-            const uint16_t* wl = m.w_bf + (l - 1) * 4096;
+            const uint16_t* wl = s.wb + (l - 1) * 64 * kStrideHid;
             f32x16 acc = {};
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
                 const bf16x8 a = frag16(s.dz + (tr * 32 + lr) * kStrideHid + ks * 16 + lk);
-                const bf16x8 b = frag16(wl + (tc * 32 + lr) * 64 + ks * 16 + lk);
+                const bf16x8 b = frag16(wl + (tc * 32 + lr) * kStrideHid + ks * 16 + lk);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
             __syncthreads();  // every wave is done reading dz
