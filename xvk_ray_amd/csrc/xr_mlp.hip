@@ -14,8 +14,9 @@
 //   k_mlp_infer   cache queries of the initial-sample pass -> L_out (persistent,
 //                 weights in LDS, 8 waves per workgroup)
 //   k_mlp_train   forward + backward on the frame's training samples (4 waves per
-//                 64-sample tile), FP32 gradients accumulated with atomics into
-//                 32 partial copies
+//                 64-sample tile), FP32 gradients accumulated in registers over
+//                 the workgroup's tiles, then added with atomics into 32
+//                 partial copies
 //   k_mlp_adam    sums the partials, removes the loss scale, Adam step on the
 //                 FP32 master weights, repacks the BF16 copies
 //   k_mlp_eval    forward on caller-given inputs (testing)
@@ -397,6 +398,17 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
     load_wt_lds(m, false, s.wt, s.bias, s.w5);
     load_w_lds(m, s.wb);
     __syncthreads();
+    // This wave's gradient tiles, accumulated over every tile the workgroup
+    // trains and added into the partial copy once at the end: dw[l - 1] is
+    // output tile (wv >> 1, wv & 1) of dW_l for l = 1..4, dw[4] that tile of
+    // dW_0 and dw[5] (waves 0 and 1) tile (2, wv) of dW_0; db[l] is this
+    // lane's column sum of dz_l over rows 16 wv .. 16 wv + 15; d5[c] is
+    // dW5[lane][c] and d5[3] db5[lane] (wave 0).
+    f32x16 dw[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) dw[i] = f32x16{};
+    float db[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float d5[4] = {0.f, 0.f, 0.f, 0.f};
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         // inputs: wave w < 3 writes the 24 frequency features of axis w, wave 3
         // columns 72..95 (direction, normal, albedo, padding)
@@ -461,13 +473,13 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
                 a1 = fmaf(x, s.g[r * 4 + 1], a1);
                 a2 = fmaf(x, s.g[r * 4 + 2], a2);
             }
-            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 0], a0);
-            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 1], a1);
-            atomicAdd(&grad[mlp_woff(5) + 3 * lane + 2], a2);
+            d5[0] += a0;
+            d5[1] += a1;
+            d5[2] += a2;
             if (lane < 3) {
                 float b = 0.f;
                 for (int r = 0; r < 64; ++r) b += s.g[r * 4 + lane];
-                atomicAdd(&grad[mlp_boff(5) + lane], b);
+                d5[3] += b;
             }
         }
         // dZ4 = (g . W5^T) masked by h4 > 0: lane = row, wave wv takes columns 16 wv .. 16 wv + 15
@@ -481,32 +493,29 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
         }
         __syncthreads();
         // hidden layers 4..0: dW_l = in^T . dz, db_l = colsum(dz), dz_prev = (dz . W_l^T) * (in > 0)
+#pragma unroll
         for (int l = 4; l >= 0; --l) {
             const uint16_t* in = l == 0 ? s.x : s.h[l - 1];
             const int sin = l == 0 ? kStrideIn : kStrideHid;
-            const int K = l == 0 ? kMlpIn : 64;
-            const int wo = mlp_woff(l);
-            const int nsub = (K / 32) * 2;
-            for (int sub = wv; sub < nsub; sub += kTrainWaves) {
-                const int tm = sub >> 1, tn = sub & 1;
-                f32x16 acc = {};
+            // dW_l tile (tm, tn) += in[:, 32 tm ..]^T . dz[:, 32 tn ..]
+            auto dw_tile = [&](f32x16& acc, int tm, int tn) {
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
                     const bf16x8 a = frag_gather(in + (ks * 16 + lk) * sin + tm * 32 + lr, sin);
                     const bf16x8 b = frag_gather(s.dz + (ks * 16 + lk) * kStrideHid + tn * 32 + lr, kStrideHid);
                     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
                 }
-                const int col = tn * 32 + lr;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int row = tm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                    atomicAdd(&grad[wo + row * 64 + col], acc[j]);
-                }
+            };
+            if (l > 0) {
+                dw_tile(dw[l - 1], tr, tc);
+            } else {
+                dw_tile(dw[4], tr, tc);
+                if (wv < 2) dw_tile(dw[5], 2, wv);
             }
             {  // column = lane, rows 16 wv .. 16 wv + 15
                 float b = 0.f;
                 for (int r = 16 * wv; r < 16 * wv + 16; ++r) b += bf16_float(s.dz[r * kStrideHid + lane]);
-                atomicAdd(&grad[mlp_boff(l) + lane], b);
+                db[l] += b;
             }
             if (l == 0) break;
             // dH_in = dz . W_l^T  (W_l row-major [64][64] from the BF16 copy): wave wv computes tile (tr, tc)
@@ -529,6 +538,26 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(MlpDev m, const 
             __syncthreads();
         }
         __syncthreads();
+    }
+    auto flush = [&](const f32x16& acc, int l, int tm, int tn) {
+        const int col = tn * 32 + lr;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int row = tm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            atomicAdd(&grad[mlp_woff(l) + row * 64 + col], acc[j]);
+        }
+    };
+#pragma unroll
+    for (int l = 1; l < 5; ++l) flush(dw[l - 1], l, tr, tc);
+    flush(dw[4], 0, tr, tc);
+    if (wv < 2) flush(dw[5], 0, 2, wv);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) atomicAdd(&grad[mlp_boff(l) + lane], db[l]);
+    if (wv == 0) {
+        atomicAdd(&grad[mlp_woff(5) + 3 * lane + 0], d5[0]);
+        atomicAdd(&grad[mlp_woff(5) + 3 * lane + 1], d5[1]);
+        atomicAdd(&grad[mlp_woff(5) + 3 * lane + 2], d5[2]);
+        if (lane < 3) atomicAdd(&grad[mlp_boff(5) + lane], d5[3]);
     }
 }
 
