@@ -96,7 +96,9 @@ MLP_FLOP_QUERY = 45440
 MLP_FLOP_TRAIN = 136320
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 SPREADS = (0.01, 100.0)
-STUDY_NEIGHBOURS = (5, 3, 2, 1)
+# spatial neighbour counts of the study; 0 keeps the spatial pass (the
+# centre reservoir's M cap, W and shading) without neighbours
+STUDY_NEIGHBOURS = (5, 3, 2, 1, 0)
 
 
 def mlp_study(scene, camera, args) -> dict:

@@ -104,7 +104,7 @@ struct CacheCfg {
     uint32_t frame;         // rotates the training choice
     float spread_c;         // terminate at x_s when its spread exceeds spread_c * primary footprint
 };
-// Pixels per workgroup of the training-choice kernels (256 threads x 16).
+// Pixels per workgroup of the training-choice kernels.
 constexpr uint32_t kCacheSelBlock = 4096;
 // The frame's training choice among its pending candidates (after the head
 // kernel); frame_ubo, when not null, is the device UBO holding the frame

@@ -462,8 +462,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
 // an exclusive scan of per-pixel flags over 4096-pixel blocks; the chosen ones
 // and the plain candidates join the tail queue, the others end as queries.
 // ---------------------------------------------------------------------------
-// Candidates among the 16 pixels of round r of this thread in its block
-// (pixel block*4096 + r*256 + thread: rounds read coalesced rows).
+// Candidates among the pixels of this workgroup's block (pixel
+// block*kCacheSelBlock + r*256 + thread in round r: rounds read coalesced rows).
 __global__ __launch_bounds__(256) void k_cache_count(FrameDev fr) {
     __shared__ uint32_t part[4];
     const uint32_t base = blockIdx.x * kCacheSelBlock;
@@ -515,72 +515,107 @@ __global__ __launch_bounds__(1024) void k_cache_scan(FrameDev fr, uint32_t nbloc
 // floor((r' + 1) T / C) > floor(r' T / C) (exactly min(T, C) of them).
 // Training and plain candidates still alive join the tail queue (tail_count[0]);
 // queries and finished paths write their L_out and kind.  Appends the
-// frame's query / training pixel lists and counts them.  A workgroup decides
-// its 4096 pixels first (slots from LDS counters), then takes one global slot
-// range per list and writes: one global atomic per list and workgroup (a
-// wave-level append hammered two counters 65 k times per frame: 0.58 ms).
+// frame's query / training pixel lists and counts them.  A thread of the
+// 1024-thread workgroup holds its 4 pixels (block*4096 + r*1024 + thread) in
+// registers: their slots and record words are loaded before any ranking, one
+// wave scans the (round, wave) candidate counts for the ranks, list slots
+// within the workgroup come from one LDS atomic per list and wave-round, and
+// the workgroup takes one global slot range per list (a wave-level global
+// append hammered two counters 65 k times per frame: 0.58 ms; 1024-pixel
+// workgroups made four times as many same-address range atomics and took
+// 79 against 49 us).
 // frame_ubo: when not null, the frame number comes from that device UBO (a
 // batched view's, which a replayed graph cannot take as an argument).
-__global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cache,
-                                                      const UniformBufferObject* frame_ubo) {
-    constexpr uint32_t kRounds = kCacheSelBlock / 256;
-    __shared__ uint32_t wcount[4];
-    __shared__ uint32_t cnt[3];   // go on, query list, training list
+constexpr uint32_t kSelThreads = 1024;
+__global__ __launch_bounds__(kSelThreads) void k_cache_select(FrameDev fr, CacheCfg cache,
+                                                              const UniformBufferObject* frame_ubo) {
+    constexpr uint32_t kRounds = kCacheSelBlock / kSelThreads, kWaves = kSelThreads / 64;
+    static_assert(kRounds * kWaves <= 64, "one wave scans the (round, wave) counts");
+    __shared__ uint32_t wpre[kRounds * kWaves];  // candidates of (round, wave), then their exclusive prefix
+    __shared__ uint32_t cnt[3];             // go on, query list, training list
     __shared__ uint32_t base[3];
-    __shared__ uint8_t dec[kCacheSelBlock];      // bit 0: go on, bits 1-2: kind (1 query, 2 training), 0xFF: none
-    __shared__ uint16_t idx_go[kCacheSelBlock];  // slot within the workgroup's tail range
-    __shared__ uint16_t idx_ls[kCacheSelBlock];  // slot within the workgroup's query / training range
     const uint32_t nblocks = gridDim.x;
     const uint32_t C = fr.cache_blocks[nblocks];
     const uint64_t T = cache.train_target;
     const uint32_t frame = frame_ubo ? frame_ubo->currentFrame : cache.frame;
     const uint32_t o = C ? (frame * 7919u) % C : 0u;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
     if (threadIdx.x < 3) cnt[threadIdx.x] = 0u;
-    uint32_t running = fr.cache_blocks[blockIdx.x];
     const float4* pend = fr.pending;
+    const uint32_t p0 = blockIdx.x * kCacheSelBlock + threadIdx.x;
+    uint32_t slot[kRounds], word[kRounds], lower[kRounds];
+#pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
-        const uint32_t j = r * 256 + threadIdx.x;
-        const uint32_t p = blockIdx.x * kCacheSelBlock + j;
-        const uint32_t slot = p < fr.n ? fr.cache_slot[p] : 0u;
-        const bool cand = slot != 0u;
-        const unsigned long long m = __ballot(cand);
-        if (lane == 0) wcount[wv] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t rank = running + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        for (uint32_t k = 0; k < wv; ++k) rank += wcount[k];
-        running += wcount[0] + wcount[1] + wcount[2] + wcount[3];
-        uint8_t d = 0xFFu;
+        const uint32_t p = p0 + r * kSelThreads;
+        slot[r] = p < fr.n ? fr.cache_slot[p] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        word[r] = slot[r] ? __float_as_uint(pend[4 * (size_t)(slot[r] - 1u) + 2].w) : 0u;
+        const unsigned long long m = __ballot(slot[r] != 0u);
+        lower[r] = (uint32_t)__popcll(m & below);
+        if (lane == 0) wpre[r * kWaves + wv] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const uint32_t v = lane < kRounds * kWaves ? wpre[lane] : 0u;
+        uint32_t x = v;  // inclusive scan within the wave (every lane shuffles)
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane < kRounds * kWaves) wpre[lane] = x - v;
+    }
+    __syncthreads();
+    const uint32_t running = fr.cache_blocks[blockIdx.x];
+    // per pixel: bit 0 go on, bits 1-2 kind (1 query, 2 training), bits 3-15
+    // slot in the workgroup's tail range, bits 16-28 slot in its list range;
+    // all ones: not a candidate
+    uint32_t dec[kRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+        const bool cand = slot[r] != 0u;
+        uint32_t kind = 0u;
+        bool go_on = false;
         if (cand) {
-            const float4 r2 = pend[4 * (size_t)(slot - 1u) + 2];
-            const uint32_t w = __float_as_uint(r2.w);
-            uint32_t rr = rank + o;
+            uint32_t rr = running + wpre[r * kWaves + wv] + lower[r] + o;
             if (rr >= C) rr -= C;
             const bool train = (uint64_t)C <= T || ((uint64_t)rr + 1u) * T / C > (uint64_t)rr * T / C;
-            const uint32_t kind = train ? 2u : ((w & 0xFFu) == kCachePendingQuery ? 1u : 0u);
-            const bool go_on = kind != 1u && (w & kRecordAlive) != 0u;
-            d = (uint8_t)((go_on ? 1u : 0u) | (kind << 1));
-            if (go_on) idx_go[j] = (uint16_t)atomicAdd(&cnt[0], 1u);
-            if (kind != 0u) idx_ls[j] = (uint16_t)atomicAdd(&cnt[kind], 1u);
+            kind = train ? 2u : ((word[r] & 0xFFu) == kCachePendingQuery ? 1u : 0u);
+            go_on = kind != 1u && (word[r] & kRecordAlive) != 0u;
         }
-        dec[j] = d;
-        __syncthreads();  // wcount is rewritten by the next round
+        const unsigned long long mg = __ballot(cand && go_on);
+        const unsigned long long mq = __ballot(cand && kind == 1u);
+        const unsigned long long mt = __ballot(cand && kind == 2u);
+        uint32_t bg = 0u, bq = 0u, bt = 0u;
+        if (lane == 0) {
+            if (mg) bg = atomicAdd(&cnt[0], (uint32_t)__popcll(mg));
+            if (mq) bq = atomicAdd(&cnt[1], (uint32_t)__popcll(mq));
+            if (mt) bt = atomicAdd(&cnt[2], (uint32_t)__popcll(mt));
+        }
+        bg = (uint32_t)__shfl((int)bg, 0);
+        bq = (uint32_t)__shfl((int)bq, 0);
+        bt = (uint32_t)__shfl((int)bt, 0);
+        const uint32_t ig = bg + (uint32_t)__popcll(mg & below);
+        const uint32_t il = kind == 2u ? bt + (uint32_t)__popcll(mt & below) : bq + (uint32_t)__popcll(mq & below);
+        dec[r] = cand ? ((go_on ? 1u : 0u) | (kind << 1) | (ig << 3) | (il << 16)) : 0xFFFFFFFFu;
     }
+    __syncthreads();
     if (threadIdx.x < 3) {
         uint32_t* ctr = threadIdx.x == 0 ? fr.tail_count : fr.cache_list + (threadIdx.x - 1u);
         base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(ctr, cnt[threadIdx.x]) : 0u;
     }
     __syncthreads();
+#pragma unroll
     for (uint32_t r = 0; r < kRounds; ++r) {
-        const uint32_t j = r * 256 + threadIdx.x;
-        const uint8_t d = dec[j];
-        if (d == 0xFFu) continue;
-        const uint32_t p = blockIdx.x * kCacheSelBlock + j;
-        const uint32_t slot = fr.cache_slot[p];
-        const uint32_t kind = d >> 1;
-        const float4* src = pend + 4 * (size_t)(slot - 1u);
+        const uint32_t d = dec[r];
+        if (d == 0xFFFFFFFFu) continue;
+        const uint32_t p = p0 + r * kSelThreads;
+        const uint32_t kind = (d >> 1) & 3u;
+        const float4* src = pend + 4 * (size_t)(slot[r] - 1u);
         if (d & 1u) {
-            float4* dst = fr.tail + 4 * (size_t)(base[0] + idx_go[j]);
+            float4* dst = fr.tail + 4 * (size_t)(base[0] + ((d >> 3) & 0x1FFFu));
             dst[0] = src[0];
             dst[1] = src[1];
             const float4 r2 = src[2];
@@ -592,7 +627,7 @@ __global__ __launch_bounds__(256) void k_cache_select(FrameDev fr, CacheCfg cach
         }
         // the material at x_s is kept for cache kinds only (spare word of plane 3)
         if (kind == 0u) reinterpret_cast<uint32_t*>(fr.samples + 3 * (size_t)fr.n + p)[3] = 0u;
-        else fr.cache_list[2 + (kind == 2u ? fr.n : 0u) + base[kind] + idx_ls[j]] = p;
+        else fr.cache_list[2 + (kind == 2u ? fr.n : 0u) + base[kind] + (d >> 16)] = p;
     }
     __shared__ uint32_t lds_count;
     count_rays(&fr.counters[3], threadIdx.x == 0 ? cnt[1] : 0u, &lds_count);
@@ -1605,7 +1640,7 @@ hipError_t launch_cache_choice(const FrameDev& fr, const CacheCfg& cache, const 
     const uint32_t nblocks = (fr.n + kCacheSelBlock - 1) / kCacheSelBlock;
     hipLaunchKernelGGL(k_cache_count, dim3(nblocks), dim3(256), 0, st, fr);
     hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(1024), 0, st, fr, nblocks);
-    hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(256), 0, st, fr, cache, frame_ubo);
+    hipLaunchKernelGGL(k_cache_select, dim3(nblocks), dim3(kSelThreads), 0, st, fr, cache, frame_ubo);
     return hipGetLastError();
 }
 
