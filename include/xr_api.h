@@ -137,7 +137,10 @@ typedef struct xr_scene_desc {
 typedef struct xr_restir_params {
     uint32_t temporal_m_cap;   /* M_MAX of ReSTIRGITemporalReuse.rgen.hlsl:3 (reference 30) */
     uint32_t spatial_m_cap;    /* M_MAX of ReSTIRGISpatialReuse.rgen.hlsl:6 (reference 300) */
-    uint32_t spatial_samples;  /* NUM_SPATIAL_SAMPLES, Spatial.rgen:4 (reference 6) */
+    uint32_t spatial_samples;  /* NUM_SPATIAL_SAMPLES, Spatial.rgen:4 (reference 6), at most 64; with 0
+                                  and both reuse passes run in one call, the spatial pass (the centre's
+                                  reservoir alone) runs in the temporal pass's kernel and its time counts
+                                  as temporal */
     float spatial_radius;      /* SPATIAL_RADIUS in pixels, Spatial.rgen:5 (reference 5.0) */
     uint32_t visibility;       /* XR_VIS_* (reference XR_VIS_REFERENCE) */
     uint32_t plain_max_depth;  /* mode 1 bounce cap, 0 = unbounded as rgen.hlsl:59 */

@@ -227,6 +227,64 @@ def test_restir_cornell_traced_visibility_config_constants():
     _frames(scene, 80, 60, 60.0, eyes, CORNELL_TARGET, 4, params, _strict)
 
 
+@pytest.mark.parametrize("reuse", [1, 0])
+def test_restir_without_spatial_neighbours(reuse):
+    """spatial_samples = 0: one kernel runs the temporal pass and the spatial
+    pass (which then reads nothing but the pixel's own temporal reservoir).
+    Whole frames must equal the oracle's temporal pass followed by its
+    spatial pass with no neighbours (Spatial.rgen:64-183 with
+    NUM_SPATIAL_SAMPLES = 0); a spatial pass with neighbours run alone
+    afterwards must rebuild the neighbour records the fused kernel skips."""
+    from oracle.binding import temporal as orc_temporal
+    scene = cornell_scene()
+    W, H = 80, 60
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    eyes = [CORNELL_EYE, (0.51, 0.5, -1.2), (0.52, 0.5, -1.2)]
+    views = [look_view(e, CORNELL_TARGET) for e in eyes]
+    params = N.default_params(temporal_m_cap=20, spatial_samples=0, spatial_radius=30.0, visibility=N.XR_VIS_TRACE)
+    for k, view in enumerate(views):
+        frame = k + 1
+        ubo = N.make_ubo(view, views[k - 1] if k else view, 60.0, W, H, spp=1, total=1, bounces=4, reuse=reuse,
+                         frame=frame)
+        say(f"[restir] frame {frame} without spatial neighbours, reuse {reuse}")
+        temporal = ctx.read(N.XR_BUF_TEMPORAL).copy()
+        spatial = ctx.read(N.XR_BUF_SPATIAL).copy()
+        output = ctx.read(N.XR_BUF_OUTPUT).copy()
+        ctx.render(ubo, N.XR_MODE_RESTIR, params)
+        ctx.sync()
+        samples = ctx.read(N.XR_BUF_SAMPLES).copy()
+        ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H)["samples"]
+        _strict(bytes_equal_rows(samples, ref), "samples", samples, ref)
+        motion = ctx.read(N.XR_BUF_MOTION).copy()
+        gpos = np.ascontiguousarray(ctx.read(N.XR_BUF_GBUF_POS))
+        orc_temporal(W, H, frame, params.temporal_m_cap, samples, motion, gpos, temporal, spatial, output)
+        orc.spatial(W, H, frame, reuse, 0, params.spatial_radius, params.spatial_m_cap, params.visibility, samples,
+                    temporal, spatial, output)
+        gt = ctx.read(N.XR_BUF_TEMPORAL)
+        _strict(bytes_equal_rows(gt, temporal), "temporal reservoirs", gt, temporal)
+        gs = ctx.read(N.XR_BUF_SPATIAL)
+        _strict(bytes_equal_rows(gs, spatial), "spatial reservoirs", gs, spatial)
+        go = ctx.read(N.XR_BUF_OUTPUT)
+        assert _max_lsb_diff(go, output) <= 1, "output"
+        assert (go != 0).any()
+    # the last frame's spatial pass again, with neighbours, on its own
+    p = N.XrRestirParams.from_buffer_copy(params)
+    p.spatial_samples = 3
+    p.passes = N.XR_PASS_SPATIAL
+    gt = ctx.read(N.XR_BUF_TEMPORAL).copy()
+    spatial = ctx.read(N.XR_BUF_SPATIAL).copy()
+    output = ctx.read(N.XR_BUF_OUTPUT).copy()
+    ctx.render(ubo, N.XR_MODE_RESTIR, p)
+    ctx.sync()
+    orc.spatial(W, H, len(views), reuse, 3, p.spatial_radius, p.spatial_m_cap, p.visibility, samples, gt, spatial,
+                output)
+    gs = ctx.read(N.XR_BUF_SPATIAL)
+    _strict(bytes_equal_rows(gs, spatial), "spatial reservoirs with neighbours", gs, spatial)
+    assert _max_lsb_diff(ctx.read(N.XR_BUF_OUTPUT), output) <= 1
+    ctx.close()
+
+
 def test_restir_terrain_sky():
     scene = small_terrain()
     eyes = [(0.2, 0.4, 0.2), (0.21, 0.4, 0.2)]

@@ -740,13 +740,19 @@ int xr_render(xr_ctx* c, const xr_ubo* ubo, int mode, const xr_restir_params* pa
             XR_HIP(c, launch_mlp_infer(c->mlp, fr, c->d_materials, c->stream));
         }
         XR_HIP(c, hipEventRecord(ev[2], c->stream));
-        if (passes & XR_PASS_TEMPORAL) {
+        // without spatial neighbours the spatial pass runs in the temporal
+        // pass's kernel (its time counts as temporal)
+        const bool spatial0 = (passes & XR_PASS_TEMPORAL) && (passes & XR_PASS_SPATIAL) && pp.spatial_samples == 0u;
+        if (spatial0) {
+            XR_HIP(c, launch_temporal_spatial0(fr, u.currentFrame, u.reuse, pp, c->d_gpos, c->stream));
+            c->nbr_ok = false;  // no neighbour records written
+        } else if (passes & XR_PASS_TEMPORAL) {
             XR_HIP(c, launch_temporal(fr, u.currentFrame, pp, c->d_gpos, c->stream));
             c->nbr_ok = true;
             c->nbr_ping = u.currentFrame % 2u;
         }
         XR_HIP(c, hipEventRecord(ev[3], c->stream));
-        if (passes & XR_PASS_SPATIAL) {
+        if ((passes & XR_PASS_SPATIAL) && !spatial0) {
             if (!c->nbr_ok || c->nbr_ping != u.currentFrame % 2u) {
                 XR_HIP(c, launch_nbr_build(fr, u.currentFrame, c->stream));
                 c->nbr_ok = true;

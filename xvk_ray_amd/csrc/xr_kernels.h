@@ -12,6 +12,11 @@ hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const U
                                  const CacheCfg& cache, hipStream_t st);
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
                            hipStream_t st);
+// The temporal pass followed by the spatial pass without neighbours
+// (pp.spatial_samples = 0) in one kernel: the same writes as the two passes,
+// except the neighbour records, which nothing reads then.
+hipError_t launch_temporal_spatial0(const FrameDev& fr, uint32_t frame, uint32_t reuse, const PassParams& pp,
+                                    const float4* gpos_hist, hipStream_t st);
 // Views rendered together: device arrays of their frame buffers and UBOs, a
 // shared handed-on path list (records name their view) and shared counters.
 struct BatchDev {

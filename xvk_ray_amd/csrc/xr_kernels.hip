@@ -876,8 +876,18 @@ __global__ __launch_bounds__(256) void k_nbr_build(FrameDev fr, uint32_t ping) {
 // ---------------------------------------------------------------------------
 // Temporal reuse pass (HBM bound; no rays)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void spatial_after_temporal(const FrameDev& fr, uint32_t frame, uint32_t reuse,
+                                                       const PassParams& pp, uint32_t x, uint32_t y,
+                                                       const float4* s, const Res& r, v3 t_color);
+
+// kSpatial0: the spatial pass without neighbours (spatial_samples = 0) runs
+// here on the values just computed, instead of a second kernel reading them
+// back (its result depends on this pixel's temporal reservoir alone); no
+// neighbour records are written then.
+template <bool kSpatial0 = false>
 __device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t frame, const PassParams& pp,
-                                               const float4* __restrict__ gpos_hist, uint32_t pix) {
+                                               const float4* __restrict__ gpos_hist, uint32_t pix,
+                                               uint32_t reuse = 0u) {
     const uint32_t n = fr.n, w = fr.width, h = fr.height;
     const uint32_t x = pix % w, y = pix / w;
     uint32_t rng = init_seed(x, y, w, frame);
@@ -894,7 +904,8 @@ __device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t fram
         store_planes(cur, n, pix, r0.p);
         store_planes(spat_cur, n, pix, r0.p);
         if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-        store_nbr(fr.nbr, pix, s[0], s[1], r0);
+        if (kSpatial0) spatial_after_temporal(fr, frame, reuse, pp, x, y, s, r0, mk3(0.f, 0.f, 0.f));
+        else store_nbr(fr.nbr, pix, s[0], s[1], r0);
         return;
     }
     bool found = false;
@@ -940,9 +951,13 @@ __device__ __forceinline__ void temporal_pixel(const FrameDev& fr, uint32_t fram
     if (tp_final > 1e-6f && r_m(r) > 0u) set_W(&r, r_wsum(r) / ((float)r_m(r) * tp_final));
     else set_W(&r, 0.0f);
     store_planes(cur, n, pix, r.p);
-    store_nbr(fr.nbr, pix, s[0], s[1], r);
+    if (!kSpatial0) store_nbr(fr.nbr, pix, s[0], s[1], r);
     v3 color = mk3(0.f, 0.f, 0.f);
     if (r_W(r) > 0.0f) color = shade_reservoir(r, x_curr, xyz(s[1]));
+    if (kSpatial0) {
+        spatial_after_temporal(fr, frame, reuse, pp, x, y, s, r, color);
+        return;
+    }
     fr.output[pix] = resolve_bgra8(color, 1.0f);
     if (fr.radiance) fr.radiance[pix] = f4(color, 0.0f);
 }
@@ -951,6 +966,12 @@ __global__ __launch_bounds__(256) void k_temporal(FrameDev fr, uint32_t frame, P
                                                   const float4* __restrict__ gpos_hist) {
     const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
     if (pix < fr.n) temporal_pixel(fr, frame, pp, gpos_hist, pix);
+}
+
+__global__ __launch_bounds__(256) void k_temporal_spatial0(FrameDev fr, uint32_t frame, PassParams pp,
+                                                           const float4* __restrict__ gpos_hist, uint32_t reuse) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix < fr.n) temporal_pixel<true>(fr, frame, pp, gpos_hist, pix, reuse);
 }
 
 // ---------------------------------------------------------------------------
@@ -988,6 +1009,40 @@ struct SpatialTask {
     uint32_t pM, pnidx;
 };
 
+// The centre's early outs (Spatial.rgen:66-74): no visible surface (black)
+// or no sample (white), written without a reservoir; true when taken.
+__device__ __forceinline__ bool spatial_early_out(const FrameDev& fr, uint32_t pix, float4 s1, float4 s3) {
+    if (length(xyz(s1)) == 0.0f) {
+        fr.output[pix] = 0xFF000000u;
+        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return true;
+    }
+    if (length(xyz(s3)) == 0.0f) {
+        fr.output[pix] = 0xFFFFFFFFu;
+        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return true;
+    }
+    return false;
+}
+
+// Starts the task on an empty reservoir and merges the centre's temporal
+// reservoir (MergeReservoir(r, centerRes, |L|)) from its |L_out| (t.s1.w),
+// W and M; true when the draw takes the centre's sample.
+__device__ __forceinline__ bool spatial_centre_merge(SpatialTask& t, uint32_t width, uint32_t frame, float Wc,
+                                                     uint32_t mc) {
+    t.rng = init_seed(t.x, t.y, width, frame);
+    t.sel = 0xFFFFFFFFu;  // none yet: InitReservoir's sample
+    t.sel_x = mk3(0.f, 0.f, 0.f);
+    t.sel_n = mk3(0.f, 0.f, 0.f);
+    t.ws = 0.0f;
+    t.i = 0;
+    const float wc = t.s1.w * Wc * (float)mc;
+    t.ws = t.ws + wc;
+    const bool take = rand01(&t.rng) < wc / maxf(1e-10f, t.ws);
+    t.m = 0u + mc;
+    return take;
+}
+
 // Loads the centre and merges its reservoir; false when the pixel takes an
 // early out (written here) and needs nothing more.
 __device__ __forceinline__ bool spatial_begin(SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
@@ -1000,24 +1055,14 @@ __device__ __forceinline__ bool spatial_begin(SpatialTask& t, const FrameDev& fr
     const float4* c = fr.nbr + 4 * (size_t)pix;
     t.s0 = c[0];
     t.s1 = c[1];
-    const float4 s3 = fr.samples[3 * n + pix];
-    if (length(xyz(t.s1)) == 0.0f) {
-        fr.output[pix] = 0xFF000000u;
-        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return false;
-    }
-    if (length(xyz(s3)) == 0.0f) {
-        fr.output[pix] = 0xFFFFFFFFu;
-        if (fr.radiance) fr.radiance[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return false;
-    }
+    if (spatial_early_out(fr, pix, t.s1, fr.samples[3 * n + pix])) return false;
     t.rng = init_seed(x, y, fr.width, frame);
     t.sel = 0xFFFFFFFFu;  // none yet: InitReservoir's sample
     t.sel_x = mk3(0.f, 0.f, 0.f);
     t.sel_n = mk3(0.f, 0.f, 0.f);
     t.ws = 0.0f;
     t.i = 0;
-    // MergeReservoir(r, centerRes, |L|)
+    // MergeReservoir(r, centerRes, |L|), as spatial_centre_merge
     const uint32_t mc = __float_as_uint(c[3].w);
     const float wc = t.s1.w * c[2].w * (float)mc;
     t.ws = t.ws + wc;
@@ -1105,17 +1150,15 @@ __device__ __forceinline__ bool spatial_step(SpatialTask& t, const FrameDev& fr,
     return false;
 }
 
-// After the loop: M cap, W, the selected sample copied in, shading, writes.
-__device__ __forceinline__ void spatial_finish(const SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
-                                               float4* __restrict__ wr, uint32_t reuse, const PassParams& pp) {
+// After the loop: take_sel(r) copies the selected sample in (if any), then
+// M cap, W, shading, writes.
+template <class TakeSel>
+__device__ __forceinline__ void spatial_write(const SpatialTask& t, const FrameDev& fr, TakeSel take_sel,
+                                              float4* __restrict__ wr, uint32_t reuse, const PassParams& pp) {
     const uint32_t n = fr.n;
     Res r;
     res_init(&r);
-    if (t.sel != 0xFFFFFFFFu) {
-        float4 sp[kPlanes];
-        load_planes(rd, n, t.sel, sp);
-        take_sample(&r, sp);
-    }
+    take_sel(r);
     set_wsum(&r, t.ws);
     set_m(&r, t.m);
     if (r_m(r) > pp.spatial_m_cap) set_m(&r, pp.spatial_m_cap);
@@ -1128,6 +1171,50 @@ __device__ __forceinline__ void spatial_finish(const SpatialTask& t, const Frame
     if (reuse) {
         fr.output[t.pix] = resolve_bgra8(color, 1.0f);
         if (fr.radiance) fr.radiance[t.pix] = f4(color, 0.0f);
+    }
+}
+
+__device__ __forceinline__ void spatial_finish(const SpatialTask& t, const FrameDev& fr, const float4* __restrict__ rd,
+                                               float4* __restrict__ wr, uint32_t reuse, const PassParams& pp) {
+    spatial_write(
+        t, fr,
+        [&](Res& r) {
+            if (t.sel != 0xFFFFFFFFu) {
+                float4 sp[kPlanes];
+                load_planes(rd, fr.n, t.sel, sp);
+                take_sample(&r, sp);
+            }
+        },
+        wr, reuse, pp);
+}
+
+// The spatial pass of one pixel without neighbours, in the temporal pass's
+// kernel: s = the pixel's sample planes, r = the temporal reservoir just
+// written for it (what spatial_begin and spatial_finish would read back from
+// the neighbour record and the temporal reservoirs), t_color = the temporal
+// pass's shading, which stays the output unless reuse.
+__device__ __forceinline__ void spatial_after_temporal(const FrameDev& fr, uint32_t frame, uint32_t reuse,
+                                                       const PassParams& pp, uint32_t x, uint32_t y,
+                                                       const float4* s, const Res& r, v3 t_color) {
+    const uint32_t n = fr.n, pix = y * fr.width + x;
+    SpatialTask t;
+    t.pix = pix;
+    t.x = x;
+    t.y = y;
+    t.s0 = s[0];
+    t.s1 = make_float4(s[1].x, s[1].y, s[1].z, length(xyz(r.p[4])));
+    if (spatial_early_out(fr, pix, t.s1, s[3])) return;
+    const bool take = spatial_centre_merge(t, fr.width, frame, r_W(r), r_m(r));
+    float4* wr = fr.spatial + (size_t)(1u - frame % 2u) * kPlanes * n;
+    spatial_write(
+        t, fr,
+        [&](Res& rs) {
+            if (take) take_sample(&rs, r.p);
+        },
+        wr, reuse, pp);
+    if (!reuse) {
+        fr.output[pix] = resolve_bgra8(t_color, 1.0f);
+        if (fr.radiance) fr.radiance[pix] = f4(t_color, 0.0f);
     }
 }
 
@@ -1659,6 +1746,12 @@ hipError_t launch_initial_sample(const SceneDev& sc, const FrameDev& fr, const U
 hipError_t launch_temporal(const FrameDev& fr, uint32_t frame, const PassParams& pp, const float4* gpos_hist,
                            hipStream_t st) {
     hipLaunchKernelGGL(k_temporal, dim3((fr.n + 255) / 256), dim3(256), 0, st, fr, frame, pp, gpos_hist);
+    return hipGetLastError();
+}
+hipError_t launch_temporal_spatial0(const FrameDev& fr, uint32_t frame, uint32_t reuse, const PassParams& pp,
+                                    const float4* gpos_hist, hipStream_t st) {
+    hipLaunchKernelGGL(k_temporal_spatial0, dim3((fr.n + 255) / 256), dim3(256), 0, st, fr, frame, pp, gpos_hist,
+                       reuse);
     return hipGetLastError();
 }
 hipError_t launch_nbr_build(const FrameDev& fr, uint32_t frame, hipStream_t st) {
