@@ -319,12 +319,13 @@ int xr_get_accel_info(const xr_ctx* ctx, xr_accel_info* info);
 
 /* Copies the acceleration structure to host memory for validation: node_count
  * 64-byte nodes of up to four children {float4 q; uint4 p; uint4 r; int4
- * link}.  q.xyz is the node's grid origin and the bits of q.w hold the biased
- * exponents e_x, e_y, e_z of the grid steps (step = 2^(e - 127)) in bytes 0-2
- * and the child count in byte 3; child k's box is origin + g * step per axis
- * with the planes g in byte k of p.x (lo.x), p.y (hi.x), p.z (lo.y), p.w
- * (hi.y), r.x (lo.z), r.y (hi.z).  link[k] >= 0 names a node, link[k] < 0 a
- * leaf ~(first * 16 + count - 1); slots past the child count are unused.
+ * link}.  q.xyz is the node's grid origin; the grid steps (powers of two) are
+ * the floats q.w (x), r.z (y) and r.w (z, both as float bits); child k's box
+ * is origin + g * step per axis with the planes g in byte k of p.x (lo.x),
+ * p.y (hi.x), p.z (lo.y), p.w (hi.y), r.x (lo.z), r.y (hi.z).  link[k] >= 0
+ * names a node, link[k] < 0 a leaf ~(first * 16 + count - 1).  The children
+ * fill slots 0 .. count-1; an unused slot holds the empty box lo = 255,
+ * hi = 0 on every axis and link -1.
  * Node 0 is the root and max_depth counts the nodes on the longest
  * root-to-leaf path.  Triangles are triangle_count 48-byte BVH-ordered
  * records {v0.xyz, canonical id; v1.xyz, instance; v2.xyz, primitive}. */

@@ -17,15 +17,19 @@ def decode_nodes(nodes: np.ndarray):
     """(lo, hi, links, counts) of the 64-byte four-child nodes: lo / hi as
     (n, 4, 3) float64 boxes of the children, decoded as origin + g * step."""
     origin = nodes[:, 0:3].astype(np.float64)
-    meta = nodes[:, 3].copy().view(np.uint32)
-    exps = np.stack([(meta >> (8 * a)) & 0xFF for a in range(3)], axis=1).astype(np.int64)
-    step = np.ldexp(1.0, exps - 127)  # (n, 3)
-    counts = (meta >> 24).astype(np.int64)
+    step = np.stack([nodes[:, 3], nodes[:, 10], nodes[:, 11]], axis=1).astype(np.float64)  # (n, 3)
+    mant, _ = np.frexp(step)
+    assert (mant == 0.5).all(), "grid steps are powers of two"
     planes = nodes[:, 4:10].copy().view(np.uint32)  # lox, hix, loy, hiy, loz, hiz
-    g = np.stack([(planes >> (8 * k)) & 0xFF for k in range(4)], axis=1).astype(np.float64)  # (n, 4, 6)
+    g = np.stack([(planes >> (8 * k)) & 0xFF for k in range(4)], axis=1).astype(np.int64)  # (n, 4, 6)
+    links = nodes[:, 12:16].copy().view(np.int32)
+    # used slots come first; an unused slot is the empty box 255 / 0 with link -1
+    empty = (g[:, :, 0::2] == 255).all(axis=2) & (g[:, :, 1::2] == 0).all(axis=2)
+    counts = (~empty).sum(axis=1)
+    assert (empty == (np.arange(4)[None, :] >= counts[:, None])).all(), "unused slots follow the children"
+    assert (links[empty] == -1).all(), "unused slots link the leaf of triangle 0"
     lo = origin[:, None, :] + g[:, :, 0::2] * step[:, None, :]
     hi = origin[:, None, :] + g[:, :, 1::2] * step[:, None, :]
-    links = nodes[:, 12:16].copy().view(np.int32)
     return lo, hi, links, counts
 
 

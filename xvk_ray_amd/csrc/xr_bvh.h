@@ -39,12 +39,19 @@ static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte line");
 // per plane on a per-node grid (Ylitie, Karras and Laine, "Efficient
 // Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017).
 //   q.xyz = grid origin (floats, the min corner of the union of the children)
-//   q.w   = biased exponents ex | ey << 8 | ez << 16 | child count << 24: the
-//           grid step on axis a is 2^(e_a - 127)
+//   q.w   = grid step on x, a power of two (float)
 //   p     = (lox, hix, loy, hiy): byte k holds child k's plane on that grid,
-//   r.xy  = (loz, hiz), r.zw unused
-//   link  = (link0 .. link3): >= 0 inner node, < 0 leaf ~(first*16 + count-1),
-//           unused slots (k >= child count) are never tested
+//   r     = (loz, hiz, step on y, step on z) (steps as float bits)
+//   link  = (link0 .. link3): >= 0 inner node, < 0 leaf ~(first*16 + count-1)
+// Slots past the child count hold the empty box lo = 255, hi = 0 on every
+// axis and link -1 (the leaf of BVH triangle 0).  Their near plane lies 255
+// steps beyond their far plane, which the roundings of the slab test (far
+// distance scaled by 1 + 2*gamma(3)) cannot undo for a ray whose origin is
+// within 2^29 grid steps of the node: 64 scene extents, a step being at
+// least 2^-23 of the extent since every box is padded.  A ray from farther
+// away that entered one would only test triangle 0, a real triangle, which
+// cannot change what it finds.  The traversal therefore needs no child
+// count.
 // A child's box is [origin + lo * step, origin + hi * step] per axis, rounded
 // outwards by the builder so it holds the child's padded box.  Traversal reads
 // four 16-byte words per node; its cost follows the bytes fetched per visit
@@ -75,6 +82,7 @@ struct Hit {
 struct RayPre {
     v3 org;
     v3 idir;
+    v3 oi;  // org * idir: a slab plane's distance is fma(plane, idir, -oi)
     int kx, ky, kz;
     float sx, sy, sz;
 };
@@ -90,6 +98,7 @@ XR_HD RayPre make_ray_pre(v3 org, v3 dir) {
     float dy = fabsf(dir.y) > 1e-20f ? dir.y : copysignf(1e-20f, dir.y);
     float dz = fabsf(dir.z) > 1e-20f ? dir.z : copysignf(1e-20f, dir.z);
     p.idir = mk3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    p.oi = mk3(org.x * p.idir.x, org.y * p.idir.y, org.z * p.idir.z);
     float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
     int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;

@@ -2264,15 +2264,15 @@ __global__ void k_wide_emit(const BvhNode* __restrict__ bn, const int2* __restri
             } else {
                 link[k] = s.link[k];
             }
-        } else {
-            link[k] = 0;
+        } else {  // empty box (lo 255 > hi 0 on every axis), leaf of triangle 0
+            for (int a = 0; a < 3; ++a) planes[2 * a] |= 255u << (8 * k);
+            link[k] = -1;
         }
     }
     WideNode w;
-    w.q = make_float4(nlo[0], nlo[1], nlo[2],
-                      __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16) | ((uint32_t)s.count << 24)));
+    w.q = make_float4(nlo[0], nlo[1], nlo[2], step[0]);
     w.p = make_uint4(planes[0], planes[1], planes[2], planes[3]);
-    w.r = make_uint4(planes[4], planes[5], 0u, 0u);
+    w.r = make_uint4(planes[4], planes[5], __float_as_uint(step[1]), __float_as_uint(step[2]));
     w.link = make_int4(link[0], link[1], link[2], link[3]);
     out[self_base + (uint32_t)i] = w;
     atomicMax(bound, (uint32_t)acc);

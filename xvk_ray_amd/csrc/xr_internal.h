@@ -169,10 +169,11 @@ __device__ __forceinline__ void order4(float& k0, int& v0, float& k1, int& v1, f
     order2(k1, v1, k2, v2);
 }
 
-// The four slab tests of a wide node (slots past the child count never hit).
-// On axis a the distance to grid plane g is (origin - o) / d + g * (step / d),
-// one fma from two terms formed once per node (step is a power of two, so
-// step / d is exact).  The roundings move a plane by a few ulp of the scene's
+// The four slab tests of a wide node (unused slots hold empty boxes).
+// On axis a the distance to grid plane g is o_a * (1/d) + g * (step / d),
+// one fma from two terms formed once per node, o_a * (1/d) itself one fma
+// from the ray's origin * (1/d) (step is a power of two, so step / d is
+// exact).  The roundings move a plane by a few ulp of the scene's
 // coordinates, far less than the padding the builders add to every box
 // (at least extent * 2^-16), so no child box that holds a hit is culled; the
 // far distance keeps the 1 + 2*gamma(3) scaling of the plain slab test.
@@ -182,11 +183,9 @@ __device__ __forceinline__ void wide_slabs(const WideNode* __restrict__ nd, cons
     const uint4 p = nd->p;
     const uint4 pz = nd->r;
     *lk = nd->link;
-    const uint32_t meta = __float_as_uint(q.w);
-    const uint32_t cnt = meta >> 24;
-    const float ax = (q.x - r.org.x) * r.idir.x, bx = __uint_as_float((meta & 0xFFu) << 23) * r.idir.x;
-    const float ay = (q.y - r.org.y) * r.idir.y, by = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * r.idir.y;
-    const float az = (q.z - r.org.z) * r.idir.z, bz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * r.idir.z;
+    const float ax = fmaf(q.x, r.idir.x, -r.oi.x), bx = q.w * r.idir.x;
+    const float ay = fmaf(q.y, r.idir.y, -r.oi.y), by = __uint_as_float(pz.z) * r.idir.y;
+    const float az = fmaf(q.z, r.idir.z, -r.oi.z), bz = __uint_as_float(pz.w) * r.idir.z;
     // A plane's distance fma(g, b, a) is monotonic in g, increasing where the
     // reciprocal direction is positive (it is never zero or NaN, see
     // make_ray_pre): the near plane of every child is its low plane on such
@@ -206,7 +205,7 @@ __device__ __forceinline__ void wide_slabs(const WideNode* __restrict__ nd, cons
         const float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
         float tf = fminf(fminf(t1x, t1y), t1z);
         tf = fminf(tf * 1.0000003576f, tb);
-        h[k] = tn <= tf && (uint32_t)k < cnt;
+        h[k] = tn <= tf;
         e[k] = tn;
     }
     // The links are needed only on some paths of the callers; without a use
