@@ -48,8 +48,8 @@ ALGO_BYTES = {"initial": 136, "temporal": 256, "spatial": 212}
 # 1 GiB device copy in the same kind of run: FETCH_SIZE x 2.0 (the copy reads
 # 524,316 KiB), WRITE_SIZE x 1.0 (1,048,576 KiB).  They are measured, not
 # computed here; the committed pmc summary under profiles/ holds the runs.
-PMC_FETCH_KIB = {"initial": 25314.2 + 23280.8, "temporal": 258418.1, "spatial": 622632.7}
-PMC_WRITE_KIB = {"initial": 380620.6 + 33164.3, "temporal": 340198.5, "spatial": 282938.0}
+PMC_FETCH_KIB = {"initial": 25312.8 + 23296.0, "temporal": 258479.1, "spatial": 657584.4}
+PMC_WRITE_KIB = {"initial": 380636.0 + 33106.8, "temporal": 340205.8, "spatial": 298874.2}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 # The traversal kernels are bound by VALU issue, not by HBM: wave64 VALU
@@ -57,7 +57,7 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 # command, averaged over the same frames; the initial pass adds its tail
 # launches, which its timing events bracket too) against the issue peak of 1,024
 # SIMDs, each retiring one wave64 VALU instruction per 4 cycles at 2.4 GHz.
-PMC_VALU_INSTS = {"initial": 462129976.5 + 83751663.1, "spatial": 668453687.0}
+PMC_VALU_INSTS = {"initial": 440243888.0 + 80125810.0, "spatial": 637689899.0}
 VALU_PEAK_GINST_S = 1024 * 2.4 / 4  # 614.4 G wave64 instructions per second
 
 
