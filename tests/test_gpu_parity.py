@@ -88,6 +88,42 @@ def test_trace_any_hit_matches(builder):
     assert np.array_equal(gpu["t"] >= 0, ref["t"] >= 0)
 
 
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_SAH])
+@pytest.mark.parametrize("dist", [3.0, 100.0, 1000.0])
+def test_trace_far_origins_match_brute_force(dist, builder):
+    """Rays from far outside the scene (dist scene extents from its centre)
+    aimed at random points of its box, tmax +inf: closest hits bit for bit
+    against brute force over every triangle, any-hit flags equal.  Far
+    origins stress the slab test's relative tolerance and the empty boxes of
+    unused child slots (entered only by rays from beyond ~64 extents, where
+    they lead to a real triangle's test)."""
+    scene = cornell_scene()
+    ctx = N.Context(scene, 8, 8, builder=builder)
+    orc = oracle_scene(scene)
+    rng = np.random.default_rng(int(dist) + 17)
+    n = 6000
+    centre = np.array([0.5, 0.5, 0.5])
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    org = (centre + dist * u).astype(np.float32)
+    aim = rng.uniform(0.0, 1.0, size=(n, 3))
+    d = aim - org.astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, N.RAY_DTYPE)
+    rays["origin"] = org
+    rays["direction"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    say(f"[trace-far] origins {dist} extents away, builder {builder}")
+    gpu = ctx.trace(rays)
+    ref = orc.trace32(rays, brute=True)
+    eq = bytes_equal_rows(gpu, ref)
+    assert eq.all(), describe_mismatch("closest hit, far origins", gpu, ref, eq)
+    assert (gpu["t"] > 0).mean() > 0.5
+    any_gpu = ctx.trace(rays, any_hit=True)
+    assert np.array_equal(any_gpu["t"] >= 0, ref["t"] >= 0)
+
+
 def _mixed_scale_scene() -> N.Scene:
     """A 2000-unit ground quad, 2000 millimetre triangles in a unit box far from
     the origin, 200 triangles flat in x (zero extent on one axis) in the same
