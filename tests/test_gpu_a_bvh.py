@@ -210,3 +210,47 @@ def test_gpu_sah_builds_the_host_sah_tree(which):
     assert g_info.max_depth == h_info.max_depth and len(gpu[0]) == len(host[0])
     assert _leaf_sets(*gpu) == _leaf_sets(*host)
     say(f"[bvh] build ms GPU {g_info.build_ms:.1f} host {h_info.build_ms:.1f}")
+
+
+def _degenerate_scene() -> N.Scene:
+    """3,000 copies of one triangle (one centroid: the builders' median
+    split), 1,000 zero-area triangles (points and segments), 1,000 triangles
+    flat in y on one plane, 3,000 random small triangles in the unit box."""
+    rng = np.random.default_rng(21)
+    same = np.repeat(np.array([[[0.4, 0.4, 0.4], [0.6, 0.4, 0.4], [0.4, 0.6, 0.45]]]), 3000, axis=0)
+    p = rng.uniform(0.0, 1.0, size=(1000, 1, 3))
+    seg = p + rng.uniform(0.0, 0.01, size=(1000, 1, 3)) * np.array([[0.0], [1.0], [1.0]])[None, :, :]
+    seg[:500] = p[:500]  # the first half are points
+    flat = rng.uniform(0.0, 1.0, size=(1000, 3, 3))
+    flat[:, :, 1] = 0.25
+    small = rng.uniform(0.0, 1.0, size=(3000, 1, 3)) + rng.uniform(-0.02, 0.02, size=(3000, 3, 3))
+    t = np.concatenate([same, seg, flat, small]).astype(np.float32)
+    v = np.zeros(3 * len(t), N.VERTEX_DTYPE)
+    v["pos"] = t.reshape(-1, 3)
+    v["normal"] = [0, 1, 0]
+    m = np.zeros(1, N.MATERIAL_DTYPE)
+    m["base_color"] = [0.5, 0.5, 0.5, 1]
+    m["base_color_texture_index"] = -1
+    return N.Scene.from_arrays(v, np.arange(len(v), dtype=np.uint32), m, np.array([0, 0], np.uint32))
+
+
+@pytest.mark.parametrize("builder", [N.XR_BUILDER_LBVH, N.XR_BUILDER_PLOC, N.XR_BUILDER_SAH, N.XR_BUILDER_SAH_HOST])
+def test_bvh_degenerate_triangles(builder):
+    """Coincident centroids, zero-area and axis-flat triangles: a valid tree
+    (every triangle in one leaf, every box holding what is below it) and
+    closest / any hits bit for bit against brute force over all triangles."""
+    from tests.helpers import oracle_scene, random_rays, bytes_equal_rows, describe_mismatch
+    scene = _degenerate_scene()
+    say(f"[bvh] degenerate scene ({scene.triangle_count} tris), builder {builder}")
+    ctx = N.Context(scene, 8, 8, builder=builder)
+    nodes, tris = ctx.read_accel()
+    check_bvh(nodes, tris, scene.triangle_count)
+    orc = oracle_scene(scene)
+    rays = random_rays(8000, -0.2, 1.2, seed=23, tmin=0.0, tmax=np.inf)
+    gpu = ctx.trace(rays)
+    ref = orc.trace32(rays, brute=True)
+    eq = bytes_equal_rows(gpu, ref)
+    assert eq.all(), describe_mismatch("closest hit, degenerate scene", gpu, ref, eq)
+    assert (gpu["t"] > 0).mean() > 0.1
+    any_gpu = ctx.trace(rays, any_hit=True)
+    assert np.array_equal(any_gpu["t"] >= 0, ref["t"] >= 0)

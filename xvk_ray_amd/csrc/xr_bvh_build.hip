@@ -386,9 +386,15 @@ __global__ void k_ploc_leaves(const BvhTri* __restrict__ ctris, const uint32_t* 
 }
 
 // Nearest neighbour of each cluster among the 2 * kPlocRadius around it, by
-// the area of the merged box; ties go to the lower position, which orders
-// candidates exactly as the pairs (area, lower, higher) are ordered, so the
-// globally best pair is always mutual and every iteration merges.
+// the area of the merged box; among equal areas the position's buddy
+// (positions 2k and 2k + 1 are buddies) comes first, then the lower position.
+// That orders candidates exactly as the pairs (area, not buddies, lower,
+// higher) are ordered, a strict order that does not depend on which side
+// looks, so the globally best pair is always mutual and every iteration
+// merges.  The buddy rule matters for runs of identical boxes (coincident
+// triangles): with ties going to the lowest position alone, only the run's
+// first pair was mutual, one pair merged per iteration and the run became a
+// chain as deep as it was long; with it the run pairs up completely.
 __global__ void k_ploc_nn(const int* __restrict__ clusters, int C, const Box* __restrict__ boxes,
                           int* __restrict__ nn) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,7 +407,7 @@ __global__ void k_ploc_nn(const int* __restrict__ clusters, int C, const Box* __
     for (int j = lo; j <= hi; ++j) {
         if (j == i) continue;
         const float d = box_area(box_union(bi, boxes[clusters[j]]));
-        if (d < best_d) {
+        if (d < best_d || (d == best_d && (j ^ 1) == i)) {
             best_d = d;
             best = j;
         }
