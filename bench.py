@@ -473,9 +473,10 @@ def run_c5(args) -> dict:
     """3840x2160 ReSTIR GI, max depth 6, on the large terrain (2048x2048-vertex
     fBm heightfield + 4,096 icospheres of 1,280 tris + 256 quad lights, seed
     1), camera on the seeded 128-frame path; plain ReSTIR GI vs ReSTIR GI with
-    the MLP radiance cache in FP8 E4M3 weights.  RMSE of each method's
-    radiance against an FP64 converged image of the same camera at frames 1,
-    16 and 128."""
+    the MLP radiance cache in FP8 E4M3 weights, with the workload's 5 spatial
+    neighbours, with 1 and without spatial neighbours (the cache form's
+    equal-error setting at 1080p).  RMSE of each method's radiance against an FP64
+    converged image of the same camera at frames 1, 16 and 128."""
     import numpy as np
 
     from xvk_ray_amd import native as N
@@ -489,27 +490,28 @@ def run_c5(args) -> dict:
     log(f"c5 scene {scene.triangle_count} tris ({time.perf_counter() - t:.1f} s)")
     saved = {}
 
-    def run(cache: bool) -> dict:
+    def run(cache: bool, neighbours: int = PARAMS["spatial_samples"]) -> dict:
+        key = (cache, neighbours)
         app = N.App(scene, camera, W5, H5, mode=N.XR_MODE_RESTIR, spp=1, bounces=depth, fov=FOV, sky=False,
                     reuse=True)
         ctx = app.context()
         ctx.build_accel(builder_id(args))
         info = ctx.accel_info()
         flags = N.XR_FLAG_RADIANCE | ((N.XR_FLAG_MLP_CACHE | N.XR_FLAG_MLP_FP8) if cache else 0)
-        p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, **PARAMS)
+        p = N.default_params(visibility=N.XR_VIS_TRACE, flags=flags, **dict(PARAMS, spatial_samples=neighbours))
         ctx.reset_counters()
         ctx.reset_pass_times()
         t0 = time.perf_counter()
         for f in range(1, frames + 1):
             app.draw_frame(p)
             if f in checkpoints:
-                saved[(cache, f)] = (ctx.read(N.XR_BUF_RADIANCE)[:, :3].astype(np.float64), app.last_ubo())
-                log(f"c5 {'cache' if cache else 'plain'} frame {f}")
+                saved[(key, f)] = (ctx.read(N.XR_BUF_RADIANCE)[:, :3].astype(np.float64), app.last_ubo())
+                log(f"c5 {'cache' if cache else 'plain'} K={neighbours} frame {f}")
         ctx.sync()
         wall = time.perf_counter() - t0
         pass_ms, nfr = ctx.pass_time_totals()
         ext, shadow, valid, queries, train = ctx.counters_all()
-        out = {"ms_per_frame_gpu": round(sum(pass_ms) / nfr, 4),
+        out = {"spatial_samples": neighbours, "ms_per_frame_gpu": round(sum(pass_ms) / nfr, 4),
                "pass_ms": {k: round(v / nfr, 4) for k, v in zip(("initial", "temporal", "spatial", "cache"), pass_ms)},
                "rays_per_frame": (ext + shadow) // nfr, "mrays_per_s": round((ext + shadow) / wall / 1e6, 1),
                "bvh": {"nodes": info.node_count, "depth": info.max_depth, "build_ms": round(info.build_ms, 1)}}
@@ -520,13 +522,13 @@ def run_c5(args) -> dict:
         app.close()
         return out
 
-    res_plain = run(False)
-    res_cache = run(True)
+    k5 = PARAMS["spatial_samples"]
+    runs = {(False, k5): run(False), (True, k5): run(True), (True, 1): run(True, 1), (True, 0): run(True, 0)}
     # converged images of the checkpoint cameras
     ctx = N.Context(scene, W5, H5, builder=builder_id(args))
-    rmse = {False: {}, True: {}}
+    rmse = {key: {} for key in runs}
     for f in checkpoints:
-        ubo = saved[(False, f)][1]
+        ubo = saved[((False, k5), f)][1]
         ctx.converge_reset()
         done = 0
         while done < args.c5_conv_spp:
@@ -536,11 +538,13 @@ def run_c5(args) -> dict:
             done += k
         log(f"c5 converged image for frame {f}: {done} spp")
         conv = ctx.converged_mean()
-        for cache in (False, True):
-            rad = saved[(cache, f)][0]
-            rmse[cache][f] = round(float(np.sqrt(np.mean((rad - conv) ** 2))), 6)
+        for key in runs:
+            rad = saved[(key, f)][0]
+            rmse[key][f] = round(float(np.sqrt(np.mean((rad - conv) ** 2))), 6)
     ctx.close()
-    res_plain["rmse"], res_cache["rmse"] = rmse[False], rmse[True]
+    for key, r in runs.items():
+        r["rmse"] = rmse[key]
+    res_plain, res_cache, res_cache1, res_cache0 = runs[(False, k5)], runs[(True, k5)], runs[(True, 1)], runs[(True, 0)]
     ms = res_cache["ms_per_frame_gpu"]
     res = base_result(args, "ms/frame & Mrays/s @3840x2160 ReSTIR GI + FP8 MLP radiance cache",
                       res_cache["mrays_per_s"], "Mrays/s", ms, "f32+fp8",
@@ -548,8 +552,12 @@ def run_c5(args) -> dict:
                        "triangles": scene.triangle_count, "max_depth": depth, "frames": frames,
                        "camera": "seeded 128-frame orbit", "parallelism": "single_gpu"})
     res["steps"], res["warmup"] = frames, 0
-    res.update({"plain_restir": res_plain, "mlp_cache_fp8": res_cache, "converged_spp": args.c5_conv_spp,
-                "time_ratio_cache_vs_plain": round(ms / res_plain["ms_per_frame_gpu"], 4)})
+    res.update({"plain_restir": res_plain, "mlp_cache_fp8": res_cache,
+                "mlp_cache_fp8_fewer_neighbours": [res_cache1, res_cache0], "converged_spp": args.c5_conv_spp,
+                "time_ratio_cache_vs_plain": round(ms / res_plain["ms_per_frame_gpu"], 4),
+                "time_ratio_cache_fewer_neighbours_vs_plain": [round(r["ms_per_frame_gpu"]
+                                                                     / res_plain["ms_per_frame_gpu"], 4)
+                                                               for r in (res_cache1, res_cache0)]})
     return res
 
 
