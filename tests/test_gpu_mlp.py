@@ -395,3 +395,44 @@ def test_cache_gain_over_plain_restir_and_untrained():
     assert e_trained < 0.98 * e_plain  # measured 0.0711 against 0.0750
     assert e_trained < 0.7 * e_untrained
     assert r_trained < 0.8 * r_plain and r_untrained < r_plain
+
+
+@pytest.mark.parametrize("size", [(37, 11), (1, 1)])
+def test_cache_frame_odd_size_fewer_candidates_than_target(size):
+    """A cache frame on an image smaller than a tile and the training choice's
+    4,096-pixel block, with a training target above the frame's candidate
+    count: every candidate trains, termination flags and traced fields are
+    bit-exact against the oracle, training pixels carry the network's
+    radiance after the step."""
+    W, H = size
+    scene = N.Scene.cornell(1)
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    verts, _, mats, _ = scene.arrays()
+    lo = verts["pos"].min(0).astype(np.float32)
+    inv = (np.float32(1.0) / (verts["pos"].max(0) - lo).astype(np.float32)).astype(np.float32)
+    T = 65536
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE, mlp_train_pixels=T, mlp_spread=0.01)
+    for frame in (1, 2):
+        _cache_frame(ctx, W, H, frame, params)
+    ctx.reset_counters()
+    ubo = _cache_frame(ctx, W, H, 3, params)
+    counts = ctx.counters_all()
+    w_after = ctx.mlp_params()
+    gpu = ctx.read(N.XR_BUF_SAMPLES)
+    ref = orc.initial_sample(N.ubo_to_bytes(ubo), W, H, cache=(T, 0.01))["samples"]
+    kind = ref["pad2"]
+    assert np.array_equal(gpu["pad2"], kind) and np.array_equal(gpu["pad1"], ref["pad1"])
+    train = kind == 2
+    assert not (kind == 1).any()  # T exceeds the candidates: none stops into the cache
+    assert counts[3] == 0 and counts[4] == train.sum()
+    g2, r2 = gpu.copy(), ref.copy()
+    g2["l_out"][train] = 0
+    r2["l_out"][train] = 0
+    eq = bytes_equal_rows(g2, r2)
+    assert eq.all(), describe_mismatch("cache-frame samples", g2, r2, eq)
+    if train.any():
+        idx = np.nonzero(train)[0]
+        pred = np.maximum(B.mlp_forward(w_after, _features(ref, mats, lo, inv, idx), mode=1), 0.0)
+        got = gpu["l_out"][idx].astype(np.float64)
+        assert np.abs(got - pred).max() <= 1e-2 * max(1.0, np.abs(pred).max())

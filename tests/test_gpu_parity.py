@@ -502,3 +502,71 @@ def test_rays_counted():
     t = ctx.pass_times()
     assert len(t) == 4 and all(x >= 0 for x in t)  # initial, temporal, spatial, cache
     assert t[3] < 0.05  # no cache work without the cache flag (two adjacent events)
+
+
+# ---------------------------------------------------------------------------
+# Image sizes that fill no tile: single pixels, single rows and columns, sizes
+# that are no multiple of the 32 x 8 tiles
+# ---------------------------------------------------------------------------
+ODD_SIZES = [(1, 1), (1, 9), (9, 1), (33, 9), (37, 11)]
+
+
+@pytest.mark.parametrize("size", ODD_SIZES)
+def test_restir_odd_sizes(size):
+    """ReSTIR GI pass by pass at the configuration's constants (traced
+    visibility, 5 neighbours within 30 px, which reach past every edge of
+    these images) against the oracle, two frames."""
+    W, H = size
+    scene = cornell_scene()
+    eyes = [CORNELL_EYE, (0.51, 0.5, -1.2)]
+    params = N.default_params(temporal_m_cap=20, spatial_samples=5, spatial_radius=30.0, visibility=N.XR_VIS_TRACE)
+    _frames(scene, W, H, 60.0, eyes, CORNELL_TARGET, 4, params, _strict)
+
+
+@pytest.mark.parametrize("size", ODD_SIZES)
+def test_restir_whole_frames_odd_sizes(size):
+    """Whole frames in one render call (the benchmark's form) equal the pass-by-
+    pass frames: same samples, reservoirs and output bytes after three frames."""
+    W, H = size
+    scene = cornell_scene()
+    eyes = [CORNELL_EYE, (0.51, 0.5, -1.2), (0.52, 0.5, -1.2)]
+    views = [look_view(e, CORNELL_TARGET) for e in eyes]
+    params = N.default_params(temporal_m_cap=20, spatial_samples=5, spatial_radius=30.0, visibility=N.XR_VIS_TRACE)
+    whole = N.Context(scene, W, H)
+    split = N.Context(scene, W, H)
+    for k, view in enumerate(views):
+        ubo = N.make_ubo(view, views[k - 1] if k else view, 60.0, W, H, bounces=4, reuse=1, frame=k + 1)
+        whole.render(ubo, N.XR_MODE_RESTIR, params)
+        for passes in (N.XR_PASS_INITIAL, N.XR_PASS_TEMPORAL, N.XR_PASS_SPATIAL):
+            p = N.XrRestirParams.from_buffer_copy(params)
+            p.passes = passes
+            split.render(ubo, N.XR_MODE_RESTIR, p)
+    whole.sync()
+    split.sync()
+    for which in (N.XR_BUF_SAMPLES, N.XR_BUF_TEMPORAL, N.XR_BUF_SPATIAL, N.XR_BUF_OUTPUT):
+        a, b = whole.read(which), split.read(which)
+        assert a.tobytes() == b.tobytes(), f"buffer {which} differs at {W}x{H}"
+
+
+@pytest.mark.parametrize("size", [(1, 1), (9, 1), (33, 9)])
+@pytest.mark.parametrize("max_depth", [3, 0])
+def test_plain_odd_sizes(size, max_depth):
+    """Plain accumulation over two frames, one kernel (depth 3) or the head plus
+    per-bounce launches (unbounded depth), against the oracle."""
+    W, H = size
+    scene = cornell_scene()
+    ctx = N.Context(scene, W, H)
+    orc = oracle_scene(scene)
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    accum = np.zeros((W * H, 4), np.float32)
+    out = np.zeros(W * H, np.uint32)
+    params = N.default_params(plain_max_depth=max_depth)
+    for frame in (1, 2):
+        ubo = N.make_ubo(view, view, 60.0, W, H, spp=1, total=frame, bounces=10, sky=0, reuse=0, frame=frame)
+        ctx.render(ubo, N.XR_MODE_PLAIN, params)
+        ctx.sync()
+        orc.plain(N.ubo_to_bytes(ubo), W, H, max_depth, accum, out)
+        ga = ctx.read(N.XR_BUF_ACCUMULATION)
+        eq = bytes_equal_rows(ga, accum)
+        assert eq.all(), describe_mismatch(f"accumulation {W}x{H}", ga, accum, eq)
+        assert _max_lsb_diff(ctx.read(N.XR_BUF_OUTPUT), out) <= 1
