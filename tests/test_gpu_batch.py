@@ -166,3 +166,38 @@ def test_graph_recaptured_after_new_textures_and_rebuilt_tree():
         for v in range(len(sizes)):
             for b in BUFFERS:
                 assert g.read(v, b).tobytes() == e.read(v, b).tobytes(), (f, v, b)
+
+
+TINY_SIZES = [(1, 1), (33, 9), (9, 1), (1, 17), (64, 48)]
+TINY_EYES = EYES + [((0.3, 0.5, 0.4), (0.6, 0.1, 0.6)), ((0.7, 0.45, 0.8), (0.4, 0.1, 0.3))]
+
+
+@pytest.mark.parametrize("cfg", [0, 3, 6, 9, 14])
+def test_batch_views_smaller_than_a_tile(scene, cfg):
+    """Views of one pixel, one row, one column and sizes that fill no tile, in
+    grid and persistent launch forms with the graph: each view equals the same
+    view rendered alone, over three frames."""
+    frames = 3
+    ctxs = [N.Context(scene, w, h) for w, h in TINY_SIZES]
+    p = N.default_params(**PARAMS)
+    ctx = N.Context(scene, 8, 8)
+    batch = N.Batch(ctx, TINY_SIZES)
+    batch.set_config(cfg)
+    say(f"[batch] tiny views, cfg {cfg} {N.Batch.config_info(cfg)} persistent {N.Batch.config_persistent(cfg)}")
+    prev = [None] * len(TINY_SIZES)
+    for f in range(1, frames + 1):
+        row = []
+        for v, ((eye, tgt), (w, h)) in enumerate(zip(TINY_EYES, TINY_SIZES)):
+            view = look_view(tuple(np.array(eye) + 0.004 * f), tgt)
+            row.append(N.make_ubo(view, prev[v] if prev[v] is not None else view, 45.0, w, h, bounces=4, frame=f))
+            prev[v] = view
+        batch.render(row, p)
+        for c, ubo in zip(ctxs, row):
+            c.render(ubo, N.XR_MODE_RESTIR, p)
+        ctx.sync()
+        for v, c in enumerate(ctxs):
+            c.sync()
+            for b in BUFFERS:
+                assert batch.read(v, b).tobytes() == c.read(b).tobytes(), f"cfg {cfg} frame {f} view {v} buffer {b}"
+    batch.close()
+    ctx.close()
