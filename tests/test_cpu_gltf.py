@@ -336,3 +336,63 @@ def test_struct_layout_matches_reference():
     v, _, m, _ = N.Scene.gltf(CB2).arrays()
     assert v.dtype.itemsize == 36 and m.dtype.itemsize == 32
     assert struct.calcsize("<3f3f2fi") == 36
+
+
+def test_loader_survives_damaged_files(tmp_path):
+    """cb2's glTF with JSON fields replaced by out-of-range, wrong-typed or
+    huge values (counts, offsets, component types, indices), fields deleted,
+    text damaged, and its buffer truncated: every load returns a scene or an
+    error, never crashes."""
+    import copy
+    import random
+    import shutil
+    src = Path(__file__).parent / "golden" / "gltf" / "cb2"
+    base = json.loads((src / "scene.gltf").read_text())
+    blob = (src / "scene.bin").read_bytes()
+    vals = [0, 1, -1, 3, 4, 5, 12, 255, 256, 65535, 2 ** 31 - 1, 2 ** 31, 2 ** 32, 2 ** 62, 1e300, -1e300, 1.5,
+            5121, 5123, 5125, 5126, "VEC3", "SCALAR", None, [], {}, "x", True]
+
+    def paths(o, pre=()):
+        if isinstance(o, dict):
+            for k, v in o.items():
+                yield pre + (k,)
+                yield from paths(v, pre + (k,))
+        elif isinstance(o, list):
+            for i, v in enumerate(o):
+                yield pre + (i,)
+                yield from paths(v, pre + (i,))
+
+    every = list(paths(base))
+    rng = random.Random(41)
+    loaded = 0
+    for it in range(400):
+        g = copy.deepcopy(base)
+        for _ in range(rng.randint(1, 3)):
+            p = rng.choice(every)
+            try:
+                o = g
+                for k in p[:-1]:
+                    o = o[k]
+                if rng.random() < 0.15:
+                    del o[p[-1]]
+                else:
+                    o[p[-1]] = rng.choice(vals)
+            except (KeyError, IndexError, TypeError):
+                pass
+        text = json.dumps(g)
+        if rng.random() < 0.1:
+            t = list(text)
+            for _ in range(rng.randint(1, 4)):
+                t[rng.randrange(len(t))] = rng.choice('{}[]",:0123456789.e- ')
+            text = "".join(t)
+        d = tmp_path / f"s{it % 4}"
+        d.mkdir(exist_ok=True)
+        (d / "scene.gltf").write_text(text)
+        (d / "scene.bin").write_bytes(blob if rng.random() > 0.1 else blob[:rng.randrange(len(blob))])
+        try:
+            N.Scene.gltf(d / "scene.gltf")
+            loaded += 1
+        except N.XrError:
+            pass
+    shutil.rmtree(tmp_path, ignore_errors=True)
+    assert 0 < loaded < 400

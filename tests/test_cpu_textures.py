@@ -211,3 +211,64 @@ def test_oracle_bilinear_closed_forms():
     want = ((1 - a) * (1 - b) * t[j0, i0] + a * (1 - b) * t[j0, i0 + 1] + (1 - a) * b * t[j0 + 1, i0]
             + a * b * t[j0 + 1, i0 + 1])
     assert np.allclose(o.texture_sample(0, u, v), want, atol=2e-6)
+
+
+def _chunks(data: bytes):
+    out, p = [], 8
+    while p + 12 <= len(data):
+        n = int.from_bytes(data[p:p + 4], "big")
+        out.append([data[p + 4:p + 8], bytearray(data[p + 8:p + 8 + n])])
+        p += 12 + n
+    return out
+
+
+def _assemble(chunks) -> bytes:
+    import zlib
+    b = bytearray(b"\x89PNG\r\n\x1a\n")
+    for t, d in chunks:
+        b += len(d).to_bytes(4, "big") + t + d + (zlib.crc32(t + bytes(d)) & 0xFFFFFFFF).to_bytes(4, "big")
+    return bytes(b)
+
+
+def test_decoder_survives_damaged_chunks():
+    """Damaged PNGs whose chunk CRCs are recomputed (so the damage reaches the
+    header checks, the inflater and the row filters): header bytes, deflate
+    bytes, truncated and extended chunks, and valid zlib streams of damaged
+    rows.  Every call returns 0 with an image or an error code, never
+    crashes; a decoded image has the size its header states."""
+    import zlib
+    rng = np.random.default_rng(31)
+    seeds = [_png(PIL.fromarray(_noise(13, 7, 4, 1), "RGBA"), compress_level=9),
+             _png(PIL.fromarray(_noise(5, 9, 3, 2), "RGB"), compress_level=1),
+             _png(PIL.fromarray(_noise(16, 3, 4, 3, smooth=True), "RGBA").convert("P"), compress_level=6),
+             _png(PIL.fromarray(_noise(4, 4, 1, 4)[:, :, 0], "L").convert("I;16"))]
+    decoded = 0
+    for it in range(600):
+        ch = _chunks(seeds[it % len(seeds)])
+        for _ in range(int(rng.integers(1, 4))):
+            c = ch[int(rng.integers(0, len(ch)))]
+            d = c[1]
+            op = int(rng.integers(0, 4))
+            if op == 0 and len(d):
+                d[int(rng.integers(0, len(d)))] = int(rng.integers(0, 256))
+            elif op == 1 and len(d):
+                del d[int(rng.integers(0, len(d))):]
+            elif op == 2:
+                p = int(rng.integers(0, len(d) + 1))
+                d[p:p] = bytes(rng.integers(0, 256, size=int(rng.integers(1, 8)), dtype=np.uint8))
+            elif c[0] == b"IDAT":
+                try:
+                    raw = bytearray(zlib.decompress(bytes(d)))
+                except zlib.error:
+                    continue
+                if raw:
+                    raw[int(rng.integers(0, len(raw)))] = int(rng.integers(0, 256))
+                c[1] = bytearray(zlib.compress(bytes(raw[:int(rng.integers(0, len(raw) + 1))])))
+        data = _assemble(ch)
+        rc, img = _decode(data)
+        if rc == 0:
+            decoded += 1
+            w = int.from_bytes(data[16:20], "big")
+            h = int.from_bytes(data[20:24], "big")
+            assert img.shape == (h, w, 4)
+    assert 0 < decoded < 600
