@@ -181,6 +181,8 @@ void xr_default_params(xr_restir_params* p);
 
 /* Copies the scene to device 0 and allocates the frame buffers for a
  * width x height target (InitReSTIRGITracer, ApplicationReSTIRGI.cpp:304-374).
+ * Vertex coordinates must be finite and within +-2^40 (XR_ERR_UNSUPPORTED
+ * otherwise: the traversal's slab arithmetic stays finite in that range).
  * On failure *out is NULL and xr_global_last_error() says why. */
 int xr_create(const xr_scene_desc* scene, uint32_t width, uint32_t height, xr_ctx** out);
 

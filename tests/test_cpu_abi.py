@@ -76,6 +76,19 @@ def test_create_rejects_bad_arguments_before_device_work():
     assert rc == N.XR_ERR_INVALID_ARGUMENT and "material" in msg
     rc, _, msg = _create(v, i, m, np.array([3, 0], np.uint32))
     assert rc == N.XR_ERR_INVALID_ARGUMENT
+    # coordinates must be finite and within +-2^40 (the limit itself passes
+    # validation and fails later only for want of a device)
+    for bad in (np.nan, np.inf, -np.inf, 2.0**40 * (1 + 2.0**-23), -2.0**41):
+        v3 = v.copy()
+        v3["pos"][1, 2] = bad
+        rc, h, msg = _create(v3, i, m, o)
+        assert rc == N.XR_ERR_UNSUPPORTED and "2^40" in msg and not h.value
+    v4 = v.copy()
+    v4["pos"][1] = [2.0**40, -(2.0**40), 0.0]
+    rc, h, msg = _create(v4, i, m, o)
+    assert rc != N.XR_ERR_UNSUPPORTED
+    if h.value:
+        N.lib().xr_destroy(h)
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")

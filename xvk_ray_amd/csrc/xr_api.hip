@@ -492,9 +492,14 @@ int xr_create(const xr_scene_desc* s, uint32_t width, uint32_t height, xr_ctx** 
         first[i] = io / 3;
     }
     first[s->instance_count] = s->index_count / 3;
+    // Coordinates within +-2^40 keep every wide node's grid step (at most
+    // 2^34) times a reciprocal direction (at most 1e20) finite.
     float box_lo[3] = {3e38f, 3e38f, 3e38f}, box_hi[3] = {-3e38f, -3e38f, -3e38f};
     for (uint32_t v = 0; v < s->vertex_count; ++v)
         for (int a = 0; a < 3; ++a) {
+            if (!(fabsf(s->vertices[v].pos[a]) <= 0x1p40f))
+                return set_err(nullptr, XR_ERR_UNSUPPORTED, "vertex %u coordinate %g is not finite or beyond +-2^40", v,
+                               (double)s->vertices[v].pos[a]);
             box_lo[a] = fminf(box_lo[a], s->vertices[v].pos[a]);
             box_hi[a] = fmaxf(box_hi[a], s->vertices[v].pos[a]);
         }
