@@ -1,0 +1,132 @@
+"""Context isolation and resizing on the GPU: several xr_ctx in one process
+keep their own frame buffers, history, acceleration structures and networks,
+and a resized context renders exactly what a fresh context of that size
+renders (InitReSTIRGITracer re-creates the frame resources for a new extent,
+ApplicationReSTIRGI.cpp:304-374)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from xvk_ray_amd import native as N
+
+from helpers import CORNELL_EYE, CORNELL_TARGET, cornell_scene, look_view, say, small_terrain
+
+pytestmark = pytest.mark.gpu
+
+BUFS = (N.XR_BUF_SAMPLES, N.XR_BUF_TEMPORAL, N.XR_BUF_SPATIAL, N.XR_BUF_OUTPUT)
+
+
+def _restir_params():
+    return N.default_params(temporal_m_cap=20, spatial_samples=5, spatial_radius=30.0, visibility=N.XR_VIS_TRACE)
+
+
+def _cornell_ubos(W, H, frames=3):
+    eyes = [(CORNELL_EYE[0] + 0.01 * k, CORNELL_EYE[1], CORNELL_EYE[2]) for k in range(frames)]
+    views = [look_view(e, CORNELL_TARGET) for e in eyes]
+    return [N.make_ubo(v, views[k - 1] if k else v, 60.0, W, H, bounces=4, frame=k + 1) for k, v in enumerate(views)]
+
+
+def _terrain_ubos(W, H, frames=3):
+    views = [look_view((0.2 + 0.01 * k, 0.4, 0.2), (0.5, 0.0, 0.5)) for k in range(frames)]
+    return [N.make_ubo(v, views[k - 1] if k else v, 45.0, W, H, bounces=4, sky=1, frame=k + 1)
+            for k, v in enumerate(views)]
+
+
+def _plain_ubos(W, H, frames=2):
+    view = look_view(CORNELL_EYE, CORNELL_TARGET)
+    return [N.make_ubo(view, view, 60.0, W, H, spp=1, total=f, bounces=6, reuse=0, frame=f) for f in range(1, frames + 1)]
+
+
+def _snapshot(ctx, bufs=BUFS):
+    ctx.sync()
+    return {b: ctx.read(b).tobytes() for b in bufs}
+
+
+def test_two_contexts_interleaved_equal_separate_runs():
+    """ReSTIR GI frames of two contexts (Cornell box with the LBVH, terrain with
+    the GPU SAH tree) rendered alternately equal each context rendered alone."""
+    Wa, Ha, Wb, Hb = 48, 32, 40, 24
+    ca, cb = cornell_scene(), small_terrain()
+    ua, ub = _cornell_ubos(Wa, Ha), _terrain_ubos(Wb, Hb)
+    p = _restir_params()
+    say(f"two contexts {Wa}x{Ha} cornell / {Wb}x{Hb} terrain, interleaved")
+    a = N.Context(ca, Wa, Ha)
+    b = N.Context(cb, Wb, Hb, builder=N.XR_BUILDER_SAH)
+    for k in range(len(ua)):
+        a.render(ua[k], N.XR_MODE_RESTIR, p)
+        b.render(ub[k], N.XR_MODE_RESTIR, p)
+    inter_a, inter_b = _snapshot(a), _snapshot(b)
+    a.close()
+    b.close()
+    say("the same contexts alone")
+    a1 = N.Context(ca, Wa, Ha)
+    for u in ua:
+        a1.render(u, N.XR_MODE_RESTIR, p)
+    alone_a = _snapshot(a1)
+    a1.close()
+    b1 = N.Context(cb, Wb, Hb, builder=N.XR_BUILDER_SAH)
+    for u in ub:
+        b1.render(u, N.XR_MODE_RESTIR, p)
+    alone_b = _snapshot(b1)
+    b1.close()
+    for which in BUFS:
+        assert inter_a[which] == alone_a[which], f"cornell buffer {which} differs when interleaved"
+        assert inter_b[which] == alone_b[which], f"terrain buffer {which} differs when interleaved"
+    # the two scenes render different images (the comparison is not vacuous)
+    assert inter_a[N.XR_BUF_OUTPUT] != inter_b[N.XR_BUF_OUTPUT][: len(inter_a[N.XR_BUF_OUTPUT])]
+
+
+def test_networks_are_per_context():
+    """Each context holds its own radiance-cache network: different seeds give
+    different predictions, a reset with the first seed restores them exactly,
+    and training one context leaves the other's predictions unchanged."""
+    scene = cornell_scene()
+    a = N.Context(scene, 8, 8)
+    b = N.Context(scene, 8, 8)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(256, N.XR_MLP_INPUTS)).astype(np.float32)
+    y = rng.uniform(0, 1, size=(256, 3)).astype(np.float32)
+    a.mlp_reset(1)
+    b.mlp_reset(2)
+    ya, yb = a.mlp_eval(x), b.mlp_eval(x)
+    assert not np.array_equal(ya, yb)
+    b.mlp_reset(1)
+    assert np.array_equal(b.mlp_eval(x), ya)
+    for _ in range(5):
+        b.mlp_train_rows(x, y, lr=1e-2)
+    assert not np.array_equal(b.mlp_eval(x), ya)
+    assert np.array_equal(a.mlp_eval(x), ya), "training context b changed context a's network"
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("sizes", [((48, 32), (33, 9), (64, 48)), ((17, 5), (80, 40), (1, 1))])
+def test_resized_context_renders_like_a_fresh_one(sizes):
+    """A context resized (down, then up) renders, from an empty history, the
+    same ReSTIR GI and plain frames as a context created at that size."""
+    scene = cornell_scene()
+    p = _restir_params()
+    pp = N.default_params(plain_max_depth=0)
+    ctx = N.Context(scene, *sizes[0])
+    for u in _cornell_ubos(*sizes[0], frames=2):
+        ctx.render(u, N.XR_MODE_RESTIR, p)
+    ctx.sync()
+    for W, H in sizes[1:]:
+        say(f"resize {ctx.width}x{ctx.height} -> {W}x{H}")
+        ctx.resize(W, H)
+        fresh = N.Context(scene, W, H)
+        for u in _cornell_ubos(W, H):
+            ctx.render(u, N.XR_MODE_RESTIR, p)
+            fresh.render(u, N.XR_MODE_RESTIR, p)
+        got, want = _snapshot(ctx), _snapshot(fresh)
+        for which in BUFS:
+            assert got[which] == want[which], f"buffer {which} differs after resize to {W}x{H}"
+        for u in _plain_ubos(W, H):
+            ctx.render(u, N.XR_MODE_PLAIN, pp)
+            fresh.render(u, N.XR_MODE_PLAIN, pp)
+        got = _snapshot(ctx, (N.XR_BUF_ACCUMULATION, N.XR_BUF_OUTPUT))
+        want = _snapshot(fresh, (N.XR_BUF_ACCUMULATION, N.XR_BUF_OUTPUT))
+        assert got == want, f"plain frames differ after resize to {W}x{H}"
+        fresh.close()
+    ctx.close()
