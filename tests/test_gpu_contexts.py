@@ -77,6 +77,57 @@ def test_two_contexts_interleaved_equal_separate_runs():
     assert inter_a[N.XR_BUF_OUTPUT] != inter_b[N.XR_BUF_OUTPUT][: len(inter_a[N.XR_BUF_OUTPUT])]
 
 
+def _render_alone(scene, W, H, ubos, builder, p):
+    ctx = N.Context(scene, W, H, builder=builder)
+    for u in ubos:
+        ctx.render(u, N.XR_MODE_RESTIR, p)
+    snap = _snapshot(ctx)
+    ctx.close()
+    return snap
+
+
+def test_contexts_on_two_host_threads_equal_separate_runs():
+    """Two host threads, each with its own context and stream, render at the
+    same time (the library's calls release the interpreter lock): each thread's
+    frames equal the same context rendered alone."""
+    import threading
+
+    p = _restir_params()
+    jobs = [
+        (cornell_scene(), 96, 64, _cornell_ubos(96, 64, frames=4), N.XR_BUILDER_LBVH),
+        (small_terrain(), 80, 48, _terrain_ubos(80, 48, frames=4), N.XR_BUILDER_SAH),
+    ]
+    want = [_render_alone(sc, W, H, u, b, p) for sc, W, H, u, b in jobs]
+    for rep in range(3):
+        got = [None, None]
+        errors = []
+        start = threading.Barrier(2)
+
+        def work(k):
+            try:
+                sc, W, H, u, b = jobs[k]
+                ctx = N.Context(sc, W, H, builder=b)
+                start.wait()
+                for x in u:
+                    ctx.render(x, N.XR_MODE_RESTIR, p)
+                got[k] = _snapshot(ctx)
+                ctx.close()
+            except Exception as e:  # reported below, in the main thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(k,), daemon=True) for k in range(2)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in threads), "a render thread did not finish"
+        assert not errors, errors
+        say(f"two threads, repetition {rep}: done")
+        for k in range(2):
+            for which in BUFS:
+                assert got[k][which] == want[k][which], f"job {k} buffer {which} differs on two threads"
+
+
 def test_networks_are_per_context():
     """Each context holds its own radiance-cache network: different seeds give
     different predictions, a reset with the first seed restores them exactly,
