@@ -630,7 +630,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "the frame library runs on one GPU only; --gpus must be 1",
                           "requested_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "c1":

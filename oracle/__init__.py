@@ -1,7 +1,7 @@
 """CPU oracle: a C++ restatement of the reference shaders (oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and
-by bench.py's cpu_baseline leg, always as the checker or the CPU baseline,
+by bench.py's cpu_baseline leg, always as the reference results or the CPU baseline,
 never as the thing measured or shipped.  The product package (xvk_ray_amd)
 does not import it.
 """

@@ -1,5 +1,5 @@
 """CPU restatement of the reference's glTF scene loading -- test
-infrastructure only (the checker of xrh_scene_gltf; nothing in the product
+infrastructure only (the reference results for xrh_scene_gltf; nothing in the product
 imports it).
 
 Follows AglTFModel::loadglTFFile / loadMaterials / loadNode

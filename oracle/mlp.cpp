@@ -1,6 +1,6 @@
 // CPU oracle of the MLP radiance cache (test infrastructure only: the tests,
-// the smoke check and the CPU-baseline leg of the benchmark call it as the
-// checker; the product never links it).
+// the smoke check and the CPU-baseline leg of the benchmark call it for
+// reference results; the product never links it).
 //
 // The cache has no counterpart in the reference.  What is restated here is
 // the published method the HIP kernels implement:

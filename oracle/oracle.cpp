@@ -1,6 +1,6 @@
 // oracle.cpp -- CPU restatement of the reference's path-tracing / ReSTIR GI
 // shaders.  TEST INFRASTRUCTURE ONLY: tests/, the smoke check and the bench's
-// CPU-baseline leg call it as a checker; the HIP product never links or calls
+// CPU-baseline leg call it for reference results; the HIP product never links or calls
 // it.
 //
 // The reference (0Setsuna0/XVK_Ray) cannot be built here: it is Vulkan +
@@ -35,7 +35,7 @@
 //   * normalize(v) = v * (1/sqrt(dot(v,v))), rsqrt(x) = 1/sqrt(x).
 //   * TraceRay: the nearest triangle by the watertight test of Woop, Benthin
 //     and Wald (JCGT 2013) in FP32, ties to the smaller canonical triangle id
-//     (FP32 mirror).  A separate FP64 closest-hit search is the exact judge.
+//     (FP32 mirror).  A separate FP64 closest-hit search is the exact reference.
 //   * float -> UNORM8: round(clamp(x) * 255).
 #include <math.h>
 #include <stdint.h>
@@ -492,7 +492,7 @@ bool tri_fp32(const RayPre& r, const float3* p, float tmin, float tmax, float* t
     *v_out = W / det;
     return true;
 }
-// Watertight test in FP64 (same algorithm, all double): the exact judge.
+// Watertight test in FP64 (same algorithm, all double): the exact reference.
 bool tri_fp64(const double* o, const double* d, const float3* p, double tmin, double tmax, double* t_out) {
     double ad[3] = {fabs(d[0]), fabs(d[1]), fabs(d[2])};
     int kz = (ad[0] > ad[1]) ? ((ad[0] > ad[2]) ? 0 : 2) : ((ad[1] > ad[2]) ? 1 : 2);
@@ -558,7 +558,7 @@ Hit trace32(const Scene& s, float3 org, float3 dir, float tmin, float tmax, bool
     return h;
 }
 
-// Closest hit in FP64 (exact judge): returns id and t, or id = ~0.
+// Closest hit in FP64 (exact reference): returns id and t, or id = ~0.
 void trace64(const Scene& s, const double* o, const double* d, double tmin, double tmax, uint32_t* id_out,
              double* t_out) {
     double best = tmax;

@@ -1,7 +1,7 @@
 """CPU tests of the oracle's pass restatements (no GPU).
 
 Pins each pass by closed forms, special cases and invariants:
-  * TraceRay: the oracle BVH equals brute force; FP32 equals the FP64 judge
+  * TraceRay: the oracle BVH equals brute force; FP32 equals the FP64 reference
     on >= 99.99% of rays; rays from inside a closed icosphere never escape
     (watertightness across shared edges and vertices).
   * initial sample: hand-computable two-surface scene (floor under a light).
@@ -26,7 +26,7 @@ def test_oracle_bvh_equals_brute_force():
     assert bytes_equal_rows(orc.trace32(rays), orc.trace32(rays, brute=True)).all()
 
 
-def test_fp32_trace_matches_fp64_judge():
+def test_fp32_trace_matches_fp64_reference():
     scene = N.Scene.terrain(4, 128, 16, 8)
     orc = oracle_scene(scene)
     rays = random_rays(20000, 0.0, 0.5, seed=4)

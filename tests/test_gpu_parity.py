@@ -50,8 +50,8 @@ def test_trace_closest_bit_exact(which, builder):
 
 
 @pytest.mark.parametrize("builder", BUILDERS)
-def test_trace_matches_fp64_judge(builder):
-    """North-star criterion: >= 99.99% identical closest-hit ids vs an FP64
+def test_trace_matches_fp64_reference(builder):
+    """Criterion: >= 99.99% identical closest-hit ids vs an FP64
     search; every mismatch is a distance tie within 1e-6 * t."""
     scene = small_terrain(grid=256, spheres=64, lights=16)
     ctx = N.Context(scene, 8, 8, builder=builder)
