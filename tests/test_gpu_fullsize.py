@@ -145,3 +145,38 @@ def test_deep_bvh_trace_matches_oracle(big):
     ref = orc.trace32(rays)
     eq = bytes_equal_rows(gpu, ref)
     assert eq.all(), describe_mismatch("deep closest hit", gpu, ref, eq)
+
+
+@pytest.mark.parametrize("fp8", [False, True])
+def test_fullsize_cache_long_run_stays_finite(big, fp8):
+    """128 cache frames along the camera path at full size (training on 65,536
+    samples per frame, no spatial neighbours, the equal-error setting of the
+    benchmark's study; FP8 weights in the second case): the online-trained
+    network's parameters, the radiance behind the image and the reservoirs
+    stay finite, and the training step keeps choosing its full batch."""
+    scene, cam, orc = big
+    ctx = N.Context(scene, W, H, builder=N.XR_BUILDER_SAH)
+    ctx.mlp_reset(1)
+    flags = N.XR_FLAG_MLP_CACHE | N.XR_FLAG_RADIANCE | (N.XR_FLAG_MLP_FP8 if fp8 else 0)
+    params = N.default_params(**dict(PARAMS, spatial_samples=0), flags=flags, mlp_spread=0.01)
+    frames = 128
+    views = [cam.view(k) for k in range(frames)]
+    w0 = ctx.mlp_params().copy()
+    ctx.reset_counters()
+    for k in range(frames):
+        ubo = N.make_ubo(views[k], views[k - 1] if k else views[k], 45.0, W, H, bounces=4, frame=k + 1)
+        ctx.render(ubo, N.XR_MODE_RESTIR, params)
+        if k + 1 in (32, frames):
+            ctx.sync()
+            w = ctx.mlp_params()
+            assert np.isfinite(w).all(), f"network parameters at frame {k + 1}"
+            rad = ctx.read(N.XR_BUF_RADIANCE)
+            assert np.isfinite(rad).all(), f"radiance at frame {k + 1}"
+            _check_reservoir_invariants(ctx, params, k + 1)
+            say(f"[fullsize] cache frame {k + 1} ({'FP8' if fp8 else 'BF16'}): max |param| "
+                f"{np.abs(w).max():.3g}, mean radiance {rad[:, :3].mean():.4f}")
+    assert not np.array_equal(ctx.mlp_params(), w0), "the network trained"
+    queries, trained = ctx.counters_all()[3:]
+    say(f"[fullsize] {queries / frames:.0f} cache queries and {trained / frames:.0f} training samples per frame")
+    assert trained == frames * params.mlp_train_pixels
+    assert queries > 0
