@@ -145,6 +145,60 @@ def test_c3_batched_views_with_cache_match_oracle(terrain):
     ctx.close()
 
 
+@pytest.mark.parametrize("cache", [False, True])
+def test_c3_long_history(terrain, cache):
+    """The batched workload over 64 frames of each view's camera path, replayed
+    as graphs under the autotuned configuration: every view's reservoirs stay
+    finite with M within the caps and its image opaque; without the cache the
+    64th frame of every view still equals the oracle on a pixel subset, with
+    the cache the shared network stays finite and each step trains on the
+    union of the views' samples (at most 65,536 per view, all of them for the
+    views with more candidates than that)."""
+    scene, orc = terrain
+    frames = 64
+    ctx = N.Context(scene, 8, 8)
+    cams = [N.Camera.orbit(2 + v, 8, frames) for v in range(len(C3_SIZES))]
+    views = [[c.view(f) for f in range(frames)] for c in cams]
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE if cache else 0, **PARAMS)
+    ubos = [[N.make_ubo(views[v][f], views[v][max(f - 1, 0)], 45.0, w, h, bounces=4, frame=f + 1)
+             for v, (w, h) in enumerate(C3_SIZES)] for f in range(frames)]
+    batch = N.Batch(ctx, C3_SIZES)
+    batch.set_graph(True)
+    if not cache:
+        batch.autotune(ubos[0], params, frames=2)
+        batch.reset()
+    else:
+        ctx.mlp_reset(1)
+    w0 = ctx.mlp_params().copy() if cache else None
+    batch.reset_counters()
+    for f in range(frames):
+        batch.render(ubos[f], params)
+    ctx.sync()
+    say(f"[configs] C3 {'+ cache ' if cache else ''}after {frames} frames (config {batch.config})")
+    rng = np.random.default_rng(31)
+    for v, (w, h) in enumerate(C3_SIZES):
+        for which, cap in ((N.XR_BUF_TEMPORAL, params.temporal_m_cap), (N.XR_BUF_SPATIAL, params.spatial_m_cap)):
+            r = batch.read(v, which)
+            for fld in ("w_sum", "w"):
+                assert np.isfinite(r[fld]).all() and (r[fld] >= 0).all(), f"view {v} buffer {which} {fld}"
+            assert (r["m"] <= cap).all(), f"view {v} buffer {which} M above {cap}"
+        assert ((batch.read(v, N.XR_BUF_OUTPUT) >> 24) == 0xFF).all(), f"view {v} output alpha"
+        if not cache:
+            px = np.sort(rng.choice(w * h, size=min(2000, w * h), replace=False)).astype(np.uint32)
+            valid = check_view_frame(orc, lambda which, v=v: batch.read(v, which), w, h, ubos[-1][v], frames, px,
+                                     params)
+            assert valid > 0.3
+    if cache:
+        wl = ctx.mlp_params()
+        assert np.isfinite(wl).all() and not np.array_equal(wl, w0)
+        trained = batch.counters()[4]
+        big = sum(1 for w, h in C3_SIZES if w * h >= 1280 * 720)
+        say(f"[configs] C3 + cache: {trained / frames:.0f} training samples per step")
+        assert frames * big * params.mlp_train_pixels <= trained <= frames * len(C3_SIZES) * params.mlp_train_pixels
+    batch.close()
+    ctx.close()
+
+
 def _features(samples, mats, lo, inv, idx):
     from tests.test_gpu_mlp import _features as f
     return f(samples, mats, lo, inv, idx)
