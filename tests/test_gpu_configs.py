@@ -199,6 +199,41 @@ def test_c3_long_history(terrain, cache):
     ctx.close()
 
 
+def test_c5_long_run_stays_finite():
+    """The 4K workload's whole 128-frame camera path (13.6 M triangles, depth 6,
+    FP8 radiance cache with the five spatial neighbours): after the last frame
+    the reservoirs keep their invariants, the image is opaque, the radiance
+    and the network are finite, and every frame trained on 65,536 samples."""
+    say("[configs] C5 scene (long run)")
+    scene = N.Scene.terrain(1, 2048, 4096, 256)
+    W, H = 3840, 2160
+    ctx = N.Context(scene, W, H, builder=N.XR_BUILDER_SAH)
+    cam = N.Camera.orbit(2, 8, 128)
+    params = N.default_params(flags=N.XR_FLAG_MLP_CACHE | N.XR_FLAG_MLP_FP8 | N.XR_FLAG_RADIANCE, **PARAMS)
+    ctx.mlp_reset(1)
+    ctx.reset_counters()
+    frames = 128
+    prev = cam.view(0)
+    for k in range(frames):
+        view = cam.view(k)
+        ctx.render(N.make_ubo(view, prev, 45.0, W, H, bounces=6, frame=k + 1), N.XR_MODE_RESTIR, params)
+        prev = view
+    ctx.sync()
+    trained = ctx.counters_all()[4]
+    say(f"[configs] C5 after {frames} frames: {trained / frames:.0f} training samples per frame")
+    assert trained == frames * params.mlp_train_pixels
+    assert np.isfinite(ctx.mlp_params()).all()
+    assert np.isfinite(ctx.read(N.XR_BUF_RADIANCE)).all()
+    for which, cap in ((N.XR_BUF_TEMPORAL, params.temporal_m_cap), (N.XR_BUF_SPATIAL, params.spatial_m_cap)):
+        r = ctx.read(which)
+        for fld in ("w_sum", "w"):
+            assert np.isfinite(r[fld]).all() and (r[fld] >= 0).all(), f"buffer {which} {fld}"
+        assert (r["m"] <= cap).all(), f"buffer {which} M above {cap}"
+        del r
+    assert ((ctx.read(N.XR_BUF_OUTPUT) >> 24) == 0xFF).all()
+    ctx.close()
+
+
 def _features(samples, mats, lo, inv, idx):
     from tests.test_gpu_mlp import _features as f
     return f(samples, mats, lo, inv, idx)
