@@ -237,3 +237,37 @@ def test_contexts_and_batches_release_device_memory():
         assert free[-1] - live >= 64 << 20
     drift = free[2] - free[-1]
     assert abs(drift) <= 2 << 20, f"device free memory moved by {drift / 2**20:.2f} MiB over nine cycles"
+
+
+def test_injected_failures_leave_no_trace(monkeypatch):
+    """Failures forced from the host (XR_FAULT_INJECT, read on every call): a
+    context whose device allocation fails is refused with an out-of-memory
+    error and gives back everything it had allocated; a frame whose launch
+    fails is refused before any work is queued, so the context's later
+    frames equal those of a context that never saw the failure."""
+    scene = cornell_scene()
+    W, H = 64, 48
+    p = _restir_params()
+    ubos = _cornell_ubos(W, H, frames=3)
+    N.Context(scene, W, H).close()  # the runtime's one-time set-up happens outside the measure
+    free0 = _device_free_bytes()
+    monkeypatch.setenv("XR_FAULT_INJECT", "alloc")
+    with pytest.raises(N.XrError) as e:
+        N.Context(scene, W, H)
+    assert e.value.code == N.XR_ERR_OUT_OF_MEMORY
+    monkeypatch.delenv("XR_FAULT_INJECT")
+    assert _device_free_bytes() == free0, "a refused context kept device memory"
+    ctx = N.Context(scene, W, H)
+    ctx.render(ubos[0], N.XR_MODE_RESTIR, p)
+    monkeypatch.setenv("XR_FAULT_INJECT", "launch")
+    with pytest.raises(N.XrError) as e:
+        ctx.render(ubos[1], N.XR_MODE_RESTIR, p)
+    assert e.value.code == N.XR_ERR_DEVICE and "injected" in str(e.value)
+    monkeypatch.delenv("XR_FAULT_INJECT")
+    for u in ubos[1:]:
+        ctx.render(u, N.XR_MODE_RESTIR, p)
+    got = _snapshot(ctx)
+    ctx.close()
+    want = _render_alone(scene, W, H, ubos, N.XR_BUILDER_LBVH, p)
+    for which in BUFS:
+        assert got[which] == want[which], f"buffer {which} differs after a refused frame"
